@@ -1,0 +1,144 @@
+/*
+ * mtcheck.h -- C ABI of the MI355X tree deduplicator (libmtcheck_amd.so).
+ *
+ * Drop-in boundary for MTCheck's tree approach: each entry point names the
+ * reference interface it replaces (paths relative to the TauferLab/MTCheck tree).
+ * Plain pointers and sizes only; device pointers are HIP device memory on the
+ * handle's device; `stream` is a hipStream_t (NULL = the null stream).
+ * Every function returns an mtc_status and never aborts.
+ */
+#ifndef MTCHECK_H
+#define MTCHECK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference returns void and prints on impossible states
+ * (src/tree_approach.cpp:556,843); this ABI reports them instead. */
+typedef enum mtc_status {
+  MTC_OK = 0,
+  MTC_ERR_INVALID_ARG = 1,
+  MTC_ERR_UNSUPPORTED_APPROACH = 2,
+  MTC_ERR_OUT_OF_MEMORY = 3,
+  MTC_ERR_TABLE_FULL = 4,
+  MTC_ERR_CORRUPT_DIFF = 5,
+  MTC_ERR_HIP = 6,
+  MTC_ERR_STATE = 7,
+  MTC_ERR_BUFFER_TOO_SMALL = 8
+} mtc_status;
+
+/* Approaches: the DedupMode enum, include/utils.hpp:39-49.  Every tree mode is
+ * served by the tree deduplicator, as src/dedup_files.cpp:54-63 dispatches them;
+ * Full, Basic and List return MTC_ERR_UNSUPPORTED_APPROACH. */
+typedef enum mtc_approach {
+  MTC_APPROACH_UNKNOWN = 0,
+  MTC_APPROACH_FULL = 1,
+  MTC_APPROACH_BASIC = 2,
+  MTC_APPROACH_LIST = 3,
+  MTC_APPROACH_TREE = 4,
+  MTC_APPROACH_TREE_LOW_OFFSET_REF = 5,
+  MTC_APPROACH_TREE_LOW_OFFSET = 6,
+  MTC_APPROACH_TREE_LOW_ROOT_REF = 7,
+  MTC_APPROACH_TREE_LOW_ROOT = 8
+} mtc_approach;
+
+/* Chunk hash: include/hash_functions.hpp:16-20 (chosen at run time here). */
+typedef enum mtc_hash_kind { MTC_HASH_MURMUR3 = 0, MTC_HASH_MD5 = 1 } mtc_hash_kind;
+
+/* Diff header, include/utils.hpp:29-37: 32 bytes, little-endian. */
+typedef struct mtc_header {
+  uint32_t ref_id;
+  uint32_t chkpt_id;
+  uint64_t datalen;
+  uint32_t chunk_size;
+  uint32_t num_first_ocur;
+  uint32_t num_prior_chkpts;
+  uint32_t num_shift_dupl;
+} mtc_header;
+
+typedef struct mtc_dedup mtc_dedup;
+
+/* Per-checkpoint statistics: include/tree_approach.hpp:53-59 (num_first_ocur,
+ * num_shift_dupl) and the timers of src/tree_approach.cpp:1640,1657,1672,1688. */
+typedef struct mtc_stats {
+  uint32_t current_id;
+  uint32_t baseline_id;
+  uint32_t num_first_ocur;
+  uint32_t num_shift_dupl;
+  uint32_t num_prior_chkpts;
+  uint32_t num_chunks;
+  uint64_t data_len;
+  uint64_t diff_size;
+  uint64_t data_chunks;     /* chunks stored in the data section */
+  uint64_t table_entries;
+  uint64_t table_capacity;
+  float ms_setup, ms_dedup, ms_collect, ms_copy; /* timers[0..3] */
+  uint64_t changed_chunks; /* leaves that are not fixed duplicates */
+  float ms_leaf;           /* the leaf (hash + label + insert) kernel alone */
+} mtc_stats;
+
+/* TreeDeduplicator(uint32_t bytes_per_chunk), src/tree_approach.cpp:5-9, plus the
+ * approach selection of src/utils.cpp:15-36.  `device` is a HIP device ordinal. */
+int mtc_create(mtc_dedup** out, uint32_t chunk_size, int approach, int hash_kind, int device);
+
+/* ~TreeDeduplicator, src/tree_approach.cpp (destructor). */
+int mtc_destroy(mtc_dedup* h);
+
+/* TreeDeduplicator::checkpoint(header_t&, uint8_t*, size_t, HostMirror&, bool),
+ * src/tree_approach.cpp:1577-1689.  Deduplicates `len` bytes at device pointer
+ * `dev_data`; the diff stays in library-owned device memory (mtc_diff_device,
+ * mtc_copy_diff).  make_baseline (or the first checkpoint) resets the tree and the
+ * table and sets ref_id to the current id (:1601-1606,1644-1650).  advance_id != 0
+ * increments the checkpoint id afterwards, as the other three overloads do
+ * (:1723,1747,1775).  An incremental checkpoint must keep the previous length.
+ * Returns once the diff is complete on the device (the stream is synchronised). */
+int mtc_checkpoint(mtc_dedup* h, const void* dev_data, uint64_t len, int make_baseline, int advance_id,
+                   void* stream, mtc_header* header_out, uint64_t* diff_size_out);
+
+/* The device diff of the last checkpoint (valid until the next checkpoint). */
+int mtc_diff_device(mtc_dedup* h, const void** dev_diff, uint64_t* size);
+
+/* Device-to-host copy of the last diff, src/tree_approach.cpp:1678-1684.
+ * `capacity` must be at least the diff size. */
+int mtc_copy_diff(mtc_dedup* h, void* host_dst, uint64_t capacity, void* stream);
+
+/* num_first_ocur() / num_shift_dupl() (include/tree_approach.hpp:53-59) and timers. */
+int mtc_get_stats(const mtc_dedup* h, mtc_stats* out);
+
+/* TreeDeduplicator::restart(View, vector<HostMirror>&, logname, chkpt_id),
+ * src/tree_approach.cpp:1786-1797 (-> restart_chkpt :712-1128), and the raw-pointer
+ * overload :1809-1817.  diffs[i] is the diff of checkpoint i (entries below the
+ * chain's ref_id may be NULL); diffs_on_device selects device or host pointers.
+ * The restored buffer (header.datalen bytes) is written to device pointer dev_out. */
+int mtc_restart(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t chkpt_id,
+                int diffs_on_device, void* dev_out, uint64_t out_len, int device, void* stream);
+
+/* calculate_digest_host, include/hash_functions.hpp:46-82: digest of a whole host
+ * buffer with the selected hash (restart verification). */
+int mtc_digest_host(const void* host_data, uint64_t len, int hash_kind, uint8_t out[16]);
+
+/* Leaf digests only: hash each chunk of a device buffer into dev_digests
+ * (16 bytes per chunk), the hashing half of the leaf kernel
+ * (src/tree_approach.cpp:309-323). */
+int mtc_hash_chunks(const void* dev_data, uint64_t len, uint32_t chunk_size, int hash_kind, void* dev_digests,
+                    void* stream);
+
+/* Debug views, as the serial reference leaves them (include/reference_impl.hpp
+ * :121): the full tree (16 * (2n-1) bytes, zero where no digest was computed) and
+ * every node's label.  Enable before a checkpoint with mtc_set_debug(h, 1). */
+int mtc_set_debug(mtc_dedup* h, int on);
+int mtc_copy_tree(const mtc_dedup* h, void* host_dst, uint64_t capacity);
+int mtc_copy_labels(const mtc_dedup* h, void* host_dst, uint64_t capacity);
+
+/* Message for the last error on a handle (or the thread's last handle-less call). */
+const char* mtc_last_error(const mtc_dedup* h);
+const char* mtc_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,137 @@
+// internal.hpp -- host-side state shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mtcheck.h"
+#include "device.hpp"
+
+namespace mtc {
+
+struct HipError {
+  hipError_t err;
+  const char* what;
+};
+
+#define MTC_HIP(call)                                 \
+  do {                                                \
+    hipError_t e_ = (call);                           \
+    if (e_ != hipSuccess) throw ::mtc::HipError{e_, #call}; \
+  } while (0)
+
+struct Status {
+  int code;
+  std::string msg;
+};
+
+// Device allocation owned by the library, grown on demand.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  uint64_t count = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), count(o.count) {
+    o.p = nullptr;
+    o.count = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      count = o.count;
+      o.p = nullptr;
+      o.count = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void ensure(uint64_t want) {
+    if (want <= count) return;
+    release();
+    MTC_HIP(hipMalloc((void**)&p, (want ? want : 1) * sizeof(T)));
+    count = want;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    count = 0;
+  }
+};
+
+// One fused sweep stage: parents at depths [d_out, d_in-1] over tiles of 2^(d_in-d_out)
+// positions at depth d_in.
+struct Stage {
+  uint32_t d_in, d_out;
+};
+
+// Device-resident counters and results of one checkpoint (one 256-byte block).
+struct Ctl {
+  uint32_t err;
+  uint32_t F, S, P;               // roots and prior-checkpoint table entries
+  unsigned long long claims;      // new table entries created
+  unsigned long long data_chunks; // chunks in the data section
+  unsigned long long diff_size;   // bytes of the diff
+  unsigned long long changed;     // leaves that are not fixed duplicates
+  uint32_t pad[52];
+};
+
+struct Workspace {
+  DevBuf<uint32_t> u32a, u32b, u32c;  // scan partials
+};
+
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t count, uint32_t* total, Workspace& ws,
+                        hipStream_t st);
+
+}  // namespace mtc
+
+// Opaque handle behind include/mtcheck.h.
+struct mtc_dedup {
+  int device = 0;
+  uint32_t chunk_size = 0;
+  int hash_kind = 0;
+  int approach = 0;
+  uint32_t current_id = 0;
+  uint32_t baseline_id = 0;
+  bool has_state = false;
+  uint64_t data_len = 0;
+  uint32_t n = 0;
+  uint64_t table_cap = 0;   // power of two; slot index cap is the overflow slot
+  uint64_t table_size = 0;  // entries (host mirror, refreshed after every checkpoint)
+  int debug = 0;
+
+  mtc::DevBuf<mtc::Digest> leaf_dg;   // persistent leaf digests (the tree's leaves)
+  mtc::DevBuf<uint8_t> labels;        // leaf labels of the current checkpoint
+  mtc::DevBuf<uint32_t> leaf_slot;    // table slot of each non-fixed leaf
+  mtc::DevBuf<mtc::Slot> table;
+  mtc::DevBuf<mtc::Slot> table_tmp;   // rehash target
+  mtc::DevBuf<uint8_t> fr_lab;        // stage frontiers (label, digest, slot)
+  mtc::DevBuf<mtc::Digest> fr_dg;
+  mtc::DevBuf<uint32_t> fr_slot;
+  mtc::DevBuf<uint64_t> staging;      // emitted roots, indexed by node-id range
+  mtc::DevBuf<uint32_t> seg_cnt_f, seg_cnt_s, seg_off_f, seg_off_s, seg_base;
+  mtc::DevBuf<uint32_t> first_roots;  // F, ascending node id
+  mtc::DevBuf<uint64_t> shift_tmp;    // (slot << 32 | node), ascending node id
+  mtc::DevBuf<uint32_t> sort_key[2];
+  mtc::DevBuf<uint64_t> sort_val[2];  // (prev << 32 | node)
+  mtc::DevBuf<uint32_t> sort_hist;
+  mtc::DevBuf<uint32_t> src_hist;     // per source checkpoint
+  mtc::DevBuf<uint32_t> ptable;       // P (chkpt, count) pairs
+  mtc::DevBuf<uint32_t> region_off;   // F region offsets (chunks), then total
+  mtc::DevBuf<uint8_t> diff;          // device diff of the last checkpoint
+  mtc::DevBuf<mtc::Digest> dbg_tree;  // debug: full tree, as the serial reference leaves it
+  mtc::DevBuf<uint8_t> dbg_labels;    // debug: all node labels
+  mtc::Ctl* ctl_d = nullptr;
+  mtc::Ctl* ctl_h = nullptr;          // pinned
+  mtc::Workspace ws;
+  std::vector<mtc::Stage> stages;
+  std::vector<uint32_t> seg_base_h;   // first segment of each depth
+  mtc_header last_header{};
+  uint64_t last_diff_size = 0;
+  float last_ms[5] = {0, 0, 0, 0, 0};  // setup, dedup, collect, copy, leaf kernel
+  std::string last_error;
+};

@@ -1,0 +1,391 @@
+// restart.hip -- rebuild checkpoint k from the diff chain ref_id..k.
+//
+// Semantics: src/tree_approach.cpp:712-1128.  A chunk takes its bytes from the
+// newest diff of the chain that covers its leaf, either from that diff's own
+// first-occurrence data or, for a shifted-duplicate entry (node, prev) with source
+// checkpoint s, from the first-occurrence region of diff s that contains prev.
+// Instead of the reference's per-diff pass over every chunk, the chain is resolved
+// once into a per-chunk (diff, data slot) cover array, then one scatter copy runs.
+#include <algorithm>
+#include <cstring>
+
+#include "internal.hpp"
+
+namespace mtc {
+
+constexpr uint32_t kRThreads = 256;
+constexpr uint64_t kSlotBits = 40;
+constexpr uint64_t kSlotMask = (1ull << kSlotBits) - 1;
+constexpr uint64_t kNoCover = ~0ull;
+
+struct ChainDiff {
+  const uint8_t* base;     // device
+  uint64_t size;
+  mtc_header h;
+  uint64_t off_first, off_table, off_shift, off_data;
+  uint64_t data_chunks;    // whole chunks the data section can hold
+  uint64_t region_base;    // first entry of this diff in the flat region-offset array
+};
+
+static uint32_t rgrid(uint64_t items, uint32_t cap = 1u << 20) {
+  uint64_t g = (items + kRThreads - 1) / kRThreads;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
+}
+
+__device__ __forceinline__ uint32_t ld32(const uint8_t* p) {
+  return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+
+// Region sizes of one diff's first-occurrence roots.
+__global__ void k_r_sizes(const uint8_t* first, uint32_t F, Geo g, uint32_t* sizes, uint32_t* err) {
+  const uint32_t i = blockIdx.x * kRThreads + threadIdx.x;
+  if (i >= F) return;
+  const uint32_t node = ld32(first + 4ull * i);
+  if ((uint64_t)node >= 2ull * g.n - 1) {
+    atomicOr(err, ERR_CORRUPT);
+    sizes[i] = 0;
+    return;
+  }
+  sizes[i] = leaf_count(g, node);
+}
+
+// Hash set (diff position, root node) -> region index, for prev lookups.
+__device__ __forceinline__ uint64_t rs_home(uint64_t key, uint64_t mask) {
+  key ^= key >> 33;
+  key *= 0xff51afd7ed558ccdULL;
+  key ^= key >> 33;
+  return key & mask;
+}
+
+__global__ void k_r_set_insert(const uint8_t* first, uint32_t F, uint32_t pos, uint64_t* keys, uint32_t* vals,
+                               uint64_t mask, uint32_t* err) {
+  const uint32_t i = blockIdx.x * kRThreads + threadIdx.x;
+  if (i >= F) return;
+  const uint64_t key = ((uint64_t)pos << 32) | ld32(first + 4ull * i);
+  uint64_t s = rs_home(key, mask);
+  for (uint64_t probes = 0; probes <= mask; probes++) {
+    const uint64_t old = atomicCAS((unsigned long long*)&keys[s], (unsigned long long)kNoCover, (unsigned long long)key);
+    if (old == kNoCover) {
+      vals[s] = i;
+      return;
+    }
+    if (old == key) {  // the same root twice in one diff
+      atomicOr(err, ERR_CORRUPT);
+      return;
+    }
+    s = (s + 1) & mask;
+  }
+  atomicOr(err, ERR_TABLE_FULL);
+}
+
+__device__ __forceinline__ uint32_t rs_find(const uint64_t* keys, const uint32_t* vals, uint64_t mask, uint64_t key) {
+  uint64_t s = rs_home(key, mask);
+  for (uint64_t probes = 0; probes <= mask; probes++) {
+    const uint64_t k = keys[s];
+    if (k == key) return vals[s];
+    if (k == kNoCover) return ~0u;
+    s = (s + 1) & mask;
+  }
+  return ~0u;
+}
+
+__global__ void k_r_fill64(uint64_t* p, uint64_t count, uint64_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kRThreads + threadIdx.x; i < count; i += (uint64_t)gridDim.x * kRThreads)
+    p[i] = v;
+}
+
+__global__ void k_r_fill32(uint32_t* p, uint64_t count, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kRThreads + threadIdx.x; i < count; i += (uint64_t)gridDim.x * kRThreads)
+    p[i] = v;
+}
+
+// Chunks of one diff's first-occurrence regions: cover[chunk] = (pos, data slot).
+__global__ void k_r_cover_first(const uint8_t* first, uint32_t F, const uint32_t* region_off, uint64_t total, Geo g,
+                                uint32_t pos, uint64_t* cover) {
+  for (uint64_t j = (uint64_t)blockIdx.x * kRThreads + threadIdx.x; j < total; j += (uint64_t)gridDim.x * kRThreads) {
+    uint32_t lo = 0, hi = F;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (region_off[mid] <= j) lo = mid;
+      else hi = mid;
+    }
+    uint32_t st[2], ct[2];
+    const int r = leaf_runs(g, ld32(first + 4ull * lo), st, ct);
+    const uint32_t chunk = run_at(r, st, ct, (uint32_t)(j - region_off[lo]));
+    cover[chunk] = ((uint64_t)pos << kSlotBits) | j;
+  }
+}
+
+struct ShiftArgs {
+  const uint8_t* shift;   // S (node, prev) pairs of this diff
+  uint32_t S;
+  const uint32_t* cum;    // inclusive running count by checkpoint id, ref..chkpt
+  uint32_t ncum;
+  uint32_t ref;
+  uint32_t pos;
+  Geo g;
+  const uint64_t* set_keys;
+  const uint32_t* set_vals;
+  uint64_t set_mask;
+  const uint64_t* first_ptr;   // per chain position: device address of its F roots
+  const uint64_t* region_base; // per chain position: first entry in region_off_all
+  const uint32_t* region_off_all;
+  uint64_t* cover;
+  uint32_t* err;
+};
+
+// Shifted-duplicate entries of one diff: each leaf of `node` takes the data slot
+// of the matching leaf of `prev` inside the source diff's region
+// (src/tree_approach.cpp:887-925, 1058-1118).
+__global__ void k_r_cover_shift(ShiftArgs a) {
+  const uint32_t i = blockIdx.x * kRThreads + threadIdx.x;
+  if (i >= a.S) return;
+  const uint32_t node = ld32(a.shift + 8ull * i), prev = ld32(a.shift + 8ull * i + 4);
+  const uint64_t nodes = 2ull * a.g.n - 1;
+  if (node >= nodes || prev >= nodes) {
+    atomicOr(a.err, ERR_CORRUPT);
+    return;
+  }
+  uint32_t lo = 0, hi = a.ncum;  // smallest checkpoint whose running count exceeds i
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (i < a.cum[mid]) hi = mid;
+    else lo = mid + 1;
+  }
+  if (lo >= a.ncum || lo > a.pos) {
+    atomicOr(a.err, ERR_CORRUPT);
+    return;
+  }
+  const uint32_t spos = lo;  // chain position of the source checkpoint
+  uint32_t anc = prev, region = ~0u;
+  while (true) {
+    region = rs_find(a.set_keys, a.set_vals, a.set_mask, ((uint64_t)spos << 32) | anc);
+    if (region != ~0u || anc == 0) break;
+    anc = (anc - 1) >> 1;
+  }
+  if (region == ~0u) {
+    atomicOr(a.err, ERR_CORRUPT);
+    return;
+  }
+  const uint8_t* sfirst = reinterpret_cast<const uint8_t*>(a.first_ptr[spos]);
+  const uint32_t root = ld32(sfirst + 4ull * region);
+  const uint32_t rbase = a.region_off_all[a.region_base[spos] + region];
+  uint32_t rs[2], rc[2], ns[2], nc[2], ps[2], pc[2];
+  const int rr = leaf_runs(a.g, root, rs, rc);
+  const int nr = leaf_runs(a.g, node, ns, nc);
+  const int pr = leaf_runs(a.g, prev, ps, pc);
+  const uint32_t cnt = nc[0] + (nr == 2 ? nc[1] : 0);
+  if (cnt != pc[0] + (pr == 2 ? pc[1] : 0)) {
+    atomicOr(a.err, ERR_CORRUPT);
+    return;
+  }
+  for (uint32_t j = 0; j < cnt; j++) {
+    const uint32_t dst_chunk = run_at(nr, ns, nc, j);
+    const uint32_t src_chunk = run_at(pr, ps, pc, j);
+    const uint64_t slot = rbase + run_rank(rr, rs, rc, src_chunk);
+    a.cover[dst_chunk] = ((uint64_t)spos << kSlotBits) | slot;
+  }
+}
+
+typedef uint32_t r_u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+// One scatter copy: out[chunk] = data section of diff pos, slot.
+__global__ __launch_bounds__(kRThreads) void k_r_copy(const uint64_t* cover, uint32_t n, uint32_t C, uint64_t datalen,
+                                                       const uint64_t* data_ptr, const uint64_t* data_chunks,
+                                                       uint8_t* out, int vec, uint32_t* err) {
+  __shared__ uint64_t s_src[kRThreads];
+  const uint32_t U = vec ? C / 16 : C;
+  for (uint64_t tile = blockIdx.x; tile * kRThreads < n; tile += gridDim.x) {
+    const uint64_t c0 = tile * kRThreads;
+    const uint64_t c = c0 + threadIdx.x;
+    if (c < n) {
+      const uint64_t e = cover[c];
+      uint64_t src = 0;
+      if (e == kNoCover) {
+        atomicOr(err, ERR_CORRUPT);
+      } else {
+        const uint32_t pos = (uint32_t)(e >> kSlotBits);
+        const uint64_t slot = e & kSlotMask;
+        if (slot >= data_chunks[pos]) atomicOr(err, ERR_CORRUPT);
+        else src = data_ptr[pos] + slot * C;
+      }
+      s_src[threadIdx.x] = src;
+    }
+    __syncthreads();
+    const uint64_t cnt = ((uint64_t)n - c0 < kRThreads) ? (uint64_t)n - c0 : kRThreads;
+    for (uint64_t u = threadIdx.x; u < cnt * U; u += kRThreads) {
+      const uint32_t jj = (uint32_t)(u / U), w = (uint32_t)(u % U);
+      const uint64_t src = s_src[jj];
+      if (!src) continue;
+      const uint64_t doff = (c0 + jj) * C;
+      const uint8_t* sp = reinterpret_cast<const uint8_t*>(src);
+      if (vec) {
+        const uint64_t b0 = doff + 16ull * w;
+        if (b0 + 16 <= datalen) {
+          const r_u32x4_a4 v = *reinterpret_cast<const r_u32x4_a4*>(sp + 16ull * w);
+          *reinterpret_cast<uint4*>(out + b0) = uint4{v.x, v.y, v.z, v.w};
+        } else {
+          for (uint32_t b = 0; b < 16 && b0 + b < datalen; b++) out[b0 + b] = sp[16ull * w + b];
+        }
+      } else if (doff + w < datalen) {
+        out[doff + w] = sp[w];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+static void check_err(uint32_t* err_d, hipStream_t st) {
+  uint32_t e = 0;
+  MTC_HIP(hipMemcpyAsync(&e, err_d, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  MTC_HIP(hipStreamSynchronize(st));
+  if (e & ERR_TABLE_FULL) throw Status{MTC_ERR_OUT_OF_MEMORY, "restart root set overflow"};
+  if (e) throw Status{MTC_ERR_CORRUPT_DIFF, "diff chain is inconsistent (err bits " + std::to_string(e) + ")"};
+}
+
+void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t k, bool on_device,
+                  uint8_t* out, uint64_t out_len, hipStream_t st) {
+  if (!diffs || !sizes || k >= count || !diffs[k]) throw Status{MTC_ERR_INVALID_ARG, "bad diff chain arguments"};
+  auto read_header = [&](uint32_t idx, mtc_header* h) {
+    if (!diffs[idx] || sizes[idx] < sizeof(mtc_header)) throw Status{MTC_ERR_CORRUPT_DIFF, "missing or short diff"};
+    if (on_device) MTC_HIP(hipMemcpy(h, diffs[idx], sizeof(mtc_header), hipMemcpyDeviceToHost));
+    else std::memcpy(h, diffs[idx], sizeof(mtc_header));
+  };
+  mtc_header hk;
+  read_header(k, &hk);
+  if (hk.chkpt_id != k || hk.ref_id > k || hk.chunk_size == 0 || hk.datalen == 0)
+    throw Status{MTC_ERR_CORRUPT_DIFF, "bad header of the requested diff"};
+  if (out_len < hk.datalen) throw Status{MTC_ERR_BUFFER_TOO_SMALL, "output shorter than datalen"};
+  const uint64_t C = hk.chunk_size;
+  const uint64_t n64 = (hk.datalen + C - 1) / C;
+  if (n64 > (1ull << 31)) throw Status{MTC_ERR_CORRUPT_DIFF, "too many chunks"};
+  const uint32_t n = (uint32_t)n64, ref = hk.ref_id, L = k - ref + 1;
+  const Geo g = make_geo(n);
+
+  // Chain layout, with host diffs uploaded once.
+  std::vector<ChainDiff> ch(L);
+  DevBuf<uint8_t> upload;
+  uint64_t up_total = 0;
+  if (!on_device)
+    for (uint32_t p = 0; p < L; p++) up_total += (sizes[ref + p] + 255) & ~255ull;
+  if (!on_device) upload.ensure(up_total);
+  uint64_t up_off = 0, sumF = 0;
+  std::vector<std::vector<uint32_t>> ptables(L);
+  for (uint32_t p = 0; p < L; p++) {
+    const uint32_t idx = ref + p;
+    ChainDiff& d = ch[p];
+    read_header(idx, &d.h);
+    if (d.h.chkpt_id != idx || d.h.ref_id != ref || d.h.datalen != hk.datalen || d.h.chunk_size != hk.chunk_size)
+      throw Status{MTC_ERR_CORRUPT_DIFF, "diff " + std::to_string(idx) + " does not belong to the chain"};
+    d.size = sizes[idx];
+    d.off_first = 32;
+    d.off_table = d.off_first + 4ull * d.h.num_first_ocur;
+    d.off_shift = d.off_table + 8ull * d.h.num_prior_chkpts;
+    d.off_data = d.off_shift + 8ull * d.h.num_shift_dupl;
+    if (d.off_data > d.size) throw Status{MTC_ERR_CORRUPT_DIFF, "diff shorter than its metadata"};
+    d.data_chunks = (d.size - d.off_data) / C;
+    if (on_device) {
+      d.base = static_cast<const uint8_t*>(diffs[idx]);
+    } else {
+      MTC_HIP(hipMemcpyAsync(upload.p + up_off, diffs[idx], d.size, hipMemcpyHostToDevice, st));
+      d.base = upload.p + up_off;
+      up_off += (d.size + 255) & ~255ull;
+    }
+    ptables[p].resize(2ull * d.h.num_prior_chkpts);
+    if (d.h.num_prior_chkpts) {
+      if (on_device)
+        MTC_HIP(hipMemcpy(ptables[p].data(), d.base + d.off_table, 8ull * d.h.num_prior_chkpts, hipMemcpyDeviceToHost));
+      else
+        std::memcpy(ptables[p].data(), static_cast<const uint8_t*>(diffs[idx]) + d.off_table,
+                    8ull * d.h.num_prior_chkpts);
+    }
+    d.region_base = sumF + p;
+    sumF += d.h.num_first_ocur;
+  }
+
+  Workspace ws;
+  DevBuf<uint32_t> err, region_off, cum;
+  DevBuf<uint64_t> set_keys, cover, meta;
+  DevBuf<uint32_t> set_vals;
+  err.ensure(1);
+  k_r_fill32<<<1, kRThreads, 0, st>>>(err.p, 1, 0);
+  region_off.ensure(sumF + L);
+  uint64_t set_cap = 16;
+  while (set_cap < 2 * sumF) set_cap <<= 1;
+  set_keys.ensure(set_cap);
+  set_vals.ensure(set_cap);
+  k_r_fill64<<<rgrid(set_cap, 65536), kRThreads, 0, st>>>(set_keys.p, set_cap, kNoCover);
+  cover.ensure(n);
+  k_r_fill64<<<rgrid(n, 65536), kRThreads, 0, st>>>(cover.p, n, kNoCover);
+  MTC_HIP(hipGetLastError());
+
+  // Per-position tables for the kernels: F-root address, region base, data address, data chunks.
+  std::vector<uint64_t> meta_h(4ull * L);
+  for (uint32_t p = 0; p < L; p++) {
+    meta_h[p] = reinterpret_cast<uint64_t>(ch[p].base + ch[p].off_first);
+    meta_h[L + p] = ch[p].region_base;
+    meta_h[2 * L + p] = reinterpret_cast<uint64_t>(ch[p].base + ch[p].off_data);
+    meta_h[3 * L + p] = ch[p].data_chunks;
+  }
+  meta.ensure(4ull * L);
+  MTC_HIP(hipMemcpyAsync(meta.p, meta_h.data(), 8 * meta_h.size(), hipMemcpyHostToDevice, st));
+
+  // Region offsets and the (position, root) set.
+  for (uint32_t p = 0; p < L; p++) {
+    const ChainDiff& d = ch[p];
+    const uint32_t F = d.h.num_first_ocur;
+    uint32_t* ro = region_off.p + d.region_base;
+    if (F) {
+      k_r_sizes<<<rgrid(F), kRThreads, 0, st>>>(d.base + d.off_first, F, g, ro, err.p);
+      k_r_set_insert<<<rgrid(F), kRThreads, 0, st>>>(d.base + d.off_first, F, p, set_keys.p, set_vals.p, set_cap - 1,
+                                                     err.p);
+      MTC_HIP(hipGetLastError());
+    }
+    exclusive_scan_u32(ro, ro, F, ro + F, ws, st);
+  }
+  std::vector<uint32_t> totals(L);
+  for (uint32_t p = 0; p < L; p++)
+    MTC_HIP(hipMemcpyAsync(&totals[p], region_off.p + ch[p].region_base + ch[p].h.num_first_ocur, sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, st));
+  check_err(err.p, st);
+  for (uint32_t p = 0; p < L; p++)
+    if (totals[p] > ch[p].data_chunks) throw Status{MTC_ERR_CORRUPT_DIFF, "data section shorter than its regions"};
+
+  // Cover chunks, oldest diff first so newer diffs overwrite.
+  for (uint32_t p = 0; p < L; p++) {
+    const ChainDiff& d = ch[p];
+    const uint32_t F = d.h.num_first_ocur, S = d.h.num_shift_dupl;
+    if (F) {
+      k_r_cover_first<<<rgrid(totals[p], 65536), kRThreads, 0, st>>>(d.base + d.off_first, F,
+                                                                    region_off.p + d.region_base, totals[p], g, p,
+                                                                    cover.p);
+      MTC_HIP(hipGetLastError());
+    }
+    if (S) {
+      // Running count of shifted entries by source checkpoint (ascending id), :872-883.
+      const uint32_t idx = ref + p;
+      std::vector<uint32_t> cnt(idx - ref + 1, 0);
+      for (uint32_t t = 0; t < d.h.num_prior_chkpts; t++) {
+        const uint32_t c = ptables[p][2 * t], m = ptables[p][2 * t + 1];
+        if (c < ref || c > idx) throw Status{MTC_ERR_CORRUPT_DIFF, "shifted entries name a checkpoint off the chain"};
+        cnt[c - ref] += m;
+      }
+      for (size_t t = 1; t < cnt.size(); t++) cnt[t] += cnt[t - 1];
+      if (cnt.back() != S) throw Status{MTC_ERR_CORRUPT_DIFF, "prior-checkpoint table does not sum to S"};
+      cum.ensure(cnt.size());
+      MTC_HIP(hipMemcpyAsync(cum.p, cnt.data(), 4 * cnt.size(), hipMemcpyHostToDevice, st));
+      ShiftArgs sa{d.base + d.off_shift, S, cum.p, (uint32_t)cnt.size(), ref, p, g, set_keys.p, set_vals.p,
+                   set_cap - 1, meta.p, meta.p + L, region_off.p, cover.p, err.p};
+      k_r_cover_shift<<<rgrid(S), kRThreads, 0, st>>>(sa);
+      MTC_HIP(hipGetLastError());
+      MTC_HIP(hipStreamSynchronize(st));  // cnt (host) and cum are reused per position
+    }
+  }
+  const int vec = (C % 16 == 0) && ((uintptr_t)out % 16 == 0);
+  k_r_copy<<<rgrid(n, 8192), kRThreads, 0, st>>>(cover.p, n, (uint32_t)C, hk.datalen, meta.p + 2 * L, meta.p + 3 * L,
+                                                 out, vec, err.p);
+  MTC_HIP(hipGetLastError());
+  check_err(err.p, st);
+}
+
+}  // namespace mtc

@@ -1,0 +1,175 @@
+"""Synthetic checkpoint series: plans drawn on the host, applied on the device.
+
+One series is a buffer of `nbytes` split into chunks of `chunk_size`, mutated
+between checkpoints the way the benchmark configurations describe:
+
+* step 0: every byte from the pseudo-random stream of step 0, then zero-filled
+  runs of U{1..zero_max} chunks at uniform positions until the run lengths add up
+  to zero_frac of the chunks (overlaps allowed);
+* step k >= 1: (i) runs of U{1..fresh_max} chunks re-randomised from the stream of
+  step k until their lengths add up to fresh_frac of the chunks; then (ii)
+  round(copy_frac * n) single-chunk copies from a uniform source chunk to a
+  different uniform destination, all sources read after (i) and before any write,
+  the last copy to a destination winning; (iii) everything else unchanged.
+
+The stream byte at offset b for key K is byte (b & 7) of
+splitmix64_mix(K + ((b >> 3) + 1) * 0x9E3779B97F4A7C15), so the device kernels
+(libmtcheck_amd.so, include/mtcheck_workload.h) and the numpy mirror below produce
+identical buffers.  Run and copy plans come from numpy's PCG64 seeded with
+(seed, step).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+CONFIG0_SEED = 0x4D54434B
+
+
+def _mix(z: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def mix64(x: int) -> int:
+    return int(_mix(np.array([x & 0xFFFFFFFFFFFFFFFF], np.uint64))[0])
+
+
+def stream_key(seed: int, step: int) -> int:
+    return mix64((seed * 0xD1B54A32D192ED03 + step + 1) & 0xFFFFFFFFFFFFFFFF)
+
+
+def stream_bytes(key: int, begin: int, end: int) -> np.ndarray:
+    """Bytes [begin, end) of the stream with `key` (numpy mirror of the device fill)."""
+    if end <= begin:
+        return np.zeros(0, np.uint8)
+    w0, w1 = begin >> 3, (end + 7) >> 3
+    w = np.arange(w0, w1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        words = _mix(np.uint64(key) + (w + np.uint64(1)) * GOLDEN)
+    b = words.view(np.uint8)
+    return b[begin - 8 * w0:end - 8 * w0]
+
+
+def _draw_runs(rng: np.random.Generator, n: int, frac: float, max_len: int) -> np.ndarray:
+    """(first chunk, count) runs until the counts reach round(frac * n)."""
+    target = int(round(frac * n))
+    if target <= 0 or n <= 0:
+        return np.zeros((0, 2), np.uint32)
+    max_len = max(1, min(max_len, n))
+    out, total = [], 0
+    while total < target:
+        k = max(16, int(2 * (target - total) / ((1 + max_len) / 2)) + 16)
+        lens = rng.integers(1, max_len + 1, size=k)
+        starts = rng.integers(0, n - lens + 1)
+        cs = total + np.cumsum(lens)
+        cut = int(np.searchsorted(cs, target, side="left")) + 1
+        cut = min(cut, k)
+        out.append(np.stack([starts[:cut], lens[:cut]], axis=1))
+        total = int(cs[cut - 1])
+    return np.concatenate(out).astype(np.uint32)
+
+
+@dataclass
+class StepPlan:
+    step: int
+    key: int
+    runs: np.ndarray      # (r, 2) uint32: first chunk, count
+    zero_runs: bool
+    pairs: np.ndarray     # (m, 2) uint32: destination chunk, source chunk (distinct destinations)
+
+
+class CheckpointSeries:
+    def __init__(self, nbytes: int, chunk_size: int, seed: int = CONFIG0_SEED, zero_frac: float = 0.10,
+                 zero_max: int = 32, fresh_frac: float = 0.02, fresh_max: int = 64, copy_frac: float = 0.01):
+        self.nbytes, self.chunk_size, self.seed = nbytes, chunk_size, seed
+        self.n = (nbytes + chunk_size - 1) // chunk_size
+        self.zero_frac, self.zero_max = zero_frac, zero_max
+        self.fresh_frac, self.fresh_max, self.copy_frac = fresh_frac, fresh_max, copy_frac
+
+    @classmethod
+    def with_change(cls, nbytes: int, chunk_size: int, change: float, seed: int = CONFIG0_SEED):
+        """Change fraction split 2:1 between fresh runs and shifted-duplicate copies."""
+        return cls(nbytes, chunk_size, seed, fresh_frac=2 * change / 3, copy_frac=change / 3)
+
+    def plan(self, step: int) -> StepPlan:
+        rng = np.random.Generator(np.random.PCG64([self.seed, step]))
+        key = stream_key(self.seed, step)
+        if step == 0:
+            return StepPlan(0, key, _draw_runs(rng, self.n, self.zero_frac, self.zero_max), True,
+                            np.zeros((0, 2), np.uint32))
+        runs = _draw_runs(rng, self.n, self.fresh_frac, self.fresh_max)
+        m = int(round(self.copy_frac * self.n)) if self.n > 1 else 0
+        if m:
+            dst = rng.integers(0, self.n, size=m)
+            src = rng.integers(0, self.n - 1, size=m)
+            src = src + (src >= dst)
+            # last copy to each destination wins
+            rev_dst = dst[::-1]
+            _, first = np.unique(rev_dst, return_index=True)
+            keep = np.sort(m - 1 - first)
+            pairs = np.stack([dst[keep], src[keep]], axis=1).astype(np.uint32)
+        else:
+            pairs = np.zeros((0, 2), np.uint32)
+        return StepPlan(step, key, runs, False, pairs)
+
+    # -- host mirror -----------------------------------------------------------------
+    def apply_host(self, buf: np.ndarray, step: int) -> np.ndarray:
+        p = self.plan(step)
+        C, L = self.chunk_size, self.nbytes
+        if step == 0:
+            buf[:] = stream_bytes(p.key, 0, L)
+        for first, cnt in p.runs:
+            b, e = int(first) * C, min((int(first) + int(cnt)) * C, L)
+            buf[b:e] = 0 if p.zero_runs else stream_bytes(p.key, b, e)
+        if len(p.pairs):
+            tmp = np.zeros((len(p.pairs), C), np.uint8)
+            for i, (_, s) in enumerate(p.pairs):
+                b = int(s) * C
+                seg = buf[b:min(b + C, L)]
+                tmp[i, :seg.size] = seg
+            for i, (d, _) in enumerate(p.pairs):
+                b = int(d) * C
+                e = min(b + C, L)
+                buf[b:e] = tmp[i, :e - b]
+        return buf
+
+    def host_buffer(self, step: int) -> np.ndarray:
+        buf = np.zeros(self.nbytes, np.uint8)
+        for s in range(step + 1):
+            self.apply_host(buf, s)
+        return buf
+
+    # -- device ------------------------------------------------------------------------
+    def apply_device(self, buf, step: int, stream=None) -> None:
+        """Mutate a uint8 cuda tensor in place (kernels of libmtcheck_amd.so)."""
+        import torch
+
+        from ._lib import check, lib
+        p = self.plan(step)
+        st = None if stream is None else int(getattr(stream, "cuda_stream", stream))
+        ptr = buf.data_ptr()
+        if step == 0:
+            check(lib().mtc_wl_fill_stream(ptr, 0, self.nbytes, p.key, st))
+        if len(p.runs):
+            runs = torch.from_numpy(np.ascontiguousarray(p.runs)).to(buf.device)
+            check(lib().mtc_wl_fill_runs(ptr, self.nbytes, self.chunk_size, runs.data_ptr(), len(p.runs), p.key,
+                                         1 if p.zero_runs else 0, st))
+        if len(p.pairs):
+            pairs = torch.from_numpy(np.ascontiguousarray(p.pairs)).to(buf.device)
+            tmp = torch.empty(len(p.pairs) * self.chunk_size, dtype=torch.uint8, device=buf.device)
+            check(lib().mtc_wl_copy_chunks(ptr, self.nbytes, self.chunk_size, pairs.data_ptr(), len(p.pairs),
+                                           tmp.data_ptr(), st))
+        torch.cuda.synchronize(buf.device)
+
+    def device_buffer(self, step: int, device="cuda:0"):
+        import torch
+        buf = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
+        for s in range(step + 1):
+            self.apply_device(buf, s)
+        return buf
+
