@@ -10,8 +10,7 @@
 namespace mtc {
 
 constexpr uint32_t kThreads = 256;
-constexpr uint32_t kFuse = 10;             // tree levels fused per sweep stage
-constexpr uint32_t kTile = 1u << kFuse;    // positions per stage tile
+constexpr uint32_t kFuse = 8;              // tree levels fused per sweep stage (one wavefront tile)
 constexpr uint64_t kFirstTag = 0xFFFFFFFFull;  // staging tag of a first-occurrence root
 
 // ---------------------------------------------------------------------------
@@ -27,7 +26,32 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t count, uint32_t v) {
     p[i] = v;
 }
 
-__global__ void k_zero_ctl(Ctl* c) {
+// Per-checkpoint counters are striped over kStripes addresses (by block index) so
+// that a million wavefronts do not serialise on one atomic; k_reduce_ctr folds them.
+constexpr uint32_t kStripes = 256;
+enum : uint32_t { CTR_CLAIMS = 0, CTR_CHANGED = 1, CTR_KINDS = 2 };
+
+__device__ __forceinline__ unsigned long long* stripe(unsigned long long* ctr, uint32_t kind) {
+  return ctr + kind * kStripes + (blockIdx.x & (kStripes - 1));
+}
+
+__global__ void k_reduce_ctr(const unsigned long long* ctr, Ctl* c) {
+  __shared__ unsigned long long s[CTR_KINDS];
+  if (threadIdx.x < CTR_KINDS) s[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t k = 0; k < CTR_KINDS; k++) {
+    const unsigned long long v = ctr[k * kStripes + threadIdx.x];
+    if (v) atomicAdd(&s[k], v);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    c->claims = s[CTR_CLAIMS];
+    c->changed = s[CTR_CHANGED];
+  }
+}
+
+__global__ void k_zero_ctl(Ctl* c, unsigned long long* ctr) {
+  for (uint32_t i = threadIdx.x; i < CTR_KINDS * kStripes; i += blockDim.x) ctr[i] = 0;
   if (threadIdx.x == 0) {
     c->err = 0;
     c->F = c->S = c->P = 0;
@@ -52,64 +76,82 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 }
 
 // ---------------------------------------------------------------------------
-// chunk hashing, 16-byte loads (chunk sizes that are multiples of 16, aligned data)
+// LDS-staged tile hashing.  A 256-thread block hashes 256 consecutive chunks, one
+// per thread.  The chunks are brought in as segments of SEG bytes per chunk with
+// fully coalesced 16-byte loads (each block-wide load instruction covers 4 KiB),
+// written to LDS with a 16-byte pad per chunk (so the per-thread 16-byte reads
+// of the hashing phase hit distinct banks), then each thread hashes its own chunk
+// from LDS.  Murmur3 keeps (h1, h2) across segments; MD5 keeps its state and
+// compresses every 64 bytes.
 // ---------------------------------------------------------------------------
 template <int CB>
-__device__ __forceinline__ Digest m3_chunk(const uint8_t* p) {
-  const uint4* q = reinterpret_cast<const uint4*>(p);
-  uint64_t h1 = 0, h2 = 0;
-  constexpr int NB = CB / 16;
-#pragma unroll(NB <= 32 ? NB : 8)
-  for (int b = 0; b < NB; b++) {
-    uint4 v = q[b];
-    m3_block(h1, h2, (uint64_t)v.x | ((uint64_t)v.y << 32), (uint64_t)v.z | ((uint64_t)v.w << 32));
-  }
-  return m3_final(h1, h2, CB);
-}
+struct TileShape {
+  static constexpr int SEG = CB < 128 ? CB : 128;  // bytes of each chunk per pass
+  static constexpr int PITCH = SEG + 16;           // LDS bytes per chunk
+  static constexpr int LOADS = SEG / 16;           // 16-byte loads per thread per pass
+  static constexpr int PASSES = CB / SEG;
+  static constexpr int LDS = kThreads * PITCH;
+};
 
-template <int CB>
-__device__ __forceinline__ Digest md5_chunk(const uint8_t* p) {
-  const uint4* q = reinterpret_cast<const uint4*>(p);
+// Returns the digest of chunk (first + threadIdx.x) when it is a full chunk; the
+// caller hashes a short last chunk separately.  `full` = number of full chunks.
+template <int HK, int CB>
+__device__ __forceinline__ Digest staged_hash(const uint8_t* data, uint64_t first, uint64_t full, uint8_t* lds) {
+  using S = TileShape<CB>;
+  const uint32_t t = threadIdx.x;
+  uint64_t h1 = 0, h2 = 0;
   uint32_t st[4];
   md5_init(st);
   uint32_t X[16];
-  constexpr int NB = CB / 64;
-#pragma unroll(NB <= 8 ? NB : 2)
-  for (int b = 0; b < NB; b++) {
+#pragma unroll 1
+  for (int pass = 0; pass < S::PASSES; pass++) {
+    uint4 v[S::LOADS];
 #pragma unroll
-    for (int w = 0; w < 4; w++) {
-      uint4 v = q[4 * b + w];
-      X[4 * w] = v.x;
-      X[4 * w + 1] = v.y;
-      X[4 * w + 2] = v.z;
-      X[4 * w + 3] = v.w;
+    for (int i = 0; i < S::LOADS; i++) {
+      const uint32_t u = i * (kThreads * 16) + 16 * t;  // byte within this pass's tile
+      const uint32_t j = u / S::SEG, w = u % S::SEG;
+      v[i] = uint4{0, 0, 0, 0};
+      if (first + j < full)
+        v[i] = *reinterpret_cast<const uint4*>(data + (first + j) * (uint64_t)CB + (uint64_t)pass * S::SEG + w);
     }
-    md5_compress(st, X);
-  }
-  constexpr int RW = (CB % 64) / 16;  // remaining 16-byte words
+    __syncthreads();  // the previous pass's reads are done
 #pragma unroll
-  for (int w = 0; w < 16; w++) X[w] = 0;
+    for (int i = 0; i < S::LOADS; i++) {
+      const uint32_t u = i * (kThreads * 16) + 16 * t;
+      const uint32_t j = u / S::SEG, w = u % S::SEG;
+      *reinterpret_cast<uint4*>(lds + j * S::PITCH + w) = v[i];
+    }
+    __syncthreads();
+    const uint8_t* mine = lds + t * S::PITCH;
+    if constexpr (HK == HASH_MD5) {
 #pragma unroll
-  for (int w = 0; w < RW; w++) {
-    uint4 v = q[4 * NB + w];
-    X[4 * w] = v.x;
-    X[4 * w + 1] = v.y;
-    X[4 * w + 2] = v.z;
-    X[4 * w + 3] = v.w;
-  }
-  md5_finish_words(st, X, (uint32_t)(CB % 64), (uint64_t)CB);
-  return md5_digest(st);
-}
-
-template <int HK, int CB>
-__device__ __forceinline__ Digest hash_chunk(const uint8_t* p, uint64_t nbytes) {
-  if constexpr (CB > 0) {
-    if (nbytes == (uint64_t)CB) {
-      if constexpr (HK == HASH_MD5) return md5_chunk<CB>(p);
-      else return m3_chunk<CB>(p);
+      for (int q = 0; q < S::LOADS; q++) {
+        const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * q);
+        const int wq = q & 3;  // LOADS is a multiple of 4 whenever PASSES > 1
+        X[4 * wq] = b.x;
+        X[4 * wq + 1] = b.y;
+        X[4 * wq + 2] = b.z;
+        X[4 * wq + 3] = b.w;
+        if (wq == 3) md5_compress(st, X);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < S::LOADS; q++) {
+        const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * q);
+        m3_block(h1, h2, (uint64_t)b.x | ((uint64_t)b.y << 32), (uint64_t)b.z | ((uint64_t)b.w << 32));
+      }
     }
   }
-  return hash_bytes<HK>(p, nbytes);
+  if constexpr (HK == HASH_MD5) {
+    constexpr int RW = (CB % 64) / 16;  // words of a final partial 64-byte block already in X
+    uint32_t T[16];
+#pragma unroll
+    for (int w = 0; w < 16; w++) T[w] = (w < 4 * RW) ? X[w] : 0u;
+    md5_finish_words(st, T, (uint32_t)(CB % 64), (uint64_t)CB);
+    return md5_digest(st);
+  } else {
+    return m3_final(h1, h2, CB);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -129,18 +171,36 @@ struct LeafArgs {
   uint32_t cur;
   int baseline;
   Ctl* ctl;
+  unsigned long long* ctr;
 };
+
+// Digest of chunk blockIdx.x*256 + threadIdx.x (zero beyond n).  CB > 0 selects
+// the LDS-staged path for 16-byte-aligned data and chunk sizes that are multiples
+// of 16; a short last chunk and CB == 0 use byte-granular hashing.
+template <int HK, int CB>
+__device__ __forceinline__ Digest tile_digest(const uint8_t* data, uint64_t len, uint32_t n, uint32_t C) {
+  const uint64_t first = (uint64_t)blockIdx.x * kThreads;
+  const uint64_t c = first + threadIdx.x;
+  Digest d{0, 0};
+  if constexpr (CB > 0) {
+    __shared__ __align__(16) uint8_t lds[TileShape<CB>::LDS];
+    const uint64_t full = len / CB;
+    d = staged_hash<HK, CB>(data, first, full, lds);
+    if (c < n && c >= full) d = hash_bytes<HK>(data + c * CB, len - c * CB);
+  } else {
+    if (c < n) {
+      const uint64_t off = c * C;
+      d = hash_bytes<HK>(data + off, (off + C <= len) ? C : len - off);
+    }
+  }
+  return d;
+}
 
 template <int HK, int CB>
 __global__ __launch_bounds__(kThreads) void k_leaf(LeafArgs a) {
   const uint64_t c = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   const bool valid = c < a.n;
-  Digest d{0, 0};
-  if (valid) {
-    const uint64_t off = c * a.C;
-    const uint64_t nb = (off + a.C <= a.len) ? a.C : a.len - off;
-    d = hash_chunk<HK, CB>(a.data + off, nb);
-  }
+  const Digest d = tile_digest<HK, CB>(a.data, a.len, a.n, a.C);
   Digest prev{0, 0};
   if (valid && !a.baseline) prev = a.leaf_dg[c];
   const bool fixed = valid && d == prev;
@@ -166,8 +226,8 @@ __global__ __launch_bounds__(kThreads) void k_leaf(LeafArgs a) {
   const int src_lane = mine ? 63 - __clzll((long long)mine) : (int)lane;
   const uint32_t hslot = __shfl(slot, src_lane);
   if (follower) slot = hslot;
-  wave_count_add(&a.ctl->claims, claimed);
-  wave_count_add(&a.ctl->changed, valid && !fixed);
+  wave_count_add(stripe(a.ctr, CTR_CLAIMS), claimed);
+  wave_count_add(stripe(a.ctr, CTR_CHANGED), valid && !fixed);
   if (valid) {
     a.labels[c] = fixed ? L_FIXED : L_PENDING;
     if (ins) {
@@ -182,17 +242,18 @@ template <int HK, int CB>
 __global__ __launch_bounds__(kThreads) void k_hash_chunks(const uint8_t* data, uint64_t len, uint32_t n, uint32_t C,
                                                           Digest* out) {
   const uint64_t c = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (c >= n) return;
-  const uint64_t off = c * C;
-  const uint64_t nb = (off + C <= len) ? C : len - off;
-  out[c] = hash_chunk<HK, CB>(data + off, nb);
+  const Digest d = tile_digest<HK, CB>(data, len, n, C);
+  if (c < n) out[c] = d;
 }
 
 // ---------------------------------------------------------------------------
-// Fused forest sweeps.  A stage covers parent depths [d_out, d_in-1]; block b owns
-// positions [b*T, (b+1)*T) at depth d_in (T = 2^(d_in-d_out) <= kTile) and the
-// subtree above them.  Stage 0 reads the leaves (bottom level and, when n is not a
-// power of two, the depth D-1 leaves); later stages read the previous frontier.
+// Forest sweeps.  A stage covers parent depths [d_out, d_in-1] (at most kFuse
+// levels).  One wavefront owns a tile: positions [t*T, (t+1)*T) at depth d_in
+// (T = 2^(d_in-d_out) <= 256) and the subtree above them.  Lane l holds positions
+// 4l..4l+3; levels 1 and 2 are combined in registers, levels 3.. by
+// __shfl_down, so no block barrier is needed.  Stage 0 reads the leaves (bottom
+// level and, when n is not a power of two, the depth D-1 leaves); later stages
+// read the previous stage's frontier (one entry per tile root).
 // ---------------------------------------------------------------------------
 struct SweepArgs {
   Geo g;
@@ -211,6 +272,7 @@ struct SweepArgs {
   Slot* tab;
   uint64_t mask;
   Ctl* ctl;
+  unsigned long long* ctr;
   uint64_t* staging;
   uint32_t* cnt_f;
   uint32_t* cnt_s;
@@ -231,105 +293,352 @@ __device__ __forceinline__ uint8_t resolve_leaf(const SweepArgs& a, uint64_t c) 
 
 __device__ __forceinline__ uint64_t bottom_chunk(const Geo& g, uint64_t p) { return (1ull << g.D) - g.n + p; }
 
+__device__ __forceinline__ bool is_region(uint8_t l) { return l == L_FIRST || l == L_SHIFT; }
+
+__device__ __forceinline__ uint64_t heap_id(uint32_t depth, uint64_t pos) { return ((1ull << depth) - 1) + pos; }
+
+// Wave-wide exclusive prefix and total of small per-lane counts (0..7).
+__device__ __forceinline__ uint32_t wave_rank(uint32_t cnt, uint32_t* total) {
+  const uint32_t lane = lane_id();
+  const uint64_t lt = (1ull << lane) - 1;
+  uint32_t r = 0, t = 0;
+#pragma unroll
+  for (int b = 0; b < 3; b++) {
+    const uint64_t m = __ballot((cnt >> b) & 1);
+    r += (uint32_t)__popcll(m & lt) << b;
+    t += (uint32_t)__popcll(m) << b;
+  }
+  *total = t;
+  return r;
+}
+
+__device__ __forceinline__ Digest shfl_down_dg(const Digest& d, int s) {
+  return Digest{__shfl_down(d.lo, s), __shfl_down(d.hi, s)};
+}
+
 // Sweep A: first-occurrence forest (include/reference_impl.hpp:74-82).  Resolves
 // pending leaves, hashes every node whose children are both first occurrences and
 // inserts it (insert-if-absent).
 template <int HK>
+struct SweepA {
+  const SweepArgs& a;
+  uint32_t claims = 0;
+
+  __device__ void combine(uint32_t depth, uint64_t pos, uint8_t la, const Digest& da, uint8_t lb, const Digest& db,
+                          uint8_t& lo, Digest& dout) {
+    lo = L_NONE;
+    if (la == L_FIRST && lb == L_FIRST) {
+      dout = hash_pair<HK>(da, db);
+      const uint32_t node = (uint32_t)heap_id(depth, pos);
+      bool cl = false;
+      table_insert(a.tab, a.mask, dout, ((uint64_t)a.cur << 32) | node, false, &a.ctl->err, &cl);
+      claims += cl;
+      lo = L_FIRST;
+      if (a.dbg_tree) a.dbg_tree[node] = dout;
+    }
+  }
+};
+
+template <int HK>
 __global__ __launch_bounds__(kThreads) void k_sweep_a(SweepArgs a) {
-  __shared__ uint8_t s_lab[kTile];
-  __shared__ Digest s_dg[kTile];
-  __shared__ uint8_t s_lab2[kTile / 2];
-  __shared__ Digest s_dg2[kTile / 2];
-  const uint32_t T = 1u << (a.d_in - a.d_out);
-  const uint64_t base = (uint64_t)blockIdx.x * T;
-  const uint32_t tid = threadIdx.x;
-  int any_first = 0;
-  for (uint32_t i = tid; i < T; i += kThreads) {
-    const uint64_t p = base + i;
-    uint8_t l = L_NONE;
+  const uint32_t lane = lane_id();
+  const uint64_t tile = (uint64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (tile >= (1ull << a.d_out)) return;  // whole wavefront
+  const uint32_t R = a.d_in - a.d_out, T = 1u << R;
+  const uint64_t base = tile << R;
+  uint8_t l0[4];
+  Digest g0[4];
+  bool any = false;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const uint32_t lp = 4 * lane + i;
+    l0[i] = L_NONE;
+    if (lp >= T) continue;
+    const uint64_t p = base + lp;
     if (a.stage0) {
-      if (p < a.g.m) {
-        const uint64_t c = bottom_chunk(a.g, p);
-        l = a.labels[c];
-        if (l == L_PENDING) {
-          l = resolve_leaf(a, c);
-          a.labels[c] = l;
-        }
-        if (l == L_FIRST) s_dg[i] = a.leaf_dg[c];
+      if (p >= a.g.m) continue;
+      const uint64_t c = bottom_chunk(a.g, p);
+      uint8_t l = a.labels[c];
+      if (l == L_PENDING) {
+        l = resolve_leaf(a, c);
+        a.labels[c] = l;
       }
+      l0[i] = l;
+      if (l == L_FIRST) g0[i] = a.leaf_dg[c];
     } else {
-      l = a.in_lab[p];
-      if (l == L_FIRST) s_dg[i] = a.in_dg[p];
+      l0[i] = a.in_lab[p];
+      if (l0[i] == L_FIRST) g0[i] = a.in_dg[p];
     }
-    s_lab[i] = l;
-    any_first |= (l == L_FIRST);
+    any |= (l0[i] == L_FIRST);
   }
-  if (a.stage0) {  // depth D-1 leaves of this tile
-    const uint64_t qb = base / 2;
-    for (uint32_t i = tid; i < T / 2; i += kThreads) {
-      const uint64_t q = qb + i;
-      uint8_t l = L_NONE;
-      if (q >= a.g.half) {
-        const uint64_t c = q - a.g.half;
-        l = a.labels[c];
-        if (l == L_PENDING) {
-          l = resolve_leaf(a, c);
-          a.labels[c] = l;
-        }
-        if (l == L_FIRST) s_dg2[i] = a.leaf_dg[c];
+  // Depth D-1 leaves among this tile's level-1 positions (stage 0 only).
+  uint8_t lf[2] = {L_NONE, L_NONE};
+  Digest gf[2];
+  if (a.stage0) {
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const uint32_t j = 2 * lane + k;
+      const uint64_t q = (base >> 1) + j;
+      if (j >= T / 2 || q < a.g.half) continue;
+      const uint64_t c = q - a.g.half;
+      uint8_t l = a.labels[c];
+      if (l == L_PENDING) {
+        l = resolve_leaf(a, c);
+        a.labels[c] = l;
       }
-      s_lab2[i] = l;
-      any_first |= (l == L_FIRST);
+      lf[k] = l;
+      if (l == L_FIRST) gf[k] = a.leaf_dg[c];
+      any |= (l == L_FIRST);
     }
   }
-  if (!__syncthreads_or(any_first)) {
-    if (tid == 0) a.out_lab[blockIdx.x] = L_NONE;
+  if (!__ballot(any)) {
+    if (lane == 0) a.out_lab[tile] = L_NONE;
     return;
   }
-  uint32_t claims = 0;
-  for (uint32_t r = 1; r <= a.d_in - a.d_out; r++) {
-    const uint32_t lvl = a.d_in - r;
-    const uint32_t P = T >> r;
-    const uint64_t pb = base >> r;
-    const uint32_t per = (P + kThreads - 1) / kThreads;  // 1 or 2
-    uint8_t nl[2] = {L_NONE, L_NONE};
-    Digest nd[2];
-    for (uint32_t k = 0; k < per; k++) {
-      const uint32_t i = tid * per + k;
-      if (i >= P) break;
-      const uint64_t q = pb + i;
-      if (a.stage0 && r == 1 && q >= a.g.half) {
-        nl[k] = s_lab2[i];
-        nd[k] = s_dg2[i];
-      } else if (s_lab[2 * i] == L_FIRST && s_lab[2 * i + 1] == L_FIRST) {
-        const Digest d = hash_pair<HK>(s_dg[2 * i], s_dg[2 * i + 1]);
-        const uint32_t node = (uint32_t)(((1ull << lvl) - 1) + q);
-        bool cl = false;
-        table_insert(a.tab, a.mask, d, ((uint64_t)a.cur << 32) | node, false, &a.ctl->err, &cl);
-        claims += cl;
-        nl[k] = L_FIRST;
-        nd[k] = d;
-        if (a.dbg_tree) a.dbg_tree[node] = d;
-      }
+  SweepA<HK> s{a};
+  // level 1 (two parents per lane)
+  uint8_t l1[2] = {L_NONE, L_NONE};
+  Digest g1[2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const uint32_t j = 2 * lane + k;
+    if (j >= T / 2) continue;
+    const uint64_t q = (base >> 1) + j;
+    if (a.stage0 && q >= a.g.half) {
+      l1[k] = lf[k];
+      g1[k] = gf[k];
+    } else {
+      s.combine(a.d_in - 1, q, l0[2 * k], g0[2 * k], l0[2 * k + 1], g0[2 * k + 1], l1[k], g1[k]);
     }
-    __syncthreads();
-    for (uint32_t k = 0; k < per; k++) {
-      const uint32_t i = tid * per + k;
-      if (i >= P) break;
-      s_lab[i] = nl[k];
-      if (nl[k] == L_FIRST) s_dg[i] = nd[k];
-    }
-    __syncthreads();
   }
-  const uint32_t wc = wave_sum(claims);
-  if (lane_id() == 0 && wc) atomicAdd(&a.ctl->claims, (unsigned long long)wc);
-  if (tid == 0) {
-    a.out_lab[blockIdx.x] = s_lab[0];
-    if (s_lab[0] == L_FIRST) a.out_dg[blockIdx.x] = s_dg[0];
+  uint8_t lc = l1[0];
+  Digest gc = g1[0];
+  if (R >= 2) {  // level 2 (one parent per lane)
+    lc = L_NONE;
+    if (lane < T / 4) s.combine(a.d_in - 2, (base >> 2) + lane, l1[0], g1[0], l1[1], g1[1], lc, gc);
+  }
+  for (uint32_t r = 3; r <= R; r++) {  // level r: lanes that are multiples of 2^(r-2)
+    const int sh = 1 << (r - 3);
+    const uint8_t lp = (uint8_t)__shfl_down((int)lc, sh);
+    const Digest gp = shfl_down_dg(gc, sh);
+    const uint32_t j = lane >> (r - 2);
+    if ((lane & ((2u << (r - 3)) - 1)) == 0 && j < (T >> r)) {
+      uint8_t lo;
+      Digest go;
+      s.combine(a.d_in - r, (base >> r) + j, lc, gc, lp, gp, lo, go);
+      lc = lo;
+      gc = go;
+    }
+  }
+  const uint32_t wc = wave_sum(s.claims);
+  if (lane == 0) {
+    if (wc) atomicAdd(stripe(a.ctr, CTR_CLAIMS), (unsigned long long)wc);
+    a.out_lab[tile] = lc;
+    if (lc == L_FIRST) a.out_dg[tile] = gc;
   }
 }
 
-__device__ __forceinline__ bool is_region(uint8_t l) { return l == L_FIRST || l == L_SHIFT; }
+// Sweep B: shifted-duplicate forest and region roots (include/reference_impl.hpp
+// :83-106).  Read-only on the table.  Roots are staged per (depth, tile) in
+// ascending node id; a shifted root carries the table slot of its digest.
+template <int HK>
+struct SweepB {
+  const SweepArgs& a;
+  uint64_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;  // up to four roots per lane and level, in order
+  uint32_t ne = 0, nf = 0;
+
+  __device__ void emit(uint8_t l, uint32_t slot, uint64_t node) {
+    if (!is_region(l)) return;
+    const uint64_t v = ((l == L_FIRST ? kFirstTag : (uint64_t)slot) << 32) | (uint32_t)node;
+    e0 = ne == 0 ? v : e0;  // selects instead of an indexed array keep the entries in registers
+    e1 = ne == 1 ? v : e1;
+    e2 = ne == 2 ? v : e2;
+    e3 = ne == 3 ? v : e3;
+    ne++;
+    nf += (l == L_FIRST);
+  }
+
+  __device__ void combine(uint32_t depth, uint64_t pos, uint8_t la, const Digest& da, uint32_t sa, uint8_t lb,
+                          const Digest& db, uint32_t sb, uint8_t& lo, Digest& dout, uint32_t& sout) {
+    const uint64_t node = heap_id(depth, pos), cl = 2 * node + 1;
+    lo = L_DONE;
+    if (la != lb) {
+      emit(la, sa, cl);
+      emit(lb, sb, cl + 1);
+    } else if (la == L_FIXED) {
+      lo = L_FIXED;
+    } else if (la == L_SHIFT) {
+      const Digest d = hash_pair<HK>(da, db);
+      if (a.dbg_tree) a.dbg_tree[node] = d;
+      const uint32_t s = table_find(a.tab, a.mask, d);
+      if (s != ~0u) {
+        lo = L_SHIFT;
+        dout = d;
+        sout = s;
+      } else {
+        emit(la, sa, cl);
+        emit(lb, sb, cl + 1);
+      }
+    } else if (la == L_FIRST) {
+      lo = L_FIRST;
+    }
+    if (a.dbg_labels) a.dbg_labels[node] = lo;
+  }
+
+  // Stage this level's roots (children at depth c) for tile `tile`.
+  __device__ void flush(uint32_t c, uint64_t tile) {
+    uint32_t tot, ftot;
+    const uint32_t rank = wave_rank(ne, &tot);
+    wave_rank(nf, &ftot);
+    uint64_t* dst = a.staging + heap_id(c, tile << (c - a.d_out)) + rank;
+    if (ne > 0) dst[0] = e0;
+    if (ne > 1) dst[1] = e1;
+    if (ne > 2) dst[2] = e2;
+    if (ne > 3) dst[3] = e3;
+    if (lane_id() == 0) {
+      a.cnt_f[a.seg_base[c] + tile] = ftot;
+      a.cnt_s[a.seg_base[c] + tile] = tot - ftot;
+    }
+    ne = nf = 0;
+  }
+};
+
+template <int HK>
+__global__ __launch_bounds__(kThreads) void k_sweep_b(SweepArgs a) {
+  const uint32_t lane = lane_id();
+  const uint64_t tile = (uint64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (tile >= (1ull << a.d_out)) return;  // whole wavefront
+  const uint32_t R = a.d_in - a.d_out, T = 1u << R;
+  const uint64_t base = tile << R;
+  uint8_t l0[4];
+  Digest g0[4];
+  uint32_t s0[4];
+  bool busy = false;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const uint32_t lp = 4 * lane + i;
+    l0[i] = L_NONE;
+    if (lp >= T) continue;
+    const uint64_t p = base + lp;
+    if (a.stage0) {
+      if (p >= a.g.m) continue;
+      const uint64_t c = bottom_chunk(a.g, p);
+      l0[i] = a.labels[c];
+      if (l0[i] == L_SHIFT) {
+        g0[i] = a.leaf_dg[c];
+        s0[i] = a.leaf_slot[c];
+      }
+    } else {
+      l0[i] = a.in_lab[p];
+      if (l0[i] == L_SHIFT) {
+        g0[i] = a.in_dg[p];
+        s0[i] = a.in_slot[p];
+      }
+    }
+    busy |= (l0[i] != L_FIXED);
+  }
+  uint8_t lf[2] = {L_NONE, L_NONE};
+  Digest gf[2];
+  uint32_t sf[2];
+  if (a.stage0) {
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const uint32_t j = 2 * lane + k;
+      const uint64_t q = (base >> 1) + j;
+      if (j >= T / 2 || q < a.g.half) continue;
+      const uint64_t c = q - a.g.half;
+      lf[k] = a.labels[c];
+      if (lf[k] == L_SHIFT) {
+        gf[k] = a.leaf_dg[c];
+        sf[k] = a.leaf_slot[c];
+      }
+      busy |= (lf[k] != L_FIXED);
+    }
+  }
+  if (!__ballot(busy)) {
+    // every leaf below is a fixed duplicate: the whole tile is fixed, no roots
+    if (lane < R) {
+      const uint32_t c = a.d_out + 1 + lane;
+      a.cnt_f[a.seg_base[c] + tile] = 0;
+      a.cnt_s[a.seg_base[c] + tile] = 0;
+    }
+    if (lane == 0) {
+      a.out_lab[tile] = L_FIXED;
+      if (a.d_out == 0) {
+        a.cnt_f[a.seg_base[0]] = 0;
+        a.cnt_s[a.seg_base[0]] = 0;
+      }
+    }
+    if (a.dbg_labels)
+      for (uint32_t r = 1; r <= R; r++)
+        for (uint32_t j = lane; j < (T >> r); j += 64) {
+          const uint64_t q = (base >> r) + j;
+          if (a.stage0 && r == 1 && q >= a.g.half) continue;
+          a.dbg_labels[heap_id(a.d_in - r, q)] = L_FIXED;
+        }
+    return;
+  }
+  SweepB<HK> s{a};
+  uint8_t l1[2] = {L_NONE, L_NONE};
+  Digest g1[2];
+  uint32_t s1[2] = {0, 0};
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const uint32_t j = 2 * lane + k;
+    if (j >= T / 2) continue;
+    const uint64_t q = (base >> 1) + j;
+    if (a.stage0 && q >= a.g.half) {
+      l1[k] = lf[k];
+      g1[k] = gf[k];
+      s1[k] = sf[k];
+    } else {
+      s.combine(a.d_in - 1, q, l0[2 * k], g0[2 * k], s0[2 * k], l0[2 * k + 1], g0[2 * k + 1], s0[2 * k + 1], l1[k],
+                g1[k], s1[k]);
+    }
+  }
+  s.flush(a.d_in, tile);
+  uint8_t lc = l1[0];
+  Digest gc = g1[0];
+  uint32_t sc = s1[0];
+  if (R >= 2) {
+    lc = L_NONE;
+    if (lane < T / 4)
+      s.combine(a.d_in - 2, (base >> 2) + lane, l1[0], g1[0], s1[0], l1[1], g1[1], s1[1], lc, gc, sc);
+    s.flush(a.d_in - 1, tile);
+  }
+  for (uint32_t r = 3; r <= R; r++) {
+    const int sh = 1 << (r - 3);
+    const uint8_t lp = (uint8_t)__shfl_down((int)lc, sh);
+    const Digest gp = shfl_down_dg(gc, sh);
+    const uint32_t sp = __shfl_down(sc, sh);
+    const uint32_t j = lane >> (r - 2);
+    if ((lane & ((2u << (r - 3)) - 1)) == 0 && j < (T >> r)) {
+      uint8_t lo;
+      Digest go;
+      uint32_t so = 0;
+      s.combine(a.d_in - r, (base >> r) + j, lc, gc, sc, lp, gp, sp, lo, go, so);
+      lc = lo;
+      gc = go;
+      sc = so;
+    }
+    s.flush(a.d_in - r + 1, tile);
+  }
+  if (lane == 0) {
+    a.out_lab[tile] = lc;
+    if (lc == L_SHIFT) {
+      a.out_dg[tile] = gc;
+      a.out_slot[tile] = sc;
+    }
+    if (a.d_out == 0) {
+      // Node 0: a first-occurrence root (:105-106); a shifted root too, which the
+      // reference omits and so cannot restart (documented departure).
+      const bool root = is_region(lc);
+      if (root) a.staging[0] = ((lc == L_FIRST ? kFirstTag : (uint64_t)sc) << 32) | 0u;
+      a.cnt_f[a.seg_base[0]] = (root && lc == L_FIRST) ? 1u : 0u;
+      a.cnt_s[a.seg_base[0]] = (root && lc == L_SHIFT) ? 1u : 0u;
+    }
+  }
+}
 
 // Block-wide exclusive scan of one value per thread (256 threads).
 __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t* ws) {
@@ -351,181 +660,6 @@ __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t
   __syncthreads();
   *total = tot;
   return pre + x - v;
-}
-
-// Sweep B: shifted-duplicate forest and region roots (include/reference_impl.hpp
-// :83-106).  Read-only on the table.  Roots are staged per (depth, block) in
-// ascending node id; a shifted root carries the table slot of its digest.
-template <int HK>
-__global__ __launch_bounds__(kThreads) void k_sweep_b(SweepArgs a) {
-  __shared__ uint8_t s_lab[kTile];
-  __shared__ Digest s_dg[kTile];
-  __shared__ uint32_t s_slot[kTile];
-  __shared__ uint8_t s_lab2[kTile / 2];
-  __shared__ Digest s_dg2[kTile / 2];
-  __shared__ uint32_t s_slot2[kTile / 2];
-  __shared__ uint32_t s_ws[kThreads / 64];
-  const uint32_t T = 1u << (a.d_in - a.d_out);
-  const uint64_t base = (uint64_t)blockIdx.x * T;
-  const uint32_t tid = threadIdx.x;
-  int busy = 0;
-  for (uint32_t i = tid; i < T; i += kThreads) {
-    const uint64_t p = base + i;
-    uint8_t l = L_NONE;
-    if (a.stage0) {
-      if (p < a.g.m) {
-        const uint64_t c = bottom_chunk(a.g, p);
-        l = a.labels[c];
-        if (l == L_SHIFT) {
-          s_dg[i] = a.leaf_dg[c];
-          s_slot[i] = a.leaf_slot[c];
-        }
-      }
-    } else {
-      l = a.in_lab[p];
-      if (l == L_SHIFT) {
-        s_dg[i] = a.in_dg[p];
-        s_slot[i] = a.in_slot[p];
-      }
-    }
-    s_lab[i] = l;
-    busy |= (l != L_FIXED && l != L_NONE);
-  }
-  if (a.stage0) {
-    const uint64_t qb = base / 2;
-    for (uint32_t i = tid; i < T / 2; i += kThreads) {
-      const uint64_t q = qb + i;
-      uint8_t l = L_NONE;
-      if (q >= a.g.half) {
-        const uint64_t c = q - a.g.half;
-        l = a.labels[c];
-        if (l == L_SHIFT) {
-          s_dg2[i] = a.leaf_dg[c];
-          s_slot2[i] = a.leaf_slot[c];
-        }
-      }
-      s_lab2[i] = l;
-      busy |= (l != L_FIXED && l != L_NONE);
-    }
-  }
-  if (!__syncthreads_or(busy)) {  // every leaf below is a fixed duplicate
-    for (uint32_t c = a.d_out + 1 + tid; c <= a.d_in; c += kThreads) {
-      a.cnt_f[a.seg_base[c] + blockIdx.x] = 0;
-      a.cnt_s[a.seg_base[c] + blockIdx.x] = 0;
-    }
-    if (tid == 0) {
-      a.out_lab[blockIdx.x] = L_FIXED;
-      if (a.d_out == 0) {
-        a.cnt_f[a.seg_base[0]] = 0;
-        a.cnt_s[a.seg_base[0]] = 0;
-      }
-    }
-    if (a.dbg_labels) {  // all internal nodes of this block are fixed
-      for (uint32_t r = 1; r <= a.d_in - a.d_out; r++) {
-        const uint32_t lvl = a.d_in - r;
-        const uint32_t P = T >> r;
-        const uint64_t pb = base >> r;
-        for (uint32_t i = tid; i < P; i += kThreads) {
-          const uint64_t q = pb + i;
-          if (a.stage0 && r == 1 && q >= a.g.half) continue;
-          a.dbg_labels[((1ull << lvl) - 1) + q] = L_FIXED;
-        }
-      }
-    }
-    return;
-  }
-  for (uint32_t r = 1; r <= a.d_in - a.d_out; r++) {
-    const uint32_t lvl = a.d_in - r;
-    const uint32_t P = T >> r;
-    const uint64_t pb = base >> r;
-    const uint32_t per = (P + kThreads - 1) / kThreads;
-    uint8_t nl[2] = {L_NONE, L_NONE};
-    Digest nd[2];
-    uint32_t ns[2] = {0, 0};
-    uint64_t em[4];
-    uint32_t ne = 0, nf = 0;
-    for (uint32_t k = 0; k < per; k++) {
-      const uint32_t i = tid * per + k;
-      if (i >= P) break;
-      const uint64_t q = pb + i;
-      if (a.stage0 && r == 1 && q >= a.g.half) {  // a leaf at depth D-1
-        nl[k] = s_lab2[i];
-        nd[k] = s_dg2[i];
-        ns[k] = s_slot2[i];
-        continue;
-      }
-      const uint32_t node = (uint32_t)(((1ull << lvl) - 1) + q);
-      const uint64_t cl = 2ull * node + 1;
-      const uint8_t la = s_lab[2 * i], lb = s_lab[2 * i + 1];
-      uint8_t out = L_DONE;
-      if (la != lb) {
-        if (is_region(la)) {
-          em[ne++] = (la == L_FIRST ? (kFirstTag << 32) : ((uint64_t)s_slot[2 * i] << 32)) | (uint32_t)cl;
-          nf += (la == L_FIRST);
-        }
-        if (is_region(lb)) {
-          em[ne++] = (lb == L_FIRST ? (kFirstTag << 32) : ((uint64_t)s_slot[2 * i + 1] << 32)) | (uint32_t)(cl + 1);
-          nf += (lb == L_FIRST);
-        }
-      } else if (la == L_FIXED) {
-        out = L_FIXED;
-      } else if (la == L_SHIFT) {
-        const Digest d = hash_pair<HK>(s_dg[2 * i], s_dg[2 * i + 1]);
-        if (a.dbg_tree) a.dbg_tree[node] = d;
-        const uint32_t s = table_find(a.tab, a.mask, d);
-        if (s != ~0u) {
-          out = L_SHIFT;
-          nd[k] = d;
-          ns[k] = s;
-        } else {
-          em[ne++] = ((uint64_t)s_slot[2 * i] << 32) | (uint32_t)cl;
-          em[ne++] = ((uint64_t)s_slot[2 * i + 1] << 32) | (uint32_t)(cl + 1);
-        }
-      } else if (la == L_FIRST) {
-        out = L_FIRST;
-      }
-      nl[k] = out;
-      if (a.dbg_labels) a.dbg_labels[node] = out;
-    }
-    // Stage this level's roots (children at depth lvl+1) in ascending node order.
-    uint32_t tot;
-    const uint32_t rank = block_scan_excl(ne, &tot, s_ws);
-    uint32_t ftot;
-    block_scan_excl(nf, &ftot, s_ws);
-    const uint32_t c = lvl + 1;
-    const uint64_t sbase = ((1ull << c) - 1) + ((uint64_t)blockIdx.x << (c - a.d_out));
-    for (uint32_t e = 0; e < ne; e++) a.staging[sbase + rank + e] = em[e];
-    if (tid == 0) {
-      a.cnt_f[a.seg_base[c] + blockIdx.x] = ftot;
-      a.cnt_s[a.seg_base[c] + blockIdx.x] = tot - ftot;
-    }
-    for (uint32_t k = 0; k < per; k++) {
-      const uint32_t i = tid * per + k;
-      if (i >= P) break;
-      s_lab[i] = nl[k];
-      if (nl[k] == L_SHIFT) {
-        s_dg[i] = nd[k];
-        s_slot[i] = ns[k];
-      }
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    const uint8_t l = s_lab[0];
-    a.out_lab[blockIdx.x] = l;
-    if (l == L_SHIFT) {
-      a.out_dg[blockIdx.x] = s_dg[0];
-      a.out_slot[blockIdx.x] = s_slot[0];
-    }
-    if (a.d_out == 0) {
-      // Node 0: a first-occurrence root (:105-106); a shifted root too, which the
-      // reference omits and so cannot restart (documented departure).
-      const bool root = is_region(l);
-      if (root) a.staging[0] = (l == L_FIRST ? (kFirstTag << 32) : ((uint64_t)s_slot[0] << 32)) | 0u;
-      a.cnt_f[a.seg_base[0]] = (root && l == L_FIRST) ? 1u : 0u;
-      a.cnt_s[a.seg_base[0]] = (root && l == L_SHIFT) ? 1u : 0u;
-    }
-  }
 }
 
 // A single chunk: the leaf is the root.
@@ -569,29 +703,42 @@ __global__ void k_scatter_roots(const uint64_t* staging, const uint32_t* cnt_f, 
 // Shifted entries: source checkpoint and previous node from the table
 // (src/tree_approach.cpp:574-580,670-696), then a stable sort by source.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kLdsBins = 4096;  // source-checkpoint bins counted in LDS
+
 __global__ __launch_bounds__(kThreads) void k_shift_src(const uint64_t* shift_tmp, uint32_t S, const Slot* tab,
                                                         uint64_t mask, uint32_t ref, uint32_t cur, uint32_t* key,
                                                         uint64_t* val, uint32_t* src_hist, uint32_t nbins, Ctl* ctl) {
+  __shared__ uint32_t h[kLdsBins];
+  const bool lds = nbins <= kLdsBins;
+  if (lds)
+    for (uint32_t b = threadIdx.x; b < nbins; b += kThreads) h[b] = 0;
+  __syncthreads();
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= S) return;
-  const uint64_t e = shift_tmp[i];
-  const uint32_t node = (uint32_t)e, slot = (uint32_t)(e >> 32);
-  if (slot > mask + 1) {
-    atomicOr(&ctl->err, ERR_CORRUPT);
-    key[i] = 0;
-    val[i] = node;
-    return;
+  if (i < S) {
+    const uint64_t e = shift_tmp[i];
+    const uint32_t node = (uint32_t)e, slot = (uint32_t)(e >> 32);
+    uint32_t k = 0, prev = 0;
+    if (slot > mask + 1) {
+      atomicOr(&ctl->err, ERR_CORRUPT);
+    } else {
+      const uint64_t v = tab[slot].val;
+      const uint32_t src = (uint32_t)(v >> 32);
+      prev = (uint32_t)v;
+      k = src - ref;
+      if (src < ref || src > cur) {
+        atomicOr(&ctl->err, ERR_CORRUPT);
+        k = 0;
+      }
+    }
+    key[i] = k;
+    val[i] = ((uint64_t)prev << 32) | node;
+    if (lds) atomicAdd(&h[k], 1u);
+    else atomicAdd(&src_hist[k], 1u);
   }
-  const uint64_t v = tab[slot].val;
-  const uint32_t src = (uint32_t)(v >> 32), prev = (uint32_t)v;
-  uint32_t k = src - ref;
-  if (src < ref || src > cur) {
-    atomicOr(&ctl->err, ERR_CORRUPT);
-    k = 0;
-  }
-  key[i] = k;
-  val[i] = ((uint64_t)prev << 32) | node;
-  if (k < nbins) atomicAdd(&src_hist[k], 1u);
+  __syncthreads();
+  if (lds)
+    for (uint32_t b = threadIdx.x; b < nbins; b += kThreads)
+      if (h[b]) atomicAdd(&src_hist[b], h[b]);
 }
 
 // Histogram of one 8-bit digit over a 256-entry segment.
@@ -906,11 +1053,13 @@ static void ensure_table(mtc_dedup* h, bool reset, hipStream_t st) {
     fill_table(h->table.p, cap + 1, st);
     return;
   }
-  // Every checkpoint can add at most 2n-1 entries; keep room for all of them
-  // (the reference rehashes to size + num_nodes, src/tree_approach.cpp:1622-1623).
-  if (h->table_size + nodes <= h->table_cap * 9 / 10) return;
+  // Every checkpoint can add at most 2n-1 entries; keep room for all of them, so
+  // probing always ends (the reference rehashes to size + num_nodes,
+  // src/tree_approach.cpp:1622-1623), and grow early once the table is 80% full.
+  auto fits = [&](uint64_t cap) { return h->table_size + nodes < cap && h->table_size <= cap / 10 * 8; };
+  if (fits(h->table_cap)) return;
   uint64_t cap = h->table_cap;
-  while (h->table_size + nodes > cap * 9 / 10) cap *= 2;
+  while (!fits(cap)) cap *= 2;
   if (cap > (1ull << 31)) throw Status{MTC_ERR_TABLE_FULL, "table capacity above 2^31 slots"};
   h->table_tmp.ensure(cap + 1);
   fill_table(h->table_tmp.p, cap + 1, st);
@@ -954,7 +1103,7 @@ static void run_sweeps(mtc_dedup* h, SweepArgs sa, hipStream_t st) {
       a.out_lab = h->fr_lab.p + out_off;
       a.out_dg = h->fr_dg.p + out_off;
       a.out_slot = h->fr_slot.p + out_off;
-      const uint32_t blocks = 1u << a.d_out;
+      const uint32_t blocks = (uint32_t)(((1ull << a.d_out) + 3) / 4);  // four wavefront tiles per block
       if (pass == 0) k_sweep_a<HK><<<blocks, kThreads, 0, st>>>(a);
       else k_sweep_b<HK><<<blocks, kThreads, 0, st>>>(a);
       MTC_HIP(hipGetLastError());
@@ -984,7 +1133,8 @@ void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_
     plan_tree(h);
   }
   ensure_buffers(h, st);
-  k_zero_ctl<<<1, 64, 0, st>>>(h->ctl_d);
+  h->ctr.ensure(CTR_KINDS * kStripes);
+  k_zero_ctl<<<1, kThreads, 0, st>>>(h->ctl_d, h->ctr.p);
   ensure_table(h, baseline, st);
   if (baseline) h->baseline_id = h->current_id;
   h->has_state = true;
@@ -999,7 +1149,7 @@ void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_
   MTC_HIP(hipEventRecord(ev[1], st));
 
   LeafArgs la{data, len, n, h->chunk_size, h->leaf_dg.p, h->labels.p, h->leaf_slot.p, h->table.p,
-              h->table_cap - 1, cur, baseline ? 1 : 0, h->ctl_d};
+              h->table_cap - 1, cur, baseline ? 1 : 0, h->ctl_d, h->ctr.p};
   const bool fast = (C % 16 == 0) && ((uintptr_t)data % 16 == 0);
   if (h->hash_kind == HASH_MD5) launch_leaf<HASH_MD5>(la, fast, st);
   else launch_leaf<HASH_MURMUR3>(la, fast, st);
@@ -1015,6 +1165,7 @@ void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_
   sa.tab = h->table.p;
   sa.mask = h->table_cap - 1;
   sa.ctl = h->ctl_d;
+  sa.ctr = h->ctr.p;
   sa.staging = h->staging.p;
   sa.cnt_f = h->seg_cnt_f.p;
   sa.cnt_s = h->seg_cnt_s.p;
@@ -1105,6 +1256,7 @@ void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_
     k_gather<<<grid_for(n, kThreads, 8192), kThreads, 0, st>>>(da);
     MTC_HIP(hipGetLastError());
   }
+  k_reduce_ctr<<<1, kStripes, 0, st>>>(h->ctr.p, cd);
   MTC_HIP(hipEventRecord(ev[3], st));
   sync_ctl(h, st);
   h->table_size += h->ctl_h->claims;

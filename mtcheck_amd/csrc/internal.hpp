@@ -125,6 +125,7 @@ struct mtc_dedup {
   mtc::DevBuf<uint8_t> diff;          // device diff of the last checkpoint
   mtc::DevBuf<mtc::Digest> dbg_tree;  // debug: full tree, as the serial reference leaves it
   mtc::DevBuf<uint8_t> dbg_labels;    // debug: all node labels
+  mtc::DevBuf<unsigned long long> ctr;  // striped per-checkpoint counters
   mtc::Ctl* ctl_d = nullptr;
   mtc::Ctl* ctl_h = nullptr;          // pinned
   mtc::Workspace ws;

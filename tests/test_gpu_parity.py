@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 
 def dev(a: np.ndarray):
-    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+    return torch.from_numpy(np.array(a, dtype=np.uint8, copy=True)).to("cuda:0")
 
 
 def say(msg):
@@ -215,7 +215,7 @@ def test_errors_are_reported():
     with pytest.raises(M.MtcError) as e:
         d.checkpoint(torch.zeros(2048, dtype=torch.uint8, device="cuda:0"))
     assert e.value.code == 7
-    good = d.checkpoint(a, make_baseline=True)
+    good = M.TreeDeduplicator(64).checkpoint(a, make_baseline=True)
     out = torch.zeros(4096, dtype=torch.uint8, device="cuda:0")
     with pytest.raises(M.MtcError) as e:  # truncated diff
         M.restart_into(out, [good[:40]], 0)
