@@ -96,6 +96,8 @@ int mtc_destroy(mtc_dedup* h) {
   (void)hipSetDevice(h->device);
   if (h->ctl_d) (void)hipFree(h->ctl_d);
   if (h->ctl_h) (void)hipHostFree(h->ctl_h);
+  for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
   delete h;  // device buffers are released by their destructors
   return MTC_OK;
 }

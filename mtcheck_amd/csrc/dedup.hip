@@ -5,12 +5,17 @@
 // roots) and src/tree_approach.cpp:511-710 (diff layout), in the canonical order
 // of the serial reference (roots ascending; shifted entries grouped by source
 // checkpoint, ascending node id inside a group).
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+
 #include "internal.hpp"
 
 namespace mtc {
 
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kFuse = 8;              // tree levels fused per sweep stage (one wavefront tile)
+constexpr uint64_t kLeafBlocks = 1ull << 40; // leaf-kernel blocks (default: one 256-chunk tile each)
 constexpr uint64_t kFirstTag = 0xFFFFFFFFull;  // staging tag of a first-occurrence root
 
 // ---------------------------------------------------------------------------
@@ -35,7 +40,7 @@ __device__ __forceinline__ unsigned long long* stripe(unsigned long long* ctr, u
   return ctr + kind * kStripes + (blockIdx.x & (kStripes - 1));
 }
 
-__global__ void k_reduce_ctr(const unsigned long long* ctr, Ctl* c) {
+__global__ void k_reduce_ctr(const unsigned long long* ctr, const uint32_t* changed, Ctl* c) {
   __shared__ unsigned long long s[CTR_KINDS];
   if (threadIdx.x < CTR_KINDS) s[threadIdx.x] = 0;
   __syncthreads();
@@ -46,7 +51,7 @@ __global__ void k_reduce_ctr(const unsigned long long* ctr, Ctl* c) {
   __syncthreads();
   if (threadIdx.x == 0) {
     c->claims = s[CTR_CLAIMS];
-    c->changed = s[CTR_CHANGED];
+    c->changed = *changed;
   }
 }
 
@@ -165,21 +170,19 @@ struct LeafArgs {
   uint32_t C;
   Digest* leaf_dg;
   uint8_t* labels;
-  uint32_t* leaf_slot;
-  Slot* tab;
-  uint64_t mask;
-  uint32_t cur;
   int baseline;
-  Ctl* ctl;
-  unsigned long long* ctr;
+  uint32_t* list;     // chunks to insert, packed per block
+  uint32_t* blk_cnt;  // their count per block
+  uint64_t tiles_per_block;
 };
 
-// Digest of chunk blockIdx.x*256 + threadIdx.x (zero beyond n).  CB > 0 selects
-// the LDS-staged path for 16-byte-aligned data and chunk sizes that are multiples
-// of 16; a short last chunk and CB == 0 use byte-granular hashing.
+// Digest of chunk tile*256 + threadIdx.x (zero beyond n).  CB > 0 selects the
+// LDS-staged path for 16-byte-aligned data and chunk sizes that are multiples of
+// 16; a short last chunk and CB == 0 use byte-granular hashing.
 template <int HK, int CB>
-__device__ __forceinline__ Digest tile_digest(const uint8_t* data, uint64_t len, uint32_t n, uint32_t C) {
-  const uint64_t first = (uint64_t)blockIdx.x * kThreads;
+__device__ __forceinline__ Digest tile_digest(const uint8_t* data, uint64_t len, uint32_t n, uint32_t C,
+                                              uint64_t tile) {
+  const uint64_t first = tile * kThreads;
   const uint64_t c = first + threadIdx.x;
   Digest d{0, 0};
   if constexpr (CB > 0) {
@@ -196,43 +199,238 @@ __device__ __forceinline__ Digest tile_digest(const uint8_t* data, uint64_t len,
   return d;
 }
 
-template <int HK, int CB>
-__global__ __launch_bounds__(kThreads) void k_leaf(LeafArgs a) {
-  const uint64_t c = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+// Fixed test, label and packing of one tile's chunks (chunk t*256 + threadIdx.x
+// has digest d).  Chunks to insert go to this block's region, ascending.
+__device__ __forceinline__ void leaf_epilogue(const LeafArgs& a, uint64_t t, const Digest& d, uint32_t* region,
+                                              uint32_t& run, uint32_t* s_wcnt) {
+  const uint64_t c = t * kThreads + threadIdx.x;
   const bool valid = c < a.n;
-  const Digest d = tile_digest<HK, CB>(a.data, a.len, a.n, a.C);
   Digest prev{0, 0};
   if (valid && !a.baseline) prev = a.leaf_dg[c];
   const bool fixed = valid && d == prev;
   // A fixed leaf's digest was inserted by an earlier checkpoint of this baseline,
   // so its insert is a no-op except right after a reset, where every leaf inserts.
   const bool ins = valid && (a.baseline || !fixed);
-  // Lanes holding the same digest as the lane before them (zero-filled runs) reuse
-  // that lane's table slot: the lowest leaf of a run carries the minimum anyway.
-  const uint32_t lane = lane_id();
-  uint64_t plo = __shfl_up(d.lo, 1), phi = __shfl_up(d.hi, 1);
-  int pins = __shfl_up((int)ins, 1);
-  const bool follower = ins && lane > 0 && pins && plo == d.lo && phi == d.hi;
-  const bool head = ins && !follower;
-  uint32_t slot = ~0u;
-  bool claimed = false;
-  if (head) {
-    const uint64_t v = ((uint64_t)a.cur << 32) | (uint32_t)(a.n - 1 + c);
-    slot = table_insert(a.tab, a.mask, d, v, true, &a.ctl->err, &claimed);
-  }
-  const uint64_t heads = __ballot(head);
-  const uint64_t le = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1);
-  const uint64_t mine = heads & le;
-  const int src_lane = mine ? 63 - __clzll((long long)mine) : (int)lane;
-  const uint32_t hslot = __shfl(slot, src_lane);
-  if (follower) slot = hslot;
-  wave_count_add(stripe(a.ctr, CTR_CLAIMS), claimed);
-  wave_count_add(stripe(a.ctr, CTR_CHANGED), valid && !fixed);
   if (valid) {
     a.labels[c] = fixed ? L_FIXED : L_PENDING;
-    if (ins) {
-      a.leaf_dg[c] = d;
-      a.leaf_slot[c] = slot;
+    if (ins) a.leaf_dg[c] = d;
+  }
+  const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  const uint64_t m = __ballot(ins);
+  if (lane == 0) s_wcnt[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t base = run, tot = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kThreads / 64; i++) {
+    base += (i < w) ? s_wcnt[i] : 0u;
+    tot += s_wcnt[i];
+  }
+  if (ins) region[base + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)c;
+  run += tot;
+  __syncthreads();  // s_wcnt is reused by the next tile
+}
+
+// One block streams a contiguous range of tiles (256 chunks each), software
+// pipelined: the 16-byte loads of the next (tile, segment) are in flight while the
+// current one is hashed out of LDS.  Chunks to insert are packed into the block's
+// region of the list; the table work runs densely afterwards.
+template <int HK, int CB>
+__global__ __launch_bounds__(kThreads) void k_leaf(LeafArgs a) {
+  __shared__ uint32_t s_wcnt[kThreads / 64];
+  const uint64_t ntiles = (a.n + kThreads - 1) / kThreads;
+  const uint64_t t0 = (uint64_t)blockIdx.x * a.tiles_per_block;
+  const uint64_t t1 = std::min<uint64_t>(t0 + a.tiles_per_block, ntiles);
+  uint32_t* region = a.list + t0 * kThreads;
+  uint32_t run = 0;
+  if constexpr (CB == 0) {
+    for (uint64_t t = t0; t < t1; t++)
+      leaf_epilogue(a, t, tile_digest<HK, 0>(a.data, a.len, a.n, a.C, t), region, run, s_wcnt);
+  } else {
+    using S = TileShape<CB>;
+    __shared__ __align__(16) uint8_t lds[S::LDS];
+    const uint64_t full = a.len / CB;
+    const uint64_t steps = (t1 - t0) * S::PASSES;
+    const uint32_t tid = threadIdx.x;
+    auto load = [&](uint64_t step, uint4* v) {
+      const uint64_t t = t0 + step / S::PASSES;
+      const uint32_t pass = (uint32_t)(step % S::PASSES);
+#pragma unroll
+      for (int i = 0; i < S::LOADS; i++) {
+        const uint32_t u = i * (kThreads * 16) + 16 * tid;
+        const uint32_t j = u / S::SEG, w = u % S::SEG;
+        const uint64_t chunk = t * kThreads + j;
+        v[i] = uint4{0, 0, 0, 0};
+        if (chunk < full) v[i] = *reinterpret_cast<const uint4*>(a.data + chunk * CB + pass * S::SEG + w);
+      }
+    };
+    uint4 v[S::LOADS];
+    if (steps) load(0, v);
+    uint64_t h1 = 0, h2 = 0;
+    uint32_t st[4], X[16];
+    for (uint64_t step = 0; step < steps; step++) {
+      uint4 nv[S::LOADS];
+      if (step + 1 < steps) load(step + 1, nv);
+      __syncthreads();  // the previous segment's LDS reads are done
+#pragma unroll
+      for (int i = 0; i < S::LOADS; i++) {
+        const uint32_t u = i * (kThreads * 16) + 16 * tid;
+        *reinterpret_cast<uint4*>(lds + (u / S::SEG) * S::PITCH + u % S::SEG) = v[i];
+      }
+      __syncthreads();
+      const uint32_t pass = (uint32_t)(step % S::PASSES);
+      if (pass == 0) {
+        h1 = h2 = 0;
+        md5_init(st);
+      }
+      const uint8_t* mine = lds + tid * S::PITCH;
+      if constexpr (HK == HASH_MD5) {
+#pragma unroll
+        for (int q = 0; q < S::LOADS; q++) {
+          const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * q);
+          const int wq = q & 3;  // LOADS is a multiple of 4 whenever PASSES > 1
+          X[4 * wq] = b.x;
+          X[4 * wq + 1] = b.y;
+          X[4 * wq + 2] = b.z;
+          X[4 * wq + 3] = b.w;
+          if (wq == 3) md5_compress(st, X);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < S::LOADS; q++) {
+          const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * q);
+          m3_block(h1, h2, (uint64_t)b.x | ((uint64_t)b.y << 32), (uint64_t)b.z | ((uint64_t)b.w << 32));
+        }
+      }
+      if (pass == S::PASSES - 1) {
+        const uint64_t t = t0 + step / S::PASSES;
+        const uint64_t c = t * kThreads + tid;
+        Digest d;
+        if constexpr (HK == HASH_MD5) {
+          constexpr int RW = (CB % 64) / 16;
+          uint32_t T[16];
+#pragma unroll
+          for (int w = 0; w < 16; w++) T[w] = (w < 4 * RW) ? X[w] : 0u;
+          md5_finish_words(st, T, (uint32_t)(CB % 64), (uint64_t)CB);
+          d = md5_digest(st);
+        } else {
+          d = m3_final(h1, h2, CB);
+        }
+        if (c >= full) d = (c < a.n) ? hash_bytes<HK>(a.data + c * CB, a.len - c * CB) : Digest{0, 0};
+        leaf_epilogue(a, t, d, region, run, s_wcnt);
+      }
+#pragma unroll
+      for (int i = 0; i < S::LOADS; i++) v[i] = nv[i];
+    }
+  }
+  if (threadIdx.x == 0) a.blk_cnt[blockIdx.x] = run;
+}
+
+// Concatenate the per-block regions into the dense list of chunks to insert.
+__global__ __launch_bounds__(kThreads) void k_compact(const uint32_t* list, const uint32_t* blk_cnt,
+                                                      const uint32_t* blk_off, uint64_t tiles_per_block,
+                                                      uint32_t* dense) {
+  const uint32_t b = blockIdx.x, cnt = blk_cnt[b];
+  const uint32_t* src = list + (uint64_t)b * tiles_per_block * kThreads;
+  uint32_t* dst = dense + blk_off[b];
+  for (uint32_t i = threadIdx.x; i < cnt; i += kThreads) dst[i] = src[i];
+}
+
+struct DenseArgs {
+  const uint32_t* dense;  // chunks to insert, ascending
+  const uint32_t* total;  // their number
+  uint32_t n;
+  const Digest* leaf_dg;
+  uint32_t* leaf_slot;
+  uint8_t* labels;
+  Slot* tab;
+  uint64_t mask;
+  uint32_t cur;
+  Ctl* ctl;
+  unsigned long long* ctr;
+  Geo g;
+  uint32_t R0;          // levels of the first sweep stage
+  uint8_t* tile_first;  // per stage-0 tile: holds a first-occurrence leaf
+  uint8_t* tile_busy;   // per stage-0 tile: holds a leaf that is not a fixed duplicate
+};
+
+// Stage-0 sweep tile of a chunk: bottom-level chunks by position, depth D-1 leaves
+// by their position one level up.
+__device__ __forceinline__ uint64_t chunk_tile(const Geo& g, uint32_t R0, uint64_t c) {
+  const uint64_t lead = (1ull << g.D) - g.n;  // chunks stored at depth D-1
+  if (c >= lead) return (c - lead) >> R0;
+  return (c + g.half) >> (R0 - 1);
+}
+
+// Read-only pass before any insert of this checkpoint: a digest already in the
+// table belongs to an earlier checkpoint, so its leaf is a shifted duplicate and
+// needs no insert (include/reference_impl.hpp:51,66-67).  Plain loads, so popular
+// digests (zero chunks) are served from cache.
+__global__ __launch_bounds__(kThreads) void k_lookup(DenseArgs a) {
+  const uint64_t total = *a.total;
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < total; i += (uint64_t)gridDim.x * kThreads) {
+    const uint32_t c = a.dense[i];
+    const uint32_t s = table_find(a.tab, a.mask, a.leaf_dg[c]);
+    if (s == ~0u) continue;
+    a.leaf_slot[c] = s;
+    if (a.labels[c] == L_PENDING) {
+      a.labels[c] = L_SHIFT;
+      if (a.R0) a.tile_busy[chunk_tile(a.g, a.R0, c)] = 1;
+    }
+  }
+}
+
+// Table inserts (src/tree_approach.cpp:320-349; include/reference_impl.hpp:51): the
+// packed value (cur << 32 | leaf) with atomicMin leaves the lowest leaf of a new
+// digest in the table.  Consecutive entries with the same digest (zero-filled
+// runs) share the first one's table access.
+__global__ __launch_bounds__(kThreads) void k_insert(DenseArgs a) {
+  const uint64_t total = *a.total;
+  const uint32_t lane = lane_id();
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  uint32_t claims = 0;
+  for (uint64_t wbase = (uint64_t)blockIdx.x * kThreads + (threadIdx.x & ~63u); wbase < total; wbase += stride) {
+    const uint64_t i = wbase + lane;
+    bool v = i < total;
+    uint32_t c = 0;
+    Digest d{0, 0};
+    if (v) {
+      c = a.dense[i];
+      v = a.labels[c] != L_SHIFT;  // found by k_lookup: nothing to insert
+      if (v) d = a.leaf_dg[c];
+    }
+    const uint64_t plo = __shfl_up(d.lo, 1), phi = __shfl_up(d.hi, 1);
+    const int pv = __shfl_up((int)v, 1);
+    const bool follower = v && lane > 0 && pv && plo == d.lo && phi == d.hi;
+    const bool head = v && !follower;
+    uint32_t slot = ~0u;
+    bool claimed = false;
+    if (head) slot = table_insert(a.tab, a.mask, d, ((uint64_t)a.cur << 32) | (uint32_t)(a.n - 1 + c), true,
+                                  &a.ctl->err, &claimed, true);
+    const uint64_t mine = __ballot(head) & ((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
+    const uint32_t hslot = __shfl(slot, mine ? 63 - __clzll((long long)mine) : (int)lane);
+    if (follower) slot = hslot;
+    if (v) a.leaf_slot[c] = slot;
+    claims += claimed;
+  }
+  const uint32_t wc = wave_sum(claims);
+  if (lane == 0 && wc) atomicAdd(stripe(a.ctr, CTR_CLAIMS), (unsigned long long)wc);
+}
+
+// Once every insert has landed: a pending leaf is FIRST when its digest's entry is
+// (cur, itself), SHIFT otherwise (include/reference_impl.hpp:52-69).
+__global__ __launch_bounds__(kThreads) void k_resolve(DenseArgs a) {
+  const uint64_t total = *a.total;
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < total; i += (uint64_t)gridDim.x * kThreads) {
+    const uint32_t c = a.dense[i];
+    if (a.labels[c] != L_PENDING) continue;  // a fixed leaf right after a reset
+    const uint32_t s = a.leaf_slot[c];
+    uint8_t l = L_SHIFT;
+    if (s <= a.mask + 1) l = (a.tab[s].val == (((uint64_t)a.cur << 32) | (uint32_t)(a.n - 1 + c))) ? L_FIRST : L_SHIFT;
+    a.labels[c] = l;
+    if (a.R0) {
+      const uint64_t t = chunk_tile(a.g, a.R0, c);
+      a.tile_busy[t] = 1;
+      if (l == L_FIRST) a.tile_first[t] = 1;
     }
   }
 }
@@ -242,7 +440,7 @@ template <int HK, int CB>
 __global__ __launch_bounds__(kThreads) void k_hash_chunks(const uint8_t* data, uint64_t len, uint32_t n, uint32_t C,
                                                           Digest* out) {
   const uint64_t c = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  const Digest d = tile_digest<HK, CB>(data, len, n, C);
+  const Digest d = tile_digest<HK, CB>(data, len, n, C, blockIdx.x);
   if (c < n) out[c] = d;
 }
 
@@ -279,17 +477,9 @@ struct SweepArgs {
   const uint32_t* seg_base;
   Digest* dbg_tree;
   uint8_t* dbg_labels;
+  const uint8_t* tile_first;  // stage 0: tile holds a first-occurrence leaf
+  const uint8_t* tile_busy;   // stage 0: tile holds a leaf that is not a fixed duplicate
 };
-
-// FIRST if this leaf holds the table entry of its digest (lowest leaf of a digest
-// new to this checkpoint), SHIFT otherwise.
-__device__ __forceinline__ uint8_t resolve_leaf(const SweepArgs& a, uint64_t c) {
-  const uint32_t s = a.leaf_slot[c];
-  if (s > a.mask + 1) return L_SHIFT;  // insert failed; error already raised
-  const uint64_t v = a.tab[s].val;
-  const uint64_t mine = ((uint64_t)a.cur << 32) | (uint32_t)(a.g.n - 1 + c);
-  return v == mine ? L_FIRST : L_SHIFT;
-}
 
 __device__ __forceinline__ uint64_t bottom_chunk(const Geo& g, uint64_t p) { return (1ull << g.D) - g.n + p; }
 
@@ -322,20 +512,47 @@ __device__ __forceinline__ Digest shfl_down_dg(const Digest& d, int s) {
 template <int HK>
 struct SweepA {
   const SweepArgs& a;
-  uint32_t claims = 0;
+  Digest* rec_dg;      // this wavefront's LDS record of nodes to insert (256 entries)
+  uint32_t* rec_node;
+  uint32_t count = 0;  // entries recorded so far (wave-uniform)
+  Digest d0, d1;       // this lane's records of the current level
+  uint32_t n0 = 0, n1 = 0, nrec = 0;
 
+  // Both children first occurrences: hash the pair and record the node; the
+  // wavefront inserts all its records together once the tile is done (they only
+  // have to land before sweep B looks anything up).
   __device__ void combine(uint32_t depth, uint64_t pos, uint8_t la, const Digest& da, uint8_t lb, const Digest& db,
                           uint8_t& lo, Digest& dout) {
     lo = L_NONE;
     if (la == L_FIRST && lb == L_FIRST) {
       dout = hash_pair<HK>(da, db);
       const uint32_t node = (uint32_t)heap_id(depth, pos);
-      bool cl = false;
-      table_insert(a.tab, a.mask, dout, ((uint64_t)a.cur << 32) | node, false, &a.ctl->err, &cl);
-      claims += cl;
+      if (nrec == 0) {
+        d0 = dout;
+        n0 = node;
+      } else {
+        d1 = dout;
+        n1 = node;
+      }
+      nrec++;
       lo = L_FIRST;
       if (a.dbg_tree) a.dbg_tree[node] = dout;
     }
+  }
+
+  __device__ void flush() {
+    uint32_t tot;
+    const uint32_t r = wave_rank(nrec, &tot);
+    if (nrec > 0) {
+      rec_dg[count + r] = d0;
+      rec_node[count + r] = n0;
+    }
+    if (nrec > 1) {
+      rec_dg[count + r + 1] = d1;
+      rec_node[count + r + 1] = n1;
+    }
+    count += tot;
+    nrec = 0;
   }
 };
 
@@ -346,6 +563,10 @@ __global__ __launch_bounds__(kThreads) void k_sweep_a(SweepArgs a) {
   if (tile >= (1ull << a.d_out)) return;  // whole wavefront
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
   const uint64_t base = tile << R;
+  if (a.stage0 && !a.tile_first[tile]) {  // no first-occurrence leaf below this tile
+    if (lane == 0) a.out_lab[tile] = L_NONE;
+    return;
+  }
   uint8_t l0[4];
   Digest g0[4];
   bool any = false;
@@ -358,13 +579,8 @@ __global__ __launch_bounds__(kThreads) void k_sweep_a(SweepArgs a) {
     if (a.stage0) {
       if (p >= a.g.m) continue;
       const uint64_t c = bottom_chunk(a.g, p);
-      uint8_t l = a.labels[c];
-      if (l == L_PENDING) {
-        l = resolve_leaf(a, c);
-        a.labels[c] = l;
-      }
-      l0[i] = l;
-      if (l == L_FIRST) g0[i] = a.leaf_dg[c];
+      l0[i] = a.labels[c];
+      if (l0[i] == L_FIRST) g0[i] = a.leaf_dg[c];
     } else {
       l0[i] = a.in_lab[p];
       if (l0[i] == L_FIRST) g0[i] = a.in_dg[p];
@@ -381,21 +597,19 @@ __global__ __launch_bounds__(kThreads) void k_sweep_a(SweepArgs a) {
       const uint64_t q = (base >> 1) + j;
       if (j >= T / 2 || q < a.g.half) continue;
       const uint64_t c = q - a.g.half;
-      uint8_t l = a.labels[c];
-      if (l == L_PENDING) {
-        l = resolve_leaf(a, c);
-        a.labels[c] = l;
-      }
-      lf[k] = l;
-      if (l == L_FIRST) gf[k] = a.leaf_dg[c];
-      any |= (l == L_FIRST);
+      lf[k] = a.labels[c];
+      if (lf[k] == L_FIRST) gf[k] = a.leaf_dg[c];
+      any |= (lf[k] == L_FIRST);
     }
   }
   if (!__ballot(any)) {
     if (lane == 0) a.out_lab[tile] = L_NONE;
     return;
   }
-  SweepA<HK> s{a};
+  __shared__ Digest rec_dg[kThreads / 64][256];
+  __shared__ uint32_t rec_node[kThreads / 64][256];
+  const uint32_t w = threadIdx.x >> 6;
+  SweepA<HK> s{a, rec_dg[w], rec_node[w]};
   // level 1 (two parents per lane)
   uint8_t l1[2] = {L_NONE, L_NONE};
   Digest g1[2];
@@ -411,11 +625,13 @@ __global__ __launch_bounds__(kThreads) void k_sweep_a(SweepArgs a) {
       s.combine(a.d_in - 1, q, l0[2 * k], g0[2 * k], l0[2 * k + 1], g0[2 * k + 1], l1[k], g1[k]);
     }
   }
+  s.flush();
   uint8_t lc = l1[0];
   Digest gc = g1[0];
   if (R >= 2) {  // level 2 (one parent per lane)
     lc = L_NONE;
     if (lane < T / 4) s.combine(a.d_in - 2, (base >> 2) + lane, l1[0], g1[0], l1[1], g1[1], lc, gc);
+    s.flush();
   }
   for (uint32_t r = 3; r <= R; r++) {  // level r: lanes that are multiples of 2^(r-2)
     const int sh = 1 << (r - 3);
@@ -429,13 +645,24 @@ __global__ __launch_bounds__(kThreads) void k_sweep_a(SweepArgs a) {
       lc = lo;
       gc = go;
     }
+    s.flush();
   }
-  const uint32_t wc = wave_sum(s.claims);
   if (lane == 0) {
-    if (wc) atomicAdd(stripe(a.ctr, CTR_CLAIMS), (unsigned long long)wc);
     a.out_lab[tile] = lc;
     if (lc == L_FIRST) a.out_dg[tile] = gc;
   }
+  // Insert-if-absent of this tile's first-occurrence nodes (reference_impl.hpp:80).
+  // LDS accesses of one wavefront execute in order, so the records are visible.
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  uint32_t claims = 0;
+  for (uint32_t i = lane; i < s.count; i += 64) {
+    bool cl = false;
+    table_insert(a.tab, a.mask, rec_dg[w][i], ((uint64_t)a.cur << 32) | rec_node[w][i], false, &a.ctl->err, &cl,
+                 true);
+    claims += cl;
+  }
+  const uint32_t wc = wave_sum(claims);
+  if (lane == 0 && wc) atomicAdd(stripe(a.ctr, CTR_CLAIMS), (unsigned long long)wc);
 }
 
 // Sweep B: shifted-duplicate forest and region roots (include/reference_impl.hpp
@@ -495,13 +722,25 @@ struct SweepB {
     if (ne > 1) dst[1] = e1;
     if (ne > 2) dst[2] = e2;
     if (ne > 3) dst[3] = e3;
-    if (lane_id() == 0) {
+    if (lane_id() == 0 && tot) {  // counts are zero-filled before the sweep
       a.cnt_f[a.seg_base[c] + tile] = ftot;
       a.cnt_s[a.seg_base[c] + tile] = tot - ftot;
     }
     ne = nf = 0;
   }
 };
+
+// Every leaf below the tile is a fixed duplicate: the tile is fixed and emits no roots.
+__device__ void fixed_tile(const SweepArgs& a, uint64_t tile, uint32_t R, uint32_t T, uint64_t base, uint32_t lane) {
+  if (lane == 0) a.out_lab[tile] = L_FIXED;  // root counts stay zero
+  if (a.dbg_labels)
+    for (uint32_t r = 1; r <= R; r++)
+      for (uint32_t j = lane; j < (T >> r); j += 64) {
+        const uint64_t q = (base >> r) + j;
+        if (a.stage0 && r == 1 && q >= a.g.half) continue;
+        a.dbg_labels[heap_id(a.d_in - r, q)] = L_FIXED;
+      }
+}
 
 template <int HK>
 __global__ __launch_bounds__(kThreads) void k_sweep_b(SweepArgs a) {
@@ -510,6 +749,10 @@ __global__ __launch_bounds__(kThreads) void k_sweep_b(SweepArgs a) {
   if (tile >= (1ull << a.d_out)) return;  // whole wavefront
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
   const uint64_t base = tile << R;
+  if (a.stage0 && !a.tile_busy[tile]) {
+    fixed_tile(a, tile, R, T, base, lane);
+    return;
+  }
   uint8_t l0[4];
   Digest g0[4];
   uint32_t s0[4];
@@ -556,26 +799,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_b(SweepArgs a) {
     }
   }
   if (!__ballot(busy)) {
-    // every leaf below is a fixed duplicate: the whole tile is fixed, no roots
-    if (lane < R) {
-      const uint32_t c = a.d_out + 1 + lane;
-      a.cnt_f[a.seg_base[c] + tile] = 0;
-      a.cnt_s[a.seg_base[c] + tile] = 0;
-    }
-    if (lane == 0) {
-      a.out_lab[tile] = L_FIXED;
-      if (a.d_out == 0) {
-        a.cnt_f[a.seg_base[0]] = 0;
-        a.cnt_s[a.seg_base[0]] = 0;
-      }
-    }
-    if (a.dbg_labels)
-      for (uint32_t r = 1; r <= R; r++)
-        for (uint32_t j = lane; j < (T >> r); j += 64) {
-          const uint64_t q = (base >> r) + j;
-          if (a.stage0 && r == 1 && q >= a.g.half) continue;
-          a.dbg_labels[heap_id(a.d_in - r, q)] = L_FIXED;
-        }
+    fixed_tile(a, tile, R, T, base, lane);
     return;
   }
   SweepB<HK> s{a};
@@ -665,11 +889,7 @@ __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t
 // A single chunk: the leaf is the root.
 __global__ void k_single_leaf(SweepArgs a) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint8_t l = a.labels[0];
-  if (l == L_PENDING) {
-    l = resolve_leaf(a, 0);
-    a.labels[0] = l;
-  }
+  const uint8_t l = a.labels[0];
   const bool root = is_region(l);
   if (root) a.staging[0] = (l == L_FIRST ? (kFirstTag << 32) : ((uint64_t)a.leaf_slot[0] << 32)) | 0u;
   a.cnt_f[0] = (root && l == L_FIRST) ? 1u : 0u;
@@ -941,7 +1161,8 @@ __global__ __launch_bounds__(kThreads) void k_rehash(const Slot* old, uint64_t o
 // ---------------------------------------------------------------------------
 template <int HK>
 static void launch_leaf(const LeafArgs& a, bool fast, hipStream_t st) {
-  const uint32_t grid = (uint32_t)((a.n + kThreads - 1) / kThreads);
+  const uint64_t ntiles = (a.n + kThreads - 1) / kThreads;
+  const uint32_t grid = (uint32_t)((ntiles + a.tiles_per_block - 1) / a.tiles_per_block);
   if (fast) {
     switch (a.C) {
       case 16: k_leaf<HK, 16><<<grid, kThreads, 0, st>>>(a); return;
@@ -1016,6 +1237,7 @@ static void plan_tree(mtc_dedup* h) {
   }
   h->seg_base_h.assign(g.D + 2, 0);
   for (uint32_t c = 0; c <= g.D; c++) h->seg_base_h[c + 1] = h->seg_base_h[c] + nb[c];
+  h->plan_dirty = true;
 }
 
 static void ensure_buffers(mtc_dedup* h, hipStream_t st) {
@@ -1035,12 +1257,25 @@ static void ensure_buffers(mtc_dedup* h, hipStream_t st) {
   h->seg_off_f.ensure(nseg + 1);
   h->seg_off_s.ensure(nseg + 1);
   h->seg_base.ensure(h->seg_base_h.size());
-  MTC_HIP(hipMemcpyAsync(h->seg_base.p, h->seg_base_h.data(), h->seg_base_h.size() * sizeof(uint32_t),
-                         hipMemcpyHostToDevice, st));
+  if (h->plan_dirty) {
+    MTC_HIP(hipMemcpyAsync(h->seg_base.p, h->seg_base_h.data(), h->seg_base_h.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, st));
+    MTC_HIP(hipStreamSynchronize(st));  // the host vector is pageable
+    h->plan_dirty = false;
+  }
+  h->list.ensure(n);
+  h->dense.ensure(n);
+  const uint64_t max_blk = (n + kThreads - 1) / kThreads;  // at most one block per tile
+  h->blk_cnt.ensure(max_blk);
+  h->blk_off.ensure(max_blk + 1);
+  h->tile_flags.ensure(2 * ((g.D > kFuse) ? (1ull << (g.D - kFuse)) : 1) + 4);
   h->first_roots.ensure(n);
   h->shift_tmp.ensure(n);
   h->region_off.ensure(n + 1);
-  h->diff.ensure(32 + 12 * n + 8ull * (h->current_id + 2) + n * (uint64_t)h->chunk_size);
+  // P <= checkpoints since the baseline; reserve in steps of 4096 so the arena is
+  // not reallocated every checkpoint.
+  const uint64_t pcap = ((uint64_t)h->current_id + 2 + 4095) / 4096 * 4096;
+  h->diff.ensure(32 + 12 * n + 8 * pcap + n * (uint64_t)h->chunk_size);
 }
 
 static void ensure_table(mtc_dedup* h, bool reset, hipStream_t st) {
@@ -1088,14 +1323,20 @@ static void run_sweeps(mtc_dedup* h, SweepArgs sa, hipStream_t st) {
     MTC_HIP(hipGetLastError());
     return;
   }
-  // Sweep A over all stages, then sweep B: every first-occurrence insert must land
-  // before any shifted-duplicate lookup (reference_impl.hpp:74-104).
+  // Sweep A over all stages, its inserts, then sweep B: every first-occurrence
+  // insert must land before any shifted-duplicate lookup (reference_impl.hpp:74-104).
   for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) {
+      const uint32_t nseg = h->seg_base_h.back();
+      k_fill_u32<<<grid_for(nseg, kThreads, 4096), kThreads, 0, st>>>(h->seg_cnt_f.p, nseg, 0);
+      k_fill_u32<<<grid_for(nseg, kThreads, 4096), kThreads, 0, st>>>(h->seg_cnt_s.p, nseg, 0);
+    }
     for (size_t s = 0; s < h->stages.size(); s++) {
       SweepArgs a = sa;
       a.d_in = h->stages[s].d_in;
       a.d_out = h->stages[s].d_out;
       a.stage0 = (s == 0);
+
       const uint64_t in_off = (s % 2 == 1) ? 0 : half_cap, out_off = (s % 2 == 0) ? 0 : half_cap;
       a.in_lab = h->fr_lab.p + in_off;
       a.in_dg = h->fr_dg.p + in_off;
@@ -1111,15 +1352,29 @@ static void run_sweeps(mtc_dedup* h, SweepArgs sa, hipStream_t st) {
   }
 }
 
+// Host-side phase timer, printed per checkpoint when MTC_HOST_TRACE is set.
+struct HostTrace {
+  bool on;
+  std::chrono::steady_clock::time_point t0, last;
+  std::string line;
+  HostTrace() : on(std::getenv("MTC_HOST_TRACE") != nullptr) { t0 = last = std::chrono::steady_clock::now(); }
+  void mark(const char* what) {
+    if (!on) return;
+    const auto now = std::chrono::steady_clock::now();
+    line += std::string(what) + "=" + std::to_string(std::chrono::duration<double, std::micro>(now - last).count()) + "us ";
+    last = now;
+  }
+  void print() {
+    if (on) std::fprintf(stderr, "[mtc host] %stotal=%.1fus\n", line.c_str(),
+                         std::chrono::duration<double, std::micro>(last - t0).count());
+  }
+};
+
 void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_baseline, hipStream_t st) {
-  hipEvent_t ev[5];
-  for (auto& e : ev) MTC_HIP(hipEventCreate(&e));
-  struct EvGuard {
-    hipEvent_t* e;
-    ~EvGuard() {
-      for (int i = 0; i < 5; i++) (void)hipEventDestroy(e[i]);
-    }
-  } guard{ev};
+  HostTrace ht;
+  if (!h->ev[0])
+    for (auto& e : h->ev) MTC_HIP(hipEventCreate(&e));
+  hipEvent_t* ev = h->ev;
   MTC_HIP(hipEventRecord(ev[0], st));
   const uint64_t C = h->chunk_size;
   const uint64_t n64 = (len + C - 1) / C;
@@ -1147,14 +1402,42 @@ void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_
                                                                         2 * (2ull * n - 1), 0);
   }
   MTC_HIP(hipEventRecord(ev[1], st));
+  ht.mark("setup");
 
-  LeafArgs la{data, len, n, h->chunk_size, h->leaf_dg.p, h->labels.p, h->leaf_slot.p, h->table.p,
-              h->table_cap - 1, cur, baseline ? 1 : 0, h->ctl_d, h->ctr.p};
+  // The leaf kernel runs one contiguous range of tiles per block (kLeafBlocks blocks).
+  const uint64_t ntiles_leaf = (n + kThreads - 1) / kThreads;
+  static const uint64_t leaf_blocks = [] {
+    const char* e = std::getenv("MTC_LEAF_BLOCKS");  // tuning knob for the persistent grid
+    return (e && std::atoll(e) > 0) ? (uint64_t)std::atoll(e) : kLeafBlocks;
+  }();
+  const uint64_t tpb = (ntiles_leaf + leaf_blocks - 1) / leaf_blocks;
+  const uint32_t nblk = (uint32_t)((ntiles_leaf + tpb - 1) / tpb);
+  LeafArgs la{data, len, n, h->chunk_size, h->leaf_dg.p, h->labels.p, baseline ? 1 : 0, h->list.p, h->blk_cnt.p, tpb};
   const bool fast = (C % 16 == 0) && ((uintptr_t)data % 16 == 0);
   if (h->hash_kind == HASH_MD5) launch_leaf<HASH_MD5>(la, fast, st);
   else launch_leaf<HASH_MURMUR3>(la, fast, st);
   MTC_HIP(hipGetLastError());
   MTC_HIP(hipEventRecord(ev[4], st));
+
+  // Dense table inserts over the changed chunks, then FIRST/SHIFT resolution.
+  exclusive_scan_u32(h->blk_cnt.p, h->blk_off.p, nblk, h->blk_off.p + nblk, h->ws, st);
+  k_compact<<<nblk, kThreads, 0, st>>>(h->list.p, h->blk_cnt.p, h->blk_off.p, tpb, h->dense.p);
+  MTC_HIP(hipGetLastError());
+  const uint32_t R0 = h->stages.empty() ? 0 : h->stages[0].d_in - h->stages[0].d_out;
+  const uint64_t ntile0 = h->stages.empty() ? 1 : (1ull << h->stages[0].d_out);
+  k_fill_u32<<<grid_for((2 * ntile0 + 3) / 4), kThreads, 0, st>>>(reinterpret_cast<uint32_t*>(h->tile_flags.p),
+                                                                 (2 * ntile0 + 3) / 4, 0);
+  DenseArgs ia{h->dense.p, h->blk_off.p + nblk, n, h->leaf_dg.p, h->leaf_slot.p, h->labels.p, h->table.p,
+               h->table_cap - 1, cur, h->ctl_d, h->ctr.p, g, R0, h->tile_flags.p, h->tile_flags.p + ntile0};
+  const uint32_t dense_grid = grid_for(n, kThreads, 16384);
+  if (!baseline) {  // a reset table holds nothing to find
+    k_lookup<<<dense_grid, kThreads, 0, st>>>(ia);
+    MTC_HIP(hipGetLastError());
+  }
+  k_insert<<<dense_grid, kThreads, 0, st>>>(ia);
+  MTC_HIP(hipGetLastError());
+  k_resolve<<<dense_grid, kThreads, 0, st>>>(ia);
+  MTC_HIP(hipGetLastError());
 
   SweepArgs sa{};
   sa.g = g;
@@ -1172,8 +1455,11 @@ void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_
   sa.seg_base = h->seg_base.p;
   sa.dbg_tree = h->debug ? h->dbg_tree.p : nullptr;
   sa.dbg_labels = h->debug ? h->dbg_labels.p : nullptr;
+  sa.tile_first = h->tile_flags.p;
+  sa.tile_busy = h->tile_flags.p + ntile0;
   if (h->hash_kind == HASH_MD5) run_sweeps<HASH_MD5>(h, sa, st);
   else run_sweeps<HASH_MURMUR3>(h, sa, st);
+  ht.mark("sweeps launched");
   if (h->debug) {  // leaf digests and labels into the debug views
     MTC_HIP(hipMemcpyAsync(h->dbg_tree.p + (n - 1), h->leaf_dg.p, 16ull * n, hipMemcpyDeviceToDevice, st));
     MTC_HIP(hipMemcpyAsync(h->dbg_labels.p + (n - 1), h->labels.p, n, hipMemcpyDeviceToDevice, st));
@@ -1186,6 +1472,7 @@ void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_
   exclusive_scan_u32(h->seg_cnt_s.p, h->seg_off_s.p, nseg, &cd->S, h->ws, st);
   MTC_HIP(hipEventRecord(ev[2], st));
   sync_ctl(h, st);
+  ht.mark("sync 1");
   const uint32_t F = h->ctl_h->F, S = h->ctl_h->S;
   k_scatter_roots<<<grid_for(nseg), kThreads, 0, st>>>(h->staging.p, h->seg_cnt_f.p, h->seg_cnt_s.p, h->seg_off_f.p,
                                                        h->seg_off_s.p, h->seg_base.p, g.D + 1, nseg,
@@ -1256,17 +1543,20 @@ void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_
     k_gather<<<grid_for(n, kThreads, 8192), kThreads, 0, st>>>(da);
     MTC_HIP(hipGetLastError());
   }
-  k_reduce_ctr<<<1, kStripes, 0, st>>>(h->ctr.p, cd);
+  k_reduce_ctr<<<1, kStripes, 0, st>>>(h->ctr.p, h->blk_off.p + nblk, cd);
   MTC_HIP(hipEventRecord(ev[3], st));
   sync_ctl(h, st);
+  ht.mark("final sync");
   h->table_size += h->ctl_h->claims;
   h->last_diff_size = h->ctl_h->diff_size;
-  MTC_HIP(hipMemcpy(&h->last_header, h->diff.p, sizeof(mtc_header), hipMemcpyDeviceToHost));
+  h->last_header = mtc_header{h->baseline_id, cur, len, h->chunk_size, h->ctl_h->F, h->ctl_h->P, h->ctl_h->S};
   MTC_HIP(hipEventElapsedTime(&h->last_ms[0], ev[0], ev[1]));
   MTC_HIP(hipEventElapsedTime(&h->last_ms[1], ev[1], ev[2]));
   MTC_HIP(hipEventElapsedTime(&h->last_ms[2], ev[2], ev[3]));
   MTC_HIP(hipEventElapsedTime(&h->last_ms[4], ev[1], ev[4]));
   h->last_ms[3] = 0;
+  ht.mark("epilogue");
+  ht.print();
 }
 
 }  // namespace mtc

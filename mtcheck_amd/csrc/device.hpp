@@ -322,10 +322,11 @@ constexpr uint32_t kMaxSpin = 1u << 20;
 
 // Insert d with value v.  use_min: val = min(val, v) (leaf inserts); otherwise
 // val is set only if still empty (insert-if-absent, first-occurrence nodes).
-// Returns the slot index, or ~0u on error (bit set in *err).  *claimed = 1
-// when this call created the entry.
+// cas_first: the digest is expected to be new, so each probe starts with the claim
+// instead of a read.  Returns the slot index, or ~0u on error (bit set in *err).
+// *claimed = 1 when this call created the entry.
 __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& d, uint64_t v, bool use_min,
-                                        uint32_t* err, bool* claimed) {
+                                        uint32_t* err, bool* claimed, bool cas_first = false) {
   *claimed = false;
   if (d.lo == TEMPTY) {  // overflow slot, keyed by k1 alone
     Slot* s = tab + mask + 1;
@@ -343,7 +344,7 @@ __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& 
   uint32_t probes = 0, spins = 0;
   while (true) {
     Slot* s = tab + i;
-    uint64_t k0 = ld_relaxed(&s->k0);
+    uint64_t k0 = cas_first ? TEMPTY : ld_relaxed(&s->k0);
     if (k0 == TEMPTY) {
       uint64_t old = cas_relaxed(&s->k0, TEMPTY, d.lo);
       if (old == TEMPTY) {
@@ -358,8 +359,15 @@ __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& 
     if (k0 == d.lo) {
       uint64_t k1 = ld_relaxed(&s->k1);
       if (k1 == d.hi) {
-        if (use_min) min_relaxed(&s->val, v);
-        else cas_relaxed(&s->val, TEMPTY, v);
+        // The value only ever decreases: skip the atomic when it cannot win.  This
+        // keeps lanes that repeat a popular digest (zero chunks) from serialising
+        // on one address.
+        const uint64_t now = ld_relaxed(&s->val);
+        if (use_min) {
+          if (now > v) min_relaxed(&s->val, v);
+        } else if (now == TEMPTY) {
+          cas_relaxed(&s->val, TEMPTY, v);
+        }
         return (uint32_t)i;
       }
       if (k1 == TEMPTY) {  // claimer has not published k1 yet: look again

@@ -107,6 +107,10 @@ struct mtc_dedup {
   mtc::DevBuf<mtc::Digest> leaf_dg;   // persistent leaf digests (the tree's leaves)
   mtc::DevBuf<uint8_t> labels;        // leaf labels of the current checkpoint
   mtc::DevBuf<uint32_t> leaf_slot;    // table slot of each non-fixed leaf
+  mtc::DevBuf<uint32_t> list;         // chunks to insert, packed per leaf block
+  mtc::DevBuf<uint32_t> dense;        // the same, concatenated
+  mtc::DevBuf<uint32_t> blk_cnt, blk_off;
+  mtc::DevBuf<uint8_t> tile_flags;    // per stage-0 tile: has-first, then busy
   mtc::DevBuf<mtc::Slot> table;
   mtc::DevBuf<mtc::Slot> table_tmp;   // rehash target
   mtc::DevBuf<uint8_t> fr_lab;        // stage frontiers (label, digest, slot)
@@ -131,6 +135,8 @@ struct mtc_dedup {
   mtc::Workspace ws;
   std::vector<mtc::Stage> stages;
   std::vector<uint32_t> seg_base_h;   // first segment of each depth
+  bool plan_dirty = false;            // seg_base_h not yet on the device
+  hipEvent_t ev[5] = {};              // phase timers, created on first use
   mtc_header last_header{};
   uint64_t last_diff_size = 0;
   float last_ms[5] = {0, 0, 0, 0, 0};  // setup, dedup, collect, copy, leaf kernel
