@@ -35,6 +35,8 @@ def parse():
     p.add_argument("--cpu-sample-mib", type=int, default=1024)
     p.add_argument("--cpu-steps", type=int, default=4)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                   help="only run a plain device copy of GIB GiB three times (scales the HBM byte counters)")
     return p.parse_args()
 
 
@@ -68,12 +70,45 @@ def cpu_baseline(args, hk):
                       f"({t:.1f} s) with the serial oracle"}
 
 
+def copy_calibration(gib: float):
+    """A plain device copy of known size (read GIB GiB, write GIB GiB) for scaling
+    the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.bitwise_not(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}), flush=True)
+
+
+def leaf_traffic(nbytes: int, C: int, hash_name: str, change: float):
+    """HBM bytes per k_leaf dispatch measured for this workload by the two
+    rocprofv3 --pmc passes summarised under profiles/ (None when no summary
+    covers this workload)."""
+    path = os.path.join(ROOT, "profiles", f"k_leaf_pmc_traffic_{nbytes >> 30}gib_c{C}_{hash_name}.md")
+    if not os.path.exists(path):
+        return None
+    fields = {}
+    for line in open(path):
+        if line.startswith("- ") and ": " in line:
+            k, v = line[2:].split(": ", 1)
+            fields[k.strip()] = v.strip().strip("`")
+    if fields.get("workload") != f"size={nbytes} chunk={C} hash={hash_name} change={change:g}":
+        return None
+    return int(fields["traffic_bytes_per_dispatch"])
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(independent buffers shard one per GPU with no communication)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     import numpy as np
     import torch
 
@@ -128,14 +163,15 @@ def main():
         "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {"workload": f"plain single-buffer dedup, {args.size_gib:g} GiB uint8, {C} B chunks, "
                                f"{args.hash}, {args.change * 100:g}% change per checkpoint (2:1 fresh runs:copies)",
-                   "global_batch": 1, "seq_len": 0, "parallelism": "single buffer, one GPU"},
+                   "size_bytes": nbytes, "chunk_size": C, "hash": args.hash, "change": args.change},
         "hbm_frac_checkpoint": round(ckpt_frac, 4),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "diff_mib_mean": round(float(np.mean(dsize)) / (1 << 20), 2),
         "roofline": {"kernel": "k_leaf (hash + fixed test + table insert)", "bound": "hbm",
                      "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": leaf_traffic(nbytes, C, args.hash, args.change),
                      "avg_ms": round(leaf_avg_ms, 4)},
     }
     del versions
