@@ -16,6 +16,8 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmtcheck_amd.so")
 SOURCES = ["api.hip", "dedup.hip", "restart.hip", "scan.hip", "workload.hip"]
+CLI = ["dedup_chkpt_files", "restart_chkpt_files"]
+BIN = os.path.join(PKG, "bin")
 ARCH = os.environ.get("MTC_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -47,6 +49,14 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    os.makedirs(BIN, exist_ok=True)
+    for prog in CLI:  # host programs on the C++ wrapper, linked against the library
+        cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+               os.path.join(PKG, "cli", prog + ".cpp"), "-L", PKG, "-lmtcheck_amd",
+               "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(BIN, prog)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {prog}:\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {LIB}", file=sys.stderr)
     return LIB
