@@ -35,6 +35,11 @@ def parse():
     p.add_argument("--cpu-sample-mib", type=int, default=1024)
     p.add_argument("--cpu-steps", type=int, default=4)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-batched", action="store_true", help="skip the batched multi-buffer measurement")
+    p.add_argument("--only-batched", action="store_true", help="profiling aid: run the batched measurement alone")
+    p.add_argument("--batch-count", type=int, default=256)
+    p.add_argument("--batch-steps", type=int, default=10)
+    p.add_argument("--no-autotune", action="store_true")
     p.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                    help="only run a plain device copy of GIB GiB three times (scales the HBM byte counters)")
     return p.parse_args()
@@ -100,6 +105,64 @@ def leaf_traffic(nbytes: int, C: int, hash_name: str, change: float):
     return int(fields["traffic_bytes_per_dispatch"])
 
 
+def header_tuple(h):
+    return (h.ref_id, h.chkpt_id, h.datalen, h.chunk_size, h.num_first_ocur, h.num_prior_chkpts, h.num_shift_dupl)
+
+
+def batched_bench(args):
+    """Many independent buffers of uneven sizes: one batched step (graph-replayed
+    batched launches) against one checkpoint call per buffer, same inputs; the
+    baseline step is untimed.  Headers and sizes of every member are compared each
+    step, and full diffs of a sample of members at the end."""
+    import torch
+
+    import mtcheck_amd as M
+    from mtcheck_amd.autotune import DEFAULT, autotune
+    from mtcheck_amd.workload import batch_members
+    members = batch_members(args.batch_count)
+    total = sum(s.nbytes for s in members)
+    tuned = {} if args.no_autotune else autotune({s.chunk_size for s in members})
+    bufs = [torch.empty(s.nbytes, dtype=torch.uint8, device="cuda:0") for s in members]
+    A = [M.TreeDeduplicator(s.chunk_size) for s in members]
+    for d, s in zip(A, members):
+        d.set_tuning(*tuned.get(s.chunk_size, DEFAULT))
+    P = [M.TreeDeduplicator(s.chunk_size) for s in members]
+    batch = M.Batch(use_graph=True)
+    tb, tp, lat_b = [], [], []
+    exact = True
+    for k in range(args.batch_steps):
+        for s, b in zip(members, bufs):
+            s.apply_device(b, k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hdrs, sizes = batch.checkpoint(A, bufs, make_baseline=(k == 0))
+        t1 = time.perf_counter()
+        solo = [d.checkpoint_device(b, make_baseline=(k == 0)) for d, b in zip(P, bufs)]
+        t2 = time.perf_counter()
+        if k:
+            tb.append(t1 - t0)
+            tp.append(t2 - t1)
+        for h, sz, (hs, _, szs) in zip(hdrs, sizes, solo):
+            exact &= header_tuple(h) == header_tuple(hs) and sz == szs
+    sample = range(0, len(members), max(1, len(members) // 16))
+    for i in sample:
+        exact &= A[i].copy_last_diff() == P[i].copy_last_diff()
+    vb = total * len(tb) / sum(tb) / 1e9
+    vp = total * len(tp) / sum(tp) / 1e9
+    res = {"workload": f"{len(members)} buffers, log-uniform 1 MiB-512 MiB ({total / 2**30:.1f} GiB), chunk size "
+                       f"from {{64,128,512}}, change U[0.5%,5%] split 2:1, {args.batch_steps} checkpoints",
+           "value": round(vb, 2), "unit": "GB/s", "per_buffer_value": round(vp, 2), "speedup": round(vb / vp, 3),
+           "ms_per_step": round(sum(tb) / len(tb) * 1e3, 3), "per_buffer_ms_per_step": round(sum(tp) / len(tp) * 1e3, 3),
+           "bit_exact": bool(exact), "graph": True,
+           "tuning": {str(c): list(t) for c, t in sorted(tuned.items())}}
+    for d in A + P:
+        d.close()
+    batch.close()
+    del bufs
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -108,6 +171,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.only_batched:
+        print(json.dumps({"batched": batched_bench(args)}), flush=True)
         return
     import numpy as np
     import torch
@@ -175,7 +241,10 @@ def main():
                      "avg_ms": round(leaf_avg_ms, 4)},
     }
     del versions
+    d.close()
     torch.cuda.empty_cache()
+    if not args.no_batched:
+        out["batched"] = batched_bench(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args, hk)
     print(json.dumps(out), flush=True)

@@ -134,6 +134,28 @@ int mtc_set_debug(mtc_dedup* h, int on);
 int mtc_copy_tree(const mtc_dedup* h, void* host_dst, uint64_t capacity);
 int mtc_copy_labels(const mtc_dedup* h, void* host_dst, uint64_t capacity);
 
+/* Schedule parameters of a handle (no counterpart in the reference, whose Kokkos
+ * policies are fixed): 256-chunk tiles streamed per leaf-kernel block (1..4096),
+ * table load factor at a baseline (capacity = chunks / load_factor, in (0.05, 0.7]),
+ * and tree levels fused per sweep stage (1..8).  The load factor and fuse depth
+ * take effect at the next baseline.  Results do not depend on them. */
+int mtc_set_tuning(mtc_dedup* h, uint32_t leaf_tiles_per_block, float load_factor, uint32_t fuse_levels);
+
+/* Batched checkpoints.  The reference checkpoints one buffer per deduplicator
+ * object per call (src/tree_approach.cpp:1577-1776; one object per MPI rank,
+ * src/dedup_files_mpi.cpp:72-81).  A batch runs one checkpoint of each of `count`
+ * handles (each with its own state, chunk size, hash and history) as one sequence
+ * of launches on the batch's stream, after the work already queued on `stream`.
+ * With use_graph, steps whose launch shape repeats are replayed from one captured
+ * hipGraph.  Each handle's result is identical to mtc_checkpoint on it alone. */
+typedef struct mtc_batch mtc_batch;
+int mtc_batch_create(mtc_batch** out, int device, int use_graph);
+int mtc_batch_destroy(mtc_batch* b);
+int mtc_checkpoint_batch(mtc_batch* b, mtc_dedup* const* handles, const void* const* dev_data, const uint64_t* lens,
+                         const int* make_baseline, uint32_t count, int advance_id, void* stream,
+                         mtc_header* headers_out, uint64_t* diff_sizes_out);
+const char* mtc_batch_last_error(const mtc_batch* b);
+
 /* Message for the last error on a handle (or the thread's last handle-less call). */
 const char* mtc_last_error(const mtc_dedup* h);
 const char* mtc_status_string(int status);

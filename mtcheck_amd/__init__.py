@@ -6,12 +6,12 @@ TreeDeduplicator interface plus the synthetic workload generator.
 """
 from .tree import (BASIC, DONE, FIRST_OCUR, FIXED_DUPL, FULL, LIST, MD5, MURMUR3, SHIFT_DUPL, TREE,
                    TREE_LOW_OFFSET, TREE_LOW_OFFSET_REF, TREE_LOW_ROOT, TREE_LOW_ROOT_REF, UNKNOWN,
-                   TreeDeduplicator, debug_labels, debug_tree, digest_host, digest_str, get_mode, hash_chunks,
+                   Batch, TreeDeduplicator, debug_labels, debug_tree, digest_host, digest_str, get_mode, hash_chunks,
                    parse_header, restart_into)
 from ._lib import MtcError
 
 __all__ = [
-    "TreeDeduplicator", "restart_into", "digest_host", "digest_str", "hash_chunks", "parse_header", "get_mode",
+    "TreeDeduplicator", "Batch", "restart_into", "digest_host", "digest_str", "hash_chunks", "parse_header", "get_mode",
     "debug_tree", "debug_labels", "MtcError", "MURMUR3", "MD5", "UNKNOWN", "FULL", "BASIC", "LIST", "TREE",
     "TREE_LOW_OFFSET_REF", "TREE_LOW_OFFSET", "TREE_LOW_ROOT_REF", "TREE_LOW_ROOT", "FIRST_OCUR", "FIXED_DUPL",
     "SHIFT_DUPL", "DONE",

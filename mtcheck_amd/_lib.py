@@ -22,7 +22,8 @@ STATUS_NAMES = {
 EXPORTS = [
     "mtc_create", "mtc_destroy", "mtc_checkpoint", "mtc_diff_device", "mtc_copy_diff", "mtc_get_stats",
     "mtc_restart", "mtc_digest_host", "mtc_hash_chunks", "mtc_set_debug", "mtc_copy_tree", "mtc_copy_labels",
-    "mtc_last_error", "mtc_status_string",
+    "mtc_last_error", "mtc_status_string", "mtc_set_tuning", "mtc_batch_create", "mtc_batch_destroy",
+    "mtc_checkpoint_batch", "mtc_batch_last_error",
     "mtc_wl_fill_stream", "mtc_wl_fill_runs", "mtc_wl_copy_chunks",
 ]
 
@@ -75,11 +76,19 @@ def lib():
         L.mtc_last_error.restype = ctypes.c_char_p
         L.mtc_status_string.argtypes = [i32]
         L.mtc_status_string.restype = ctypes.c_char_p
+        L.mtc_set_tuning.argtypes = [vp, u32, ctypes.c_float, u32]
+        L.mtc_batch_create.argtypes = [ctypes.POINTER(vp), i32, i32]
+        L.mtc_batch_destroy.argtypes = [vp]
+        L.mtc_checkpoint_batch.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(u64),
+                                           ctypes.POINTER(i32), u32, i32, vp, ctypes.POINTER(Header),
+                                           ctypes.POINTER(u64)]
+        L.mtc_batch_last_error.argtypes = [vp]
+        L.mtc_batch_last_error.restype = ctypes.c_char_p
         L.mtc_wl_fill_stream.argtypes = [vp, u64, u64, u64, vp]
         L.mtc_wl_fill_runs.argtypes = [vp, u64, u32, vp, u32, u64, i32, vp]
         L.mtc_wl_copy_chunks.argtypes = [vp, u64, u32, vp, u32, vp, vp]
         for name in EXPORTS:
-            if name not in ("mtc_last_error", "mtc_status_string"):
+            if name not in ("mtc_last_error", "mtc_status_string", "mtc_batch_last_error"):
                 getattr(L, name).restype = i32
         _lib = L
     return _lib

@@ -1,0 +1,54 @@
+"""Schedule autotuner: per (chunk size, hash) picks the leaf kernel's tiles per
+block, the table load factor and the sweep fuse depth by timing incremental
+checkpoints of a synthetic sample buffer on the device.  The choice never changes
+results (diffs are identical for every schedule), only speed."""
+from __future__ import annotations
+
+import itertools
+import time
+
+from .tree import MURMUR3, TreeDeduplicator
+from .workload import CheckpointSeries
+
+DEFAULT = (1, 0.25, 8)
+CANDIDATES = list(itertools.product((1, 2, 4, 8), (0.25, 0.4), (6, 8)))
+
+
+def time_schedule(series: CheckpointSeries, bufs, hash_kind: int, tuning, reps: int = 2) -> float:
+    """Mean seconds of the incremental checkpoints of the pre-generated versions."""
+    import torch
+    d = TreeDeduplicator(series.chunk_size, hash_kind=hash_kind)
+    d.set_tuning(*tuning)
+    d.checkpoint_device(bufs[0], make_baseline=True)
+    best = float("inf")
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for b in bufs[1:]:
+            d.checkpoint_device(b)
+        torch.cuda.synchronize()
+        best = min(best, (time.perf_counter() - t0) / (len(bufs) - 1))
+        d.checkpoint_device(bufs[0], make_baseline=True)
+    d.close()
+    return best
+
+
+def autotune(chunk_sizes, hash_kind: int = MURMUR3, sample_bytes: int = 256 << 20, change: float = 0.02,
+             steps: int = 3, candidates=None) -> dict:
+    """{chunk size: (leaf_tiles_per_block, load_factor, fuse_levels)}."""
+    import torch
+    out = {}
+    for C in sorted(set(int(c) for c in chunk_sizes)):
+        series = CheckpointSeries.with_change(sample_bytes, C, change, seed=0xA070 + C)
+        bufs = []
+        for k in range(steps + 1):
+            v = torch.empty(sample_bytes, dtype=torch.uint8, device="cuda:0")
+            if k:
+                v.copy_(bufs[-1])
+            series.apply_device(v, k)
+            bufs.append(v)
+        timings = {t: time_schedule(series, bufs, hash_kind, t) for t in (candidates or CANDIDATES)}
+        out[C] = min(timings, key=timings.get)
+        del bufs
+        torch.cuda.empty_cache()
+    return out

@@ -9,7 +9,19 @@ void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_
 void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t k, bool on_device,
                   uint8_t* out, uint64_t out_len, hipStream_t st);
 void hash_chunks(const uint8_t* data, uint64_t len, uint32_t C, int hk, Digest* out, hipStream_t st);
+void checkpoint_batch_impl(mtc_dedup* const* hs, const uint8_t* const* datas, const uint64_t* lens,
+                           const int* baselines, uint32_t count, StepCtx& ctx, hipStream_t st, bool use_graph);
 }  // namespace mtc
+
+// A batch context: its own stream (default priority) and step state.
+struct mtc_batch {
+  int device = 0;
+  int use_graph = 1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ready = nullptr;
+  mtc::StepCtx step;
+  std::string last_error;
+};
 
 using mtc::HipError;
 using mtc::Status;
@@ -96,8 +108,6 @@ int mtc_destroy(mtc_dedup* h) {
   (void)hipSetDevice(h->device);
   if (h->ctl_d) (void)hipFree(h->ctl_d);
   if (h->ctl_h) (void)hipHostFree(h->ctl_h);
-  for (auto& e : h->ev)
-    if (e) (void)hipEventDestroy(e);
   delete h;  // device buffers are released by their destructors
   return MTC_OK;
 }
@@ -226,6 +236,81 @@ int mtc_copy_labels(const mtc_dedup* h, void* host_dst, uint64_t capacity) {
     MTC_HIP(hipMemcpy(host_dst, h->dbg_labels.p, bytes, hipMemcpyDeviceToHost));
   });
 }
+
+int mtc_set_tuning(mtc_dedup* h, uint32_t leaf_tiles_per_block, float load_factor, uint32_t fuse_levels) {
+  if (!h) return MTC_ERR_INVALID_ARG;
+  return guarded(h, [&] {
+    if (leaf_tiles_per_block == 0 || leaf_tiles_per_block > 4096 || !(load_factor > 0.05f && load_factor <= 0.7f) ||
+        fuse_levels == 0 || fuse_levels > 8)
+      throw Status{MTC_ERR_INVALID_ARG, "tuning out of range (tiles 1..4096, load factor (0.05, 0.7], fuse 1..8)"};
+    h->tune.leaf_tiles_per_block = leaf_tiles_per_block;
+    h->tune.fuse = fuse_levels;
+    if (h->tune.load_factor != load_factor || h->has_state) {
+      // Load factor and fuse depth shape the table and the tree plan: they take
+      // effect at the next baseline.
+      h->tune.load_factor = load_factor;
+    }
+  });
+}
+
+int mtc_batch_create(mtc_batch** out, int device, int use_graph) {
+  if (!out) return MTC_ERR_INVALID_ARG;
+  *out = nullptr;
+  mtc_batch* b = nullptr;
+  int rc = guarded(nullptr, [&] {
+    use_device(device);
+    b = new mtc_batch();
+    b->device = device;
+    b->use_graph = use_graph ? 1 : 0;
+    MTC_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    MTC_HIP(hipEventCreateWithFlags(&b->ready, hipEventDisableTiming));
+  });
+  if (rc != MTC_OK) {
+    if (b) mtc_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return MTC_OK;
+}
+
+int mtc_batch_destroy(mtc_batch* b) {
+  if (!b) return MTC_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  b->step.reset_graph();
+  if (b->ready) (void)hipEventDestroy(b->ready);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return MTC_OK;
+}
+
+int mtc_checkpoint_batch(mtc_batch* b, mtc_dedup* const* handles, const void* const* dev_data, const uint64_t* lens,
+                         const int* make_baseline, uint32_t count, int advance_id, void* stream,
+                         mtc_header* headers_out, uint64_t* diff_sizes_out) {
+  if (!b || (count && (!handles || !dev_data || !lens))) return MTC_ERR_INVALID_ARG;
+  int rc = guarded(nullptr, [&] {
+    if (count == 0) return;
+    MTC_HIP(hipSetDevice(b->device));
+    for (uint32_t i = 0; i < count; i++) {
+      if (!handles[i] || !dev_data[i] || lens[i] == 0) throw Status{MTC_ERR_INVALID_ARG, "empty member in batch"};
+      if (handles[i]->device != b->device) throw Status{MTC_ERR_INVALID_ARG, "member on another device"};
+    }
+    // Work the caller queued on `stream` (the inputs) comes first.
+    MTC_HIP(hipEventRecord(b->ready, static_cast<hipStream_t>(stream)));
+    MTC_HIP(hipStreamWaitEvent(b->stream, b->ready, 0));
+    mtc::checkpoint_batch_impl(handles, reinterpret_cast<const uint8_t* const*>(dev_data), lens, make_baseline, count,
+                               b->step, b->stream, b->use_graph != 0);
+    for (uint32_t i = 0; i < count; i++) {
+      if (headers_out) headers_out[i] = handles[i]->last_header;
+      if (diff_sizes_out) diff_sizes_out[i] = handles[i]->last_diff_size;
+      if (advance_id) handles[i]->current_id += 1;
+    }
+  });
+  if (rc != MTC_OK) b->last_error = g_last_error;
+  return rc;
+}
+
+const char* mtc_batch_last_error(const mtc_batch* b) { return b ? b->last_error.c_str() : g_last_error.c_str(); }
 
 const char* mtc_last_error(const mtc_dedup* h) { return h ? h->last_error.c_str() : g_last_error.c_str(); }
 

@@ -5,28 +5,91 @@
 // roots) and src/tree_approach.cpp:511-710 (diff layout), in the canonical order
 // of the serial reference (roots ascending; shifted entries grouped by source
 // checkpoint, ascending node id inside a group).
+//
+// Every kernel of the pipeline serves a batch of deduplicators ("members") in one
+// launch: a Launch<A> carries one argument block per member and the block offsets
+// of each member's share of the grid, so one checkpoint of one buffer is a batch
+// of one and a step over hundreds of buffers costs the same number of launches.
+// Grids depend only on the members' geometry and every data-dependent count (roots,
+// shifted entries, chunks to insert) is read on the device, so a whole step has no
+// host synchronisation until its end and can be captured as one hipGraph.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
 
 #include "internal.hpp"
 
 namespace mtc {
 
 constexpr uint32_t kThreads = 256;
-constexpr uint32_t kFuse = 8;              // tree levels fused per sweep stage (one wavefront tile)
-constexpr uint64_t kLeafBlocks = 1ull << 40; // leaf-kernel blocks (default: one 256-chunk tile each)
+constexpr uint32_t kFuse = 8;                  // most tree levels fused per sweep stage (one wavefront tile)
 constexpr uint64_t kFirstTag = 0xFFFFFFFFull;  // staging tag of a first-occurrence root
+constexpr uint32_t kScanTile = 8 * kThreads;   // elements per block-tile of the batched scan
+constexpr uint32_t kSortBlocks = 1024;         // most blocks per member of the stable sort
+
+// ---------------------------------------------------------------------------
+// Batched launches
+// ---------------------------------------------------------------------------
+template <class A>
+struct Launch {
+  const A* args;           // one per member
+  const uint32_t* prefix;  // count + 1 block offsets
+  const uint16_t* group;   // member owning block 64*i (only when count > 1)
+  uint32_t count;
+};
+
+struct Where {
+  uint32_t m, blk, nblk;  // member, block within the member's share, share size
+};
+
+// Pointers inside argument blocks are loaded from memory, so the compiler cannot
+// tell that they address global memory and would emit generic (flat) accesses.
+// A round trip through the global address space lets it use global instructions.
+template <class T>
+__device__ __forceinline__ T* gl(T* p) {
+  return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)p;
+}
+#define MTC_GLOBAL(f) c.f = gl(c.f)
+
+// The member of this block: a table entry per 64 blocks, then a short forward walk
+// (members with no blocks are skipped by the walk).  All values are wave-uniform.
+template <class A>
+__device__ __forceinline__ Where where(const Launch<A>& L) {
+  const uint32_t b = blockIdx.x;
+  uint32_t m = 0;
+  if (L.count > 1) {
+    m = L.group[b >> 6];
+    while (L.prefix[m + 1] <= b) m++;
+  }
+  return Where{m, b - L.prefix[m], L.prefix[m + 1] - L.prefix[m]};
+}
 
 // ---------------------------------------------------------------------------
 // small utility kernels
 // ---------------------------------------------------------------------------
-__global__ void k_fill_u64(uint64_t* p, uint64_t count, uint64_t v) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
-    p[i] = v;
+struct FillSeg {
+  uint32_t* p;
+  uint64_t count;  // 32-bit words
+  uint32_t v;
+  __device__ FillSeg global() const {
+    FillSeg c = *this;
+    MTC_GLOBAL(p);
+    return c;
+  }
+};
+
+__global__ __launch_bounds__(kThreads) void k_fill(Launch<FillSeg> L) {
+  const Where w = where(L);
+  const FillSeg s = L.args[w.m].global();
+  for (uint64_t i = (uint64_t)w.blk * kThreads + threadIdx.x; i < s.count; i += (uint64_t)w.nblk * kThreads)
+    s.p[i] = s.v;
 }
 
-__global__ void k_fill_u32(uint32_t* p, uint64_t count, uint32_t v) {
+__global__ void k_fill_u64(uint64_t* p, uint64_t count, uint64_t v) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = v;
 }
@@ -40,31 +103,39 @@ __device__ __forceinline__ unsigned long long* stripe(unsigned long long* ctr, u
   return ctr + kind * kStripes + (blockIdx.x & (kStripes - 1));
 }
 
-__global__ void k_reduce_ctr(const unsigned long long* ctr, const uint32_t* changed, Ctl* c) {
+struct CtrArgs {
+  const unsigned long long* ctr;
+  const uint32_t* changed;
+  Ctl* ctl;
+  Ctl* out;  // the member's entry of the step's result array
+  __device__ CtrArgs global() const {
+    CtrArgs c = *this;
+    MTC_GLOBAL(ctr); MTC_GLOBAL(changed); MTC_GLOBAL(ctl); MTC_GLOBAL(out);
+    return c;
+  }
+};
+
+// Fold the striped counters, then copy the member's control block into the step's
+// contiguous result array (one device-to-host copy for the whole batch).
+__global__ __launch_bounds__(kStripes) void k_finish(Launch<CtrArgs> L) {
+  const Where w = where(L);
+  const CtrArgs a = L.args[w.m].global();
   __shared__ unsigned long long s[CTR_KINDS];
   if (threadIdx.x < CTR_KINDS) s[threadIdx.x] = 0;
   __syncthreads();
   for (uint32_t k = 0; k < CTR_KINDS; k++) {
-    const unsigned long long v = ctr[k * kStripes + threadIdx.x];
+    const unsigned long long v = a.ctr[k * kStripes + threadIdx.x];
     if (v) atomicAdd(&s[k], v);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    c->claims = s[CTR_CLAIMS];
-    c->changed = *changed;
+    a.ctl->claims = s[CTR_CLAIMS];
+    a.ctl->changed = *a.changed;
   }
-}
-
-__global__ void k_zero_ctl(Ctl* c, unsigned long long* ctr) {
-  for (uint32_t i = threadIdx.x; i < CTR_KINDS * kStripes; i += blockDim.x) ctr[i] = 0;
-  if (threadIdx.x == 0) {
-    c->err = 0;
-    c->F = c->S = c->P = 0;
-    c->claims = 0;
-    c->data_chunks = 0;
-    c->diff_size = 0;
-    c->changed = 0;
-  }
+  __syncthreads();
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(a.ctl);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(a.out);
+  if (threadIdx.x < sizeof(Ctl) / 4) dst[threadIdx.x] = src[threadIdx.x];
 }
 
 static uint32_t grid_for(uint64_t items, uint32_t per_block = kThreads, uint32_t cap = 1u << 20) {
@@ -78,6 +149,110 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads).
+__device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t* ws) {
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) ws[w] = x;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kThreads / 64; i++) {
+    if (i < w) pre += ws[i];
+    tot += ws[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return pre + x - v;
+}
+
+// ---------------------------------------------------------------------------
+// Batched exclusive scan of u32 arrays: per-tile sums, a per-array scan of the
+// sums, then the tiles with their offsets.  The element count may live on the
+// device (count_dev), bounded by count_max.
+// ---------------------------------------------------------------------------
+struct ScanSeg {
+  const uint32_t* in;
+  uint32_t* out;          // may equal in
+  const uint32_t* count_dev;
+  uint32_t count_max;
+  uint32_t* total;        // optional: sum of all elements
+  uint32_t* sums;         // ceil(count_max / kScanTile) words
+  __device__ ScanSeg global() const {
+    ScanSeg c = *this;
+    MTC_GLOBAL(in); MTC_GLOBAL(out); MTC_GLOBAL(count_dev); MTC_GLOBAL(total); MTC_GLOBAL(sums);
+    return c;
+  }
+};
+
+__device__ __forceinline__ uint32_t scan_count(const ScanSeg& s) {
+  return s.count_dev ? min(*s.count_dev, s.count_max) : s.count_max;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(Launch<ScanSeg> L) {
+  __shared__ uint32_t ws[kThreads / 64];
+  const Where w = where(L);
+  const ScanSeg s = L.args[w.m].global();
+  const uint32_t cnt = scan_count(s), ntiles = (cnt + kScanTile - 1) / kScanTile;
+  for (uint32_t tile = w.blk; tile < ntiles; tile += w.nblk) {
+    const uint64_t base = (uint64_t)tile * kScanTile;
+    uint32_t v = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint64_t i = base + j * kThreads + threadIdx.x;
+      if (i < cnt) v += s.in[i];
+    }
+    uint32_t tot;
+    block_scan_excl(v, &tot, ws);
+    if (threadIdx.x == 0) s.sums[tile] = tot;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_top(Launch<ScanSeg> L) {
+  __shared__ uint32_t ws[kThreads / 64];
+  const Where w = where(L);
+  const ScanSeg s = L.args[w.m].global();
+  const uint32_t cnt = scan_count(s), ntiles = (cnt + kScanTile - 1) / kScanTile;
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < ntiles; b0 += kThreads) {
+    const uint32_t i = b0 + threadIdx.x;
+    const uint32_t v = i < ntiles ? s.sums[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_scan_excl(v, &tot, ws);
+    if (i < ntiles) s.sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && s.total) *s.total = carry;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_down(Launch<ScanSeg> L) {
+  __shared__ uint32_t ws[kThreads / 64];
+  const Where w = where(L);
+  const ScanSeg s = L.args[w.m].global();
+  const uint32_t cnt = scan_count(s), ntiles = (cnt + kScanTile - 1) / kScanTile;
+  for (uint32_t tile = w.blk; tile < ntiles; tile += w.nblk) {
+    const uint64_t base = (uint64_t)tile * kScanTile + 8ull * threadIdx.x;
+    uint32_t v[8], run = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) {
+      v[j] = (base + j < cnt) ? s.in[base + j] : 0u;
+      const uint32_t x = v[j];
+      v[j] = run;
+      run += x;
+    }
+    uint32_t tot;
+    const uint32_t off = s.sums[tile] + block_scan_excl(run, &tot, ws);
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++)
+      if (base + j < cnt) s.out[base + j] = off + v[j];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -174,6 +349,11 @@ struct LeafArgs {
   uint32_t* list;     // chunks to insert, packed per block
   uint32_t* blk_cnt;  // their count per block
   uint64_t tiles_per_block;
+  __device__ LeafArgs global() const {
+    LeafArgs c = *this;
+    MTC_GLOBAL(data); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(labels); MTC_GLOBAL(list); MTC_GLOBAL(blk_cnt);
+    return c;
+  }
 };
 
 // Digest of chunk tile*256 + threadIdx.x (zero beyond n).  CB > 0 selects the
@@ -235,10 +415,12 @@ __device__ __forceinline__ void leaf_epilogue(const LeafArgs& a, uint64_t t, con
 // current one is hashed out of LDS.  Chunks to insert are packed into the block's
 // region of the list; the table work runs densely afterwards.
 template <int HK, int CB>
-__global__ __launch_bounds__(kThreads) void k_leaf(LeafArgs a) {
+__global__ __launch_bounds__(kThreads) void k_leaf(Launch<LeafArgs> L) {
   __shared__ uint32_t s_wcnt[kThreads / 64];
+  const Where wh = where(L);
+  const LeafArgs a = L.args[wh.m].global();
   const uint64_t ntiles = (a.n + kThreads - 1) / kThreads;
-  const uint64_t t0 = (uint64_t)blockIdx.x * a.tiles_per_block;
+  const uint64_t t0 = (uint64_t)wh.blk * a.tiles_per_block;
   const uint64_t t1 = std::min<uint64_t>(t0 + a.tiles_per_block, ntiles);
   uint32_t* region = a.list + t0 * kThreads;
   uint32_t run = 0;
@@ -249,7 +431,7 @@ __global__ __launch_bounds__(kThreads) void k_leaf(LeafArgs a) {
     using S = TileShape<CB>;
     __shared__ __align__(16) uint8_t lds[S::LDS];
     const uint64_t full = a.len / CB;
-    const uint64_t steps = (t1 - t0) * S::PASSES;
+    const uint64_t steps = (t1 > t0 ? t1 - t0 : 0) * S::PASSES;
     const uint32_t tid = threadIdx.x;
     auto load = [&](uint64_t step, uint4* v) {
       const uint64_t t = t0 + step / S::PASSES;
@@ -322,17 +504,35 @@ __global__ __launch_bounds__(kThreads) void k_leaf(LeafArgs a) {
       for (int i = 0; i < S::LOADS; i++) v[i] = nv[i];
     }
   }
-  if (threadIdx.x == 0) a.blk_cnt[blockIdx.x] = run;
+  if (threadIdx.x == 0) a.blk_cnt[wh.blk] = run;
 }
 
 // Concatenate the per-block regions into the dense list of chunks to insert.
-__global__ __launch_bounds__(kThreads) void k_compact(const uint32_t* list, const uint32_t* blk_cnt,
-                                                      const uint32_t* blk_off, uint64_t tiles_per_block,
-                                                      uint32_t* dense) {
-  const uint32_t b = blockIdx.x, cnt = blk_cnt[b];
-  const uint32_t* src = list + (uint64_t)b * tiles_per_block * kThreads;
-  uint32_t* dst = dense + blk_off[b];
-  for (uint32_t i = threadIdx.x; i < cnt; i += kThreads) dst[i] = src[i];
+struct CompactArgs {
+  const uint32_t* list;
+  const uint32_t* blk_cnt;
+  const uint32_t* blk_off;
+  uint64_t tiles_per_block;
+  uint32_t* dense;
+  uint32_t nblk;  // leaf blocks
+  __device__ CompactArgs global() const {
+    CompactArgs c = *this;
+    MTC_GLOBAL(list); MTC_GLOBAL(blk_cnt); MTC_GLOBAL(blk_off); MTC_GLOBAL(dense);
+    return c;
+  }
+};
+
+constexpr uint32_t kCompactPer = 8;  // leaf blocks per compaction block
+
+__global__ __launch_bounds__(kThreads) void k_compact(Launch<CompactArgs> L) {
+  const Where w = where(L);
+  const CompactArgs a = L.args[w.m].global();
+  for (uint32_t b = w.blk * kCompactPer; b < min(a.nblk, (w.blk + 1) * kCompactPer); b++) {
+    const uint32_t cnt = a.blk_cnt[b];
+    const uint32_t* src = a.list + (uint64_t)b * a.tiles_per_block * kThreads;
+    uint32_t* dst = a.dense + a.blk_off[b];
+    for (uint32_t i = threadIdx.x; i < cnt; i += kThreads) dst[i] = src[i];
+  }
 }
 
 struct DenseArgs {
@@ -351,6 +551,11 @@ struct DenseArgs {
   uint32_t R0;          // levels of the first sweep stage
   uint8_t* tile_first;  // per stage-0 tile: holds a first-occurrence leaf
   uint8_t* tile_busy;   // per stage-0 tile: holds a leaf that is not a fixed duplicate
+  __device__ DenseArgs global() const {
+    DenseArgs c = *this;
+    MTC_GLOBAL(dense); MTC_GLOBAL(total); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(labels); MTC_GLOBAL(tab); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(tile_first); MTC_GLOBAL(tile_busy);
+    return c;
+  }
 };
 
 // Stage-0 sweep tile of a chunk: bottom-level chunks by position, depth D-1 leaves
@@ -365,9 +570,11 @@ __device__ __forceinline__ uint64_t chunk_tile(const Geo& g, uint32_t R0, uint64
 // table belongs to an earlier checkpoint, so its leaf is a shifted duplicate and
 // needs no insert (include/reference_impl.hpp:51,66-67).  Plain loads, so popular
 // digests (zero chunks) are served from cache.
-__global__ __launch_bounds__(kThreads) void k_lookup(DenseArgs a) {
+__global__ __launch_bounds__(kThreads) void k_lookup(Launch<DenseArgs> L) {
+  const Where w = where(L);
+  const DenseArgs a = L.args[w.m].global();
   const uint64_t total = *a.total;
-  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < total; i += (uint64_t)gridDim.x * kThreads) {
+  for (uint64_t i = (uint64_t)w.blk * kThreads + threadIdx.x; i < total; i += (uint64_t)w.nblk * kThreads) {
     const uint32_t c = a.dense[i];
     const uint32_t s = table_find(a.tab, a.mask, a.leaf_dg[c]);
     if (s == ~0u) continue;
@@ -383,12 +590,14 @@ __global__ __launch_bounds__(kThreads) void k_lookup(DenseArgs a) {
 // packed value (cur << 32 | leaf) with atomicMin leaves the lowest leaf of a new
 // digest in the table.  Consecutive entries with the same digest (zero-filled
 // runs) share the first one's table access.
-__global__ __launch_bounds__(kThreads) void k_insert(DenseArgs a) {
+__global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
+  const Where w = where(L);
+  const DenseArgs a = L.args[w.m].global();
   const uint64_t total = *a.total;
   const uint32_t lane = lane_id();
-  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  const uint64_t stride = (uint64_t)w.nblk * kThreads;
   uint32_t claims = 0;
-  for (uint64_t wbase = (uint64_t)blockIdx.x * kThreads + (threadIdx.x & ~63u); wbase < total; wbase += stride) {
+  for (uint64_t wbase = (uint64_t)w.blk * kThreads + (threadIdx.x & ~63u); wbase < total; wbase += stride) {
     const uint64_t i = wbase + lane;
     bool v = i < total;
     uint32_t c = 0;
@@ -418,9 +627,11 @@ __global__ __launch_bounds__(kThreads) void k_insert(DenseArgs a) {
 
 // Once every insert has landed: a pending leaf is FIRST when its digest's entry is
 // (cur, itself), SHIFT otherwise (include/reference_impl.hpp:52-69).
-__global__ __launch_bounds__(kThreads) void k_resolve(DenseArgs a) {
+__global__ __launch_bounds__(kThreads) void k_resolve(Launch<DenseArgs> L) {
+  const Where w = where(L);
+  const DenseArgs a = L.args[w.m].global();
   const uint64_t total = *a.total;
-  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < total; i += (uint64_t)gridDim.x * kThreads) {
+  for (uint64_t i = (uint64_t)w.blk * kThreads + threadIdx.x; i < total; i += (uint64_t)w.nblk * kThreads) {
     const uint32_t c = a.dense[i];
     if (a.labels[c] != L_PENDING) continue;  // a fixed leaf right after a reset
     const uint32_t s = a.leaf_slot[c];
@@ -479,6 +690,11 @@ struct SweepArgs {
   uint8_t* dbg_labels;
   const uint8_t* tile_first;  // stage 0: tile holds a first-occurrence leaf
   const uint8_t* tile_busy;   // stage 0: tile holds a leaf that is not a fixed duplicate
+  __device__ SweepArgs global() const {
+    SweepArgs c = *this;
+    MTC_GLOBAL(leaf_dg); MTC_GLOBAL(labels); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(in_lab); MTC_GLOBAL(in_dg); MTC_GLOBAL(in_slot); MTC_GLOBAL(out_lab); MTC_GLOBAL(out_dg); MTC_GLOBAL(out_slot); MTC_GLOBAL(tab); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(staging); MTC_GLOBAL(cnt_f); MTC_GLOBAL(cnt_s); MTC_GLOBAL(seg_base); MTC_GLOBAL(dbg_tree); MTC_GLOBAL(dbg_labels); MTC_GLOBAL(tile_first); MTC_GLOBAL(tile_busy);
+    return c;
+  }
 };
 
 __device__ __forceinline__ uint64_t bottom_chunk(const Geo& g, uint64_t p) { return (1ull << g.D) - g.n + p; }
@@ -557,9 +773,11 @@ struct SweepA {
 };
 
 template <int HK>
-__global__ __launch_bounds__(kThreads) void k_sweep_a(SweepArgs a) {
+__global__ __launch_bounds__(kThreads) void k_sweep_a(Launch<SweepArgs> L) {
+  const Where wh = where(L);
+  const SweepArgs a = L.args[wh.m].global();
   const uint32_t lane = lane_id();
-  const uint64_t tile = (uint64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const uint64_t tile = (uint64_t)wh.blk * (kThreads / 64) + (threadIdx.x >> 6);
   if (tile >= (1ull << a.d_out)) return;  // whole wavefront
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
   const uint64_t base = tile << R;
@@ -743,9 +961,11 @@ __device__ void fixed_tile(const SweepArgs& a, uint64_t tile, uint32_t R, uint32
 }
 
 template <int HK>
-__global__ __launch_bounds__(kThreads) void k_sweep_b(SweepArgs a) {
+__global__ __launch_bounds__(kThreads) void k_sweep_b(Launch<SweepArgs> L) {
+  const Where wh = where(L);
+  const SweepArgs a = L.args[wh.m].global();
   const uint32_t lane = lane_id();
-  const uint64_t tile = (uint64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const uint64_t tile = (uint64_t)wh.blk * (kThreads / 64) + (threadIdx.x >> 6);
   if (tile >= (1ull << a.d_out)) return;  // whole wavefront
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
   const uint64_t base = tile << R;
@@ -864,31 +1084,12 @@ __global__ __launch_bounds__(kThreads) void k_sweep_b(SweepArgs a) {
   }
 }
 
-// Block-wide exclusive scan of one value per thread (256 threads).
-__device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t* ws) {
-  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint32_t x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    uint32_t y = __shfl_up(x, o);
-    if (lane >= (uint32_t)o) x += y;
-  }
-  if (lane == 63) ws[w] = x;
-  __syncthreads();
-  uint32_t pre = 0, tot = 0;
-#pragma unroll
-  for (uint32_t i = 0; i < kThreads / 64; i++) {
-    if (i < w) pre += ws[i];
-    tot += ws[i];
-  }
-  __syncthreads();
-  *total = tot;
-  return pre + x - v;
-}
 
 // A single chunk: the leaf is the root.
-__global__ void k_single_leaf(SweepArgs a) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void k_single_leaf(Launch<SweepArgs> L) {
+  const Where w = where(L);
+  if (threadIdx.x != 0) return;
+  const SweepArgs a = L.args[w.m].global();
   const uint8_t l = a.labels[0];
   const bool root = is_region(l);
   if (root) a.staging[0] = (l == L_FIRST ? (kFirstTag << 32) : ((uint64_t)a.leaf_slot[0] << 32)) | 0u;
@@ -899,23 +1100,36 @@ __global__ void k_single_leaf(SweepArgs a) {
 // ---------------------------------------------------------------------------
 // Root compaction: one thread per (depth, block) segment, in segment order.
 // ---------------------------------------------------------------------------
-__global__ void k_scatter_roots(const uint64_t* staging, const uint32_t* cnt_f, const uint32_t* cnt_s,
-                                const uint32_t* off_f, const uint32_t* off_s, const uint32_t* seg_base,
-                                uint32_t ndepth, uint32_t nseg, uint32_t* first_roots, uint64_t* shift_tmp) {
-  const uint32_t seg = blockIdx.x * blockDim.x + threadIdx.x;
-  if (seg >= nseg) return;
-  const uint32_t nf = cnt_f[seg], ns = cnt_s[seg];
+struct RootArgs {
+  const uint64_t* staging;
+  const uint32_t *cnt_f, *cnt_s, *off_f, *off_s, *seg_base;
+  uint32_t ndepth, nseg;
+  uint32_t* first_roots;
+  uint64_t* shift_tmp;
+  __device__ RootArgs global() const {
+    RootArgs c = *this;
+    MTC_GLOBAL(staging); MTC_GLOBAL(cnt_f); MTC_GLOBAL(cnt_s); MTC_GLOBAL(off_f); MTC_GLOBAL(off_s); MTC_GLOBAL(seg_base); MTC_GLOBAL(first_roots); MTC_GLOBAL(shift_tmp);
+    return c;
+  }
+};
+
+__global__ __launch_bounds__(kThreads) void k_scatter_roots(Launch<RootArgs> L) {
+  const Where w = where(L);
+  const RootArgs a = L.args[w.m].global();
+  const uint32_t seg = w.blk * kThreads + threadIdx.x;
+  if (seg >= a.nseg) return;
+  const uint32_t nf = a.cnt_f[seg], ns = a.cnt_s[seg];
   if (nf + ns == 0) return;
   uint32_t c = 0;
-  while (c + 1 < ndepth && seg_base[c + 1] <= seg) c++;
-  const uint32_t nb = seg_base[c + 1] - seg_base[c];  // blocks of the emitting stage (a power of two)
+  while (c + 1 < a.ndepth && a.seg_base[c + 1] <= seg) c++;
+  const uint32_t nb = a.seg_base[c + 1] - a.seg_base[c];  // blocks of the emitting stage (a power of two)
   const uint32_t span = c - (31u - (uint32_t)__clz((int)nb));
-  const uint64_t sb = ((1ull << c) - 1) + ((uint64_t)(seg - seg_base[c]) << span);
-  uint32_t of = off_f[seg], os = off_s[seg];
+  const uint64_t sb = ((1ull << c) - 1) + ((uint64_t)(seg - a.seg_base[c]) << span);
+  uint32_t of = a.off_f[seg], os = a.off_s[seg];
   for (uint32_t j = 0; j < nf + ns; j++) {
-    const uint64_t e = staging[sb + j];
-    if ((e >> 32) == kFirstTag) first_roots[of++] = (uint32_t)e;
-    else shift_tmp[os++] = e;
+    const uint64_t e = a.staging[sb + j];
+    if ((e >> 32) == kFirstTag) a.first_roots[of++] = (uint32_t)e;
+    else a.shift_tmp[os++] = e;
   }
 }
 
@@ -925,116 +1139,198 @@ __global__ void k_scatter_roots(const uint64_t* staging, const uint32_t* cnt_f, 
 // ---------------------------------------------------------------------------
 constexpr uint32_t kLdsBins = 4096;  // source-checkpoint bins counted in LDS
 
-__global__ __launch_bounds__(kThreads) void k_shift_src(const uint64_t* shift_tmp, uint32_t S, const Slot* tab,
-                                                        uint64_t mask, uint32_t ref, uint32_t cur, uint32_t* key,
-                                                        uint64_t* val, uint32_t* src_hist, uint32_t nbins, Ctl* ctl) {
+struct ShiftArgs {
+  const uint64_t* shift_tmp;
+  const uint32_t* S;
+  const Slot* tab;
+  uint64_t mask;
+  uint32_t ref, cur;
+  uint32_t* key;
+  uint64_t* val;
+  uint32_t* src_hist;
+  uint32_t nbins;
+  Ctl* ctl;
+  __device__ ShiftArgs global() const {
+    ShiftArgs c = *this;
+    MTC_GLOBAL(shift_tmp); MTC_GLOBAL(S); MTC_GLOBAL(tab); MTC_GLOBAL(key); MTC_GLOBAL(val); MTC_GLOBAL(src_hist); MTC_GLOBAL(ctl);
+    return c;
+  }
+};
+
+__global__ __launch_bounds__(kThreads) void k_shift_src(Launch<ShiftArgs> L) {
   __shared__ uint32_t h[kLdsBins];
-  const bool lds = nbins <= kLdsBins;
+  const Where w = where(L);
+  const ShiftArgs a = L.args[w.m].global();
+  const bool lds = a.nbins <= kLdsBins;
   if (lds)
-    for (uint32_t b = threadIdx.x; b < nbins; b += kThreads) h[b] = 0;
+    for (uint32_t b = threadIdx.x; b < a.nbins; b += kThreads) h[b] = 0;
   __syncthreads();
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i < S) {
-    const uint64_t e = shift_tmp[i];
+  const uint32_t S = *a.S;
+  for (uint32_t i = w.blk * kThreads + threadIdx.x; i < S; i += w.nblk * kThreads) {
+    const uint64_t e = a.shift_tmp[i];
     const uint32_t node = (uint32_t)e, slot = (uint32_t)(e >> 32);
     uint32_t k = 0, prev = 0;
-    if (slot > mask + 1) {
-      atomicOr(&ctl->err, ERR_CORRUPT);
+    if (slot > a.mask + 1) {
+      atomicOr(&a.ctl->err, ERR_CORRUPT);
     } else {
-      const uint64_t v = tab[slot].val;
+      const uint64_t v = a.tab[slot].val;
       const uint32_t src = (uint32_t)(v >> 32);
       prev = (uint32_t)v;
-      k = src - ref;
-      if (src < ref || src > cur) {
-        atomicOr(&ctl->err, ERR_CORRUPT);
+      k = src - a.ref;
+      if (src < a.ref || src > a.cur) {
+        atomicOr(&a.ctl->err, ERR_CORRUPT);
         k = 0;
       }
     }
-    key[i] = k;
-    val[i] = ((uint64_t)prev << 32) | node;
+    a.key[i] = k;
+    a.val[i] = ((uint64_t)prev << 32) | node;
     if (lds) atomicAdd(&h[k], 1u);
-    else atomicAdd(&src_hist[k], 1u);
+    else atomicAdd(&a.src_hist[k], 1u);
   }
   __syncthreads();
   if (lds)
-    for (uint32_t b = threadIdx.x; b < nbins; b += kThreads)
-      if (h[b]) atomicAdd(&src_hist[b], h[b]);
+    for (uint32_t b = threadIdx.x; b < a.nbins; b += kThreads)
+      if (h[b]) atomicAdd(&a.src_hist[b], h[b]);
 }
 
-// Histogram of one 8-bit digit over a 256-entry segment.
-__global__ __launch_bounds__(kThreads) void k_sort_hist(const uint32_t* key, uint32_t S, uint32_t shift,
-                                                        uint32_t nbins, uint32_t nseg, uint32_t* hist) {
-  __shared__ uint32_t h[256];
-  const uint32_t seg = blockIdx.x, tid = threadIdx.x;
-  h[tid] = 0;
-  __syncthreads();
-  const uint32_t i = seg * kThreads + tid;
-  if (i < S) atomicAdd(&h[(key[i] >> shift) & (nbins - 1)], 1u);
-  __syncthreads();
-  if (tid < nbins) hist[tid * nseg + seg] = h[tid];
-}
-
-// Stable scatter by digit: rank within the segment = earlier lanes of this wave
-// with the same digit + the same digit in earlier waves.
-__global__ __launch_bounds__(kThreads) void k_sort_scatter(const uint32_t* key, const uint64_t* val, uint32_t S,
-                                                           uint32_t shift, uint32_t nbins, uint32_t nseg,
-                                                           const uint32_t* hist_off, uint32_t* key_out,
-                                                           uint64_t* val_out) {
-  __shared__ uint32_t cnt[kThreads / 64][256];
-  const uint32_t seg = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  for (uint32_t j = tid; j < (kThreads / 64) * 256; j += kThreads) (&cnt[0][0])[j] = 0;
-  __syncthreads();
-  const uint32_t i = seg * kThreads + tid;
-  const bool valid = i < S;
-  const uint32_t k = valid ? key[i] : 0u;
-  const uint32_t dg = (k >> shift) & (nbins - 1);
-  bool done = !valid;
-  uint32_t rank = 0;
-  const uint64_t lt = (1ull << lane) - 1;
-  while (true) {
-    const uint64_t pending = __ballot(!done);
-    if (pending == 0) break;
-    const int leader = __ffsll((long long)pending) - 1;
-    const uint32_t ld = __shfl(dg, leader);
-    const bool same = !done && dg == ld;
-    const uint64_t sm = __ballot(same);
-    if (same) {
-      rank = __popcll(sm & lt);
-      done = true;
-    }
-    if ((int)lane == leader) cnt[w][ld] = __popcll(sm);
+// Stable counting sort by one 8-bit digit with a fixed grid: block g of G owns a
+// contiguous range of the S entries (S read on the device), counts its digits,
+// and after a scan of the digit-major (digit, block) counts moves its entries in
+// order.
+struct SortArgs {
+  const uint32_t* key_in;
+  const uint64_t* val_in;
+  uint32_t* key_out;
+  uint64_t* val_out;
+  const uint32_t* S;
+  uint32_t shift, nb;  // digit shift and number of digit values (a power of two <= 256)
+  uint32_t* hist;      // nb * G counts, then their exclusive scan
+  __device__ SortArgs global() const {
+    SortArgs c = *this;
+    MTC_GLOBAL(key_in); MTC_GLOBAL(val_in); MTC_GLOBAL(key_out); MTC_GLOBAL(val_out); MTC_GLOBAL(S); MTC_GLOBAL(hist);
+    return c;
   }
+};
+
+__device__ __forceinline__ void sort_range(uint32_t S, uint32_t g, uint32_t G, uint32_t& lo, uint32_t& hi) {
+  const uint32_t per = ((S + G - 1) / G + kThreads - 1) / kThreads * kThreads;
+  lo = min(S, g * per);
+  hi = min(S, lo + per);
+}
+
+__global__ __launch_bounds__(kThreads) void k_sort_count(Launch<SortArgs> L) {
+  __shared__ uint32_t h[256];
+  const Where w = where(L);
+  const SortArgs a = L.args[w.m].global();
+  h[threadIdx.x] = 0;
   __syncthreads();
-  if (!valid) return;
-  uint32_t pos = hist_off[dg * nseg + seg] + rank;
-  for (uint32_t ww = 0; ww < w; ww++) pos += cnt[ww][dg];
-  key_out[pos] = k;
-  val_out[pos] = val[i];
+  uint32_t lo, hi;
+  sort_range(*a.S, w.blk, w.nblk, lo, hi);
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kThreads) atomicAdd(&h[(a.key_in[i] >> a.shift) & (a.nb - 1)], 1u);
+  __syncthreads();
+  if (threadIdx.x < a.nb) a.hist[threadIdx.x * w.nblk + w.blk] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kThreads) void k_sort_move(Launch<SortArgs> L) {
+  __shared__ uint32_t cnt[kThreads / 64][256];
+  __shared__ uint32_t run[256];
+  const Where wh = where(L);
+  const SortArgs a = L.args[wh.m].global();
+  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint64_t lt = (1ull << lane) - 1;
+  run[tid] = 0;
+  uint32_t lo, hi;
+  sort_range(*a.S, wh.blk, wh.nblk, lo, hi);
+  for (uint32_t r0 = lo; r0 < hi; r0 += kThreads) {
+    for (uint32_t j = tid; j < (kThreads / 64) * 256; j += kThreads) (&cnt[0][0])[j] = 0;
+    __syncthreads();
+    const uint32_t i = r0 + tid;
+    const bool valid = i < hi;
+    const uint32_t k = valid ? a.key_in[i] : 0u;
+    const uint32_t dg = (k >> a.shift) & (a.nb - 1);
+    // rank among earlier lanes of this wave with the same digit
+    bool done = !valid;
+    uint32_t rank = 0;
+    while (true) {
+      const uint64_t pending = __ballot(!done);
+      if (pending == 0) break;
+      const int leader = __ffsll((long long)pending) - 1;
+      const uint32_t ld = __shfl(dg, leader);
+      const bool same = !done && dg == ld;
+      const uint64_t sm = __ballot(same);
+      if (same) {
+        rank = __popcll(sm & lt);
+        done = true;
+      }
+      if ((int)lane == leader) cnt[w][ld] = __popcll(sm);
+    }
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = a.hist[dg * wh.nblk + wh.blk] + run[dg] + rank;
+      for (uint32_t ww = 0; ww < w; ww++) pos += cnt[ww][dg];
+      a.key_out[pos] = k;
+      a.val_out[pos] = a.val_in[i];
+    }
+    __syncthreads();
+    if (tid < a.nb) {
+      uint32_t add = 0;
+      for (uint32_t ww = 0; ww < kThreads / 64; ww++) add += cnt[ww][tid];
+      run[tid] += add;
+    }
+    __syncthreads();
+  }
 }
 
 // (chkpt, count) table in ascending checkpoint id (src/tree_approach.cpp:657-665).
-__global__ __launch_bounds__(kThreads) void k_ptable(const uint32_t* src_hist, uint32_t nbins, uint32_t ref,
-                                                     uint32_t* ptable, Ctl* ctl) {
+struct PtableArgs {
+  const uint32_t* src_hist;
+  uint32_t nbins, ref;
+  uint32_t* ptable;
+  Ctl* ctl;
+  __device__ PtableArgs global() const {
+    PtableArgs c = *this;
+    MTC_GLOBAL(src_hist); MTC_GLOBAL(ptable); MTC_GLOBAL(ctl);
+    return c;
+  }
+};
+
+__global__ __launch_bounds__(kThreads) void k_ptable(Launch<PtableArgs> L) {
   __shared__ uint32_t s_ws[kThreads / 64];
+  const Where w = where(L);
+  const PtableArgs a = L.args[w.m].global();
   uint32_t base = 0;
-  for (uint32_t b0 = 0; b0 < nbins; b0 += kThreads) {
+  for (uint32_t b0 = 0; b0 < a.nbins; b0 += kThreads) {
     const uint32_t b = b0 + threadIdx.x;
-    const uint32_t cnt = b < nbins ? src_hist[b] : 0u;
+    const uint32_t cnt = b < a.nbins ? a.src_hist[b] : 0u;
     uint32_t tot;
     const uint32_t r = block_scan_excl(cnt > 0, &tot, s_ws);
     if (cnt > 0) {
-      ptable[2 * (base + r)] = ref + b;
-      ptable[2 * (base + r) + 1] = cnt;
+      a.ptable[2 * (base + r)] = a.ref + b;
+      a.ptable[2 * (base + r) + 1] = cnt;
     }
     base += tot;
   }
-  if (threadIdx.x == 0) ctl->P = base;
+  if (threadIdx.x == 0) a.ctl->P = base;
 }
 
-__global__ __launch_bounds__(kThreads) void k_region_sizes(const uint32_t* first_roots, uint32_t F, Geo g,
-                                                           uint32_t* sizes) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i < F) sizes[i] = leaf_count(g, first_roots[i]);
+struct RegionArgs {
+  const uint32_t* first_roots;
+  const uint32_t* F;
+  Geo g;
+  uint32_t* sizes;
+  __device__ RegionArgs global() const {
+    RegionArgs c = *this;
+    MTC_GLOBAL(first_roots); MTC_GLOBAL(F); MTC_GLOBAL(sizes);
+    return c;
+  }
+};
+
+__global__ __launch_bounds__(kThreads) void k_region_sizes(Launch<RegionArgs> L) {
+  const Where w = where(L);
+  const RegionArgs a = L.args[w.m].global();
+  const uint32_t F = *a.F;
+  for (uint32_t i = w.blk * kThreads + threadIdx.x; i < F; i += w.nblk * kThreads)
+    a.sizes[i] = leaf_count(a.g, a.first_roots[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1055,20 +1351,20 @@ struct DiffArgs {
   Geo g;
   Ctl* ctl;
   int aligned;
+  __device__ DiffArgs global() const {
+    DiffArgs c = *this;
+    MTC_GLOBAL(diff); MTC_GLOBAL(first_roots); MTC_GLOBAL(ptable); MTC_GLOBAL(shift_sorted); MTC_GLOBAL(region_off); MTC_GLOBAL(total_chunks); MTC_GLOBAL(data); MTC_GLOBAL(ctl);
+    return c;
+  }
 };
 
-__device__ __forceinline__ void put32(uint8_t* p, uint32_t v) {
-  p[0] = (uint8_t)v;
-  p[1] = (uint8_t)(v >> 8);
-  p[2] = (uint8_t)(v >> 16);
-  p[3] = (uint8_t)(v >> 24);
-}
-
-__global__ __launch_bounds__(kThreads) void k_diff_meta(DiffArgs a) {
+__global__ __launch_bounds__(kThreads) void k_diff_meta(Launch<DiffArgs> L) {
+  const Where w = where(L);
+  const DiffArgs a = L.args[w.m].global();
   const uint32_t F = a.ctl->F, P = a.ctl->P, S = a.ctl->S;
   uint32_t* out = reinterpret_cast<uint32_t*>(a.diff);  // every metadata field is 4-byte aligned
   const uint64_t oF = 8, oP = oF + F, oS = oP + 2ull * P;
-  const uint64_t tid = (uint64_t)blockIdx.x * kThreads + threadIdx.x, stride = (uint64_t)gridDim.x * kThreads;
+  const uint64_t tid = (uint64_t)w.blk * kThreads + threadIdx.x, stride = (uint64_t)w.nblk * kThreads;
   for (uint64_t i = tid; i < F; i += stride) out[oF + i] = a.first_roots[i];
   for (uint64_t i = tid; i < 2ull * P; i += stride) out[oP + i] = a.ptable[i];
   for (uint64_t i = tid; i < S; i += stride) {
@@ -1088,7 +1384,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_meta(DiffArgs a) {
     h.num_prior_chkpts = P;
     h.num_shift_dupl = S;
     const uint32_t* hw = reinterpret_cast<const uint32_t*>(&h);
-    for (int w = 0; w < 8; w++) out[w] = hw[w];
+    for (int k = 0; k < 8; k++) out[k] = hw[k];
     a.ctl->data_chunks = chunks;
     a.ctl->diff_size = data_off + chunks * a.C;
   }
@@ -1098,15 +1394,18 @@ typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 
 // Gather the chunks of every first-occurrence region, regions in root order and
 // leaves in ascending chunk order; the short last chunk is zero padded (:636-650).
-__global__ __launch_bounds__(kThreads) void k_gather(DiffArgs a) {
+__global__ __launch_bounds__(kThreads) void k_gather(Launch<DiffArgs> L) {
   __shared__ uint32_t s_src[kThreads];
+  const Where wh = where(L);
+  const DiffArgs a = L.args[wh.m].global();
   const uint32_t F = a.ctl->F;
+  if (F == 0) return;
   const uint64_t total = *a.total_chunks;
   const uint64_t data_off = 32 + 4ull * F + 8ull * a.ctl->P + 8ull * a.ctl->S;
   uint8_t* dst0 = a.diff + data_off;
   const uint32_t C = a.C;
   const uint32_t U = a.aligned ? C / 16 : C;  // copy units per chunk
-  for (uint64_t tile = blockIdx.x; tile * kThreads < total; tile += gridDim.x) {
+  for (uint64_t tile = wh.blk; tile * kThreads < total; tile += wh.nblk) {
     const uint64_t j0 = tile * kThreads;
     const uint64_t j = j0 + threadIdx.x;
     if (j < total) {
@@ -1159,25 +1458,10 @@ __global__ __launch_bounds__(kThreads) void k_rehash(const Slot* old, uint64_t o
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-template <int HK>
-static void launch_leaf(const LeafArgs& a, bool fast, hipStream_t st) {
-  const uint64_t ntiles = (a.n + kThreads - 1) / kThreads;
-  const uint32_t grid = (uint32_t)((ntiles + a.tiles_per_block - 1) / a.tiles_per_block);
-  if (fast) {
-    switch (a.C) {
-      case 16: k_leaf<HK, 16><<<grid, kThreads, 0, st>>>(a); return;
-      case 32: k_leaf<HK, 32><<<grid, kThreads, 0, st>>>(a); return;
-      case 64: k_leaf<HK, 64><<<grid, kThreads, 0, st>>>(a); return;
-      case 128: k_leaf<HK, 128><<<grid, kThreads, 0, st>>>(a); return;
-      case 256: k_leaf<HK, 256><<<grid, kThreads, 0, st>>>(a); return;
-      case 512: k_leaf<HK, 512><<<grid, kThreads, 0, st>>>(a); return;
-      case 1024: k_leaf<HK, 1024><<<grid, kThreads, 0, st>>>(a); return;
-      case 2048: k_leaf<HK, 2048><<<grid, kThreads, 0, st>>>(a); return;
-      case 4096: k_leaf<HK, 4096><<<grid, kThreads, 0, st>>>(a); return;
-      default: break;
-    }
-  }
-  k_leaf<HK, 0><<<grid, kThreads, 0, st>>>(a);
+static uint64_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
 }
 
 template <int HK>
@@ -1209,27 +1493,46 @@ void hash_chunks(const uint8_t* data, uint64_t len, uint32_t C, int hk, Digest* 
   MTC_HIP(hipGetLastError());
 }
 
-static uint64_t next_pow2(uint64_t x) {
-  uint64_t p = 1;
-  while (p < x) p <<= 1;
-  return p;
+// Chunk sizes with an LDS-staged leaf kernel; other sizes hash byte by byte.
+static int staged_cb(uint32_t C, bool aligned) {
+  if (!aligned) return 0;
+  switch (C) {
+    case 16: case 32: case 64: case 128: case 256: case 512: case 1024: case 2048: case 4096: return (int)C;
+    default: return 0;
+  }
 }
 
-static void fill_table(Slot* t, uint64_t slots, hipStream_t st) {
-  const uint64_t words = slots * 4;
-  k_fill_u64<<<grid_for(words, kThreads, 65536), kThreads, 0, st>>>(reinterpret_cast<uint64_t*>(t), words, TEMPTY);
-  MTC_HIP(hipGetLastError());
+template <int HK, int CB>
+static void launch_leaf_cb(Launch<LeafArgs> L, uint32_t grid, hipStream_t st) {
+  k_leaf<HK, CB><<<grid, kThreads, 0, st>>>(L);
 }
 
-// Plan sweep stages and root segments for n chunks.
+template <int HK>
+static void launch_leaf(int cb, Launch<LeafArgs> L, uint32_t grid, hipStream_t st) {
+  switch (cb) {
+    case 16: launch_leaf_cb<HK, 16>(L, grid, st); break;
+    case 32: launch_leaf_cb<HK, 32>(L, grid, st); break;
+    case 64: launch_leaf_cb<HK, 64>(L, grid, st); break;
+    case 128: launch_leaf_cb<HK, 128>(L, grid, st); break;
+    case 256: launch_leaf_cb<HK, 256>(L, grid, st); break;
+    case 512: launch_leaf_cb<HK, 512>(L, grid, st); break;
+    case 1024: launch_leaf_cb<HK, 1024>(L, grid, st); break;
+    case 2048: launch_leaf_cb<HK, 2048>(L, grid, st); break;
+    case 4096: launch_leaf_cb<HK, 4096>(L, grid, st); break;
+    default: launch_leaf_cb<HK, 0>(L, grid, st); break;
+  }
+}
+
+// Plan sweep stages and root segments for n chunks (fuse: levels per stage).
 static void plan_tree(mtc_dedup* h) {
   const Geo g = make_geo(h->n);
+  const uint32_t fuse = std::max<uint32_t>(1, std::min<uint32_t>(kFuse, h->tune.fuse));
   h->stages.clear();
   std::vector<uint32_t> nb(g.D + 1, 1);  // blocks of the stage emitting roots at depth c
   if (h->n > 1) {
     uint32_t d = g.D;
     while (d > 0) {
-      const uint32_t dout = d > kFuse ? d - kFuse : 0;
+      const uint32_t dout = d > fuse ? d - fuse : 0;
       h->stages.push_back(Stage{d, dout});
       for (uint32_t c = dout + 1; c <= d; c++) nb[c] = 1u << dout;
       d = dout;
@@ -1240,13 +1543,17 @@ static void plan_tree(mtc_dedup* h) {
   h->plan_dirty = true;
 }
 
+static uint32_t sort_blocks(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(kSortBlocks, (n + 16 * kThreads - 1) / (16 * kThreads))); }
+
+static uint64_t scan_sums(uint64_t count) { return (count + kScanTile - 1) / kScanTile + 1; }
+
 static void ensure_buffers(mtc_dedup* h, hipStream_t st) {
   const uint64_t n = h->n, nodes = 2 * n - 1;
   h->leaf_dg.ensure(n);
   h->labels.ensure(n);
   h->leaf_slot.ensure(n);
   const Geo g = make_geo(h->n);
-  const uint64_t half_cap = (g.D > kFuse) ? (1ull << (g.D - kFuse)) : 1;
+  const uint64_t half_cap = h->stages.size() > 1 ? (1ull << h->stages[0].d_out) : 1;
   h->fr_lab.ensure(2 * half_cap);
   h->fr_dg.ensure(2 * half_cap);
   h->fr_slot.ensure(2 * half_cap);
@@ -1268,24 +1575,38 @@ static void ensure_buffers(mtc_dedup* h, hipStream_t st) {
   const uint64_t max_blk = (n + kThreads - 1) / kThreads;  // at most one block per tile
   h->blk_cnt.ensure(max_blk);
   h->blk_off.ensure(max_blk + 1);
-  h->tile_flags.ensure(2 * ((g.D > kFuse) ? (1ull << (g.D - kFuse)) : 1) + 4);
+  const uint64_t ntile0 = h->stages.empty() ? 1 : (1ull << h->stages[0].d_out);
+  h->tile_flags.ensure(2 * ntile0 + 4);
   h->first_roots.ensure(n);
   h->shift_tmp.ensure(n);
+  for (int b = 0; b < 2; b++) {
+    h->sort_key[b].ensure(n);
+    h->sort_val[b].ensure(n);
+  }
+  h->sort_hist.ensure(256ull * sort_blocks(h->n));
   h->region_off.ensure(n + 1);
+  // scan partial sums: leaf blocks, two root-segment arrays, the sort, the regions
+  h->scan_ws.ensure(scan_sums(max_blk) + 2 * scan_sums(nseg) + scan_sums(256ull * sort_blocks(h->n)) + scan_sums(n));
   // P <= checkpoints since the baseline; reserve in steps of 4096 so the arena is
   // not reallocated every checkpoint.
   const uint64_t pcap = ((uint64_t)h->current_id + 2 + 4095) / 4096 * 4096;
+  h->src_hist.ensure(pcap);
+  h->ptable.ensure(2 * pcap);
   h->diff.ensure(32 + 12 * n + 8 * pcap + n * (uint64_t)h->chunk_size);
+  h->ctr.ensure(CTR_KINDS * kStripes);
+  (void)g;
 }
 
-static void ensure_table(mtc_dedup* h, bool reset, hipStream_t st) {
+// The table is reset at a baseline (filled by the step's first launch) and grown
+// by a rehash when the next checkpoint could pass the load limit.
+static void prepare_table(mtc_dedup* h, bool reset, hipStream_t st) {
   const uint64_t nodes = 2ull * h->n - 1;
+  const double lf = h->tune.load_factor > 0.05 ? h->tune.load_factor : 0.25;
   if (reset) {
-    const uint64_t cap = next_pow2(std::max<uint64_t>(4 * (uint64_t)h->n, 1024));
+    const uint64_t cap = next_pow2(std::max<uint64_t>((uint64_t)(h->n / lf), 1024));
     h->table.ensure(cap + 1);
     h->table_cap = cap;
     h->table_size = 0;
-    fill_table(h->table.p, cap + 1, st);
     return;
   }
   // Every checkpoint can add at most 2n-1 entries; keep room for all of them, so
@@ -1297,7 +1618,9 @@ static void ensure_table(mtc_dedup* h, bool reset, hipStream_t st) {
   while (!fits(cap)) cap *= 2;
   if (cap > (1ull << 31)) throw Status{MTC_ERR_TABLE_FULL, "table capacity above 2^31 slots"};
   h->table_tmp.ensure(cap + 1);
-  fill_table(h->table_tmp.p, cap + 1, st);
+  k_fill_u64<<<grid_for(4 * (cap + 1), kThreads, 65536), kThreads, 0, st>>>(reinterpret_cast<uint64_t*>(h->table_tmp.p),
+                                                                            4 * (cap + 1), TEMPTY);
+  MTC_HIP(hipGetLastError());
   k_rehash<<<grid_for(h->table_cap + 1, kThreads, 65536), kThreads, 0, st>>>(h->table.p, h->table_cap + 1,
                                                                              h->table_tmp.p, cap - 1, h->ctl_d);
   MTC_HIP(hipGetLastError());
@@ -1306,53 +1629,7 @@ static void ensure_table(mtc_dedup* h, bool reset, hipStream_t st) {
   h->table_cap = cap;
 }
 
-static void sync_ctl(mtc_dedup* h, hipStream_t st) {
-  MTC_HIP(hipMemcpyAsync(h->ctl_h, h->ctl_d, sizeof(Ctl), hipMemcpyDeviceToHost, st));
-  MTC_HIP(hipStreamSynchronize(st));
-  if (h->ctl_h->err & (ERR_TABLE_FULL | ERR_TABLE_SPIN | ERR_OVERFLOW_SLOT))
-    throw Status{MTC_ERR_TABLE_FULL, "digest table insert failed (err bits " + std::to_string(h->ctl_h->err) + ")"};
-  if (h->ctl_h->err) throw Status{MTC_ERR_STATE, "inconsistent table state (err bits " + std::to_string(h->ctl_h->err) + ")"};
-}
-
-template <int HK>
-static void run_sweeps(mtc_dedup* h, SweepArgs sa, hipStream_t st) {
-  const Geo g = make_geo(h->n);
-  const uint64_t half_cap = (g.D > kFuse) ? (1ull << (g.D - kFuse)) : 1;
-  if (h->n == 1) {
-    k_single_leaf<<<1, 64, 0, st>>>(sa);
-    MTC_HIP(hipGetLastError());
-    return;
-  }
-  // Sweep A over all stages, its inserts, then sweep B: every first-occurrence
-  // insert must land before any shifted-duplicate lookup (reference_impl.hpp:74-104).
-  for (int pass = 0; pass < 2; pass++) {
-    if (pass == 1) {
-      const uint32_t nseg = h->seg_base_h.back();
-      k_fill_u32<<<grid_for(nseg, kThreads, 4096), kThreads, 0, st>>>(h->seg_cnt_f.p, nseg, 0);
-      k_fill_u32<<<grid_for(nseg, kThreads, 4096), kThreads, 0, st>>>(h->seg_cnt_s.p, nseg, 0);
-    }
-    for (size_t s = 0; s < h->stages.size(); s++) {
-      SweepArgs a = sa;
-      a.d_in = h->stages[s].d_in;
-      a.d_out = h->stages[s].d_out;
-      a.stage0 = (s == 0);
-
-      const uint64_t in_off = (s % 2 == 1) ? 0 : half_cap, out_off = (s % 2 == 0) ? 0 : half_cap;
-      a.in_lab = h->fr_lab.p + in_off;
-      a.in_dg = h->fr_dg.p + in_off;
-      a.in_slot = h->fr_slot.p + in_off;
-      a.out_lab = h->fr_lab.p + out_off;
-      a.out_dg = h->fr_dg.p + out_off;
-      a.out_slot = h->fr_slot.p + out_off;
-      const uint32_t blocks = (uint32_t)(((1ull << a.d_out) + 3) / 4);  // four wavefront tiles per block
-      if (pass == 0) k_sweep_a<HK><<<blocks, kThreads, 0, st>>>(a);
-      else k_sweep_b<HK><<<blocks, kThreads, 0, st>>>(a);
-      MTC_HIP(hipGetLastError());
-    }
-  }
-}
-
-// Host-side phase timer, printed per checkpoint when MTC_HOST_TRACE is set.
+// Host-side phase timer, printed per step when MTC_HOST_TRACE is set.
 struct HostTrace {
   bool on;
   std::chrono::steady_clock::time_point t0, last;
@@ -1370,193 +1647,571 @@ struct HostTrace {
   }
 };
 
-void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_baseline, hipStream_t st) {
-  HostTrace ht;
-  if (!h->ev[0])
-    for (auto& e : h->ev) MTC_HIP(hipEventCreate(&e));
-  hipEvent_t* ev = h->ev;
-  MTC_HIP(hipEventRecord(ev[0], st));
+// ---------------------------------------------------------------------------
+// One step: a checkpoint of every member, as one sequence of batched launches.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct LaunchRec {
+  uint64_t args = 0, prefix = 0, group = 0;
+  uint32_t count = 0, grid = 0;
+};
+
+// Host image of the step's argument arena (uploaded with one copy).
+struct ArenaImage {
+  std::vector<uint8_t> bytes;
+  uint64_t put(const void* p, uint64_t size) {
+    const uint64_t off = (bytes.size() + 255) & ~255ull;
+    bytes.resize(off + size);
+    if (size) std::memcpy(bytes.data() + off, p, size);
+    return off;
+  }
+  template <class A>
+  LaunchRec add(const std::vector<A>& args, const std::vector<uint32_t>& blocks) {
+    LaunchRec r;
+    r.count = (uint32_t)args.size();
+    std::vector<uint32_t> prefix(r.count + 1, 0);
+    for (uint32_t i = 0; i < r.count; i++) prefix[i + 1] = prefix[i] + blocks[i];
+    r.grid = prefix[r.count];
+    r.args = put(args.data(), sizeof(A) * args.size());
+    r.prefix = put(prefix.data(), sizeof(uint32_t) * prefix.size());
+    if (r.count > 1) {
+      std::vector<uint16_t> group((r.grid + 63) / 64);
+      uint32_t m = 0;
+      for (uint32_t gi = 0; gi < group.size(); gi++) {
+        while (prefix[m + 1] <= 64 * gi) m++;
+        group[gi] = (uint16_t)m;
+      }
+      r.group = put(group.data(), sizeof(uint16_t) * group.size());
+    }
+    return r;
+  }
+};
+
+template <class A>
+Launch<A> bind(const LaunchRec& r, uint8_t* base) {
+  return Launch<A>{reinterpret_cast<const A*>(base + r.args), reinterpret_cast<const uint32_t*>(base + r.prefix),
+                   r.count > 1 ? reinterpret_cast<const uint16_t*>(base + r.group) : nullptr, r.count};
+}
+
+struct Member {
+  mtc_dedup* h;
+  const uint8_t* data;
+  uint64_t len;
+  bool baseline;
+  // derived
+  uint32_t n = 0, cur = 0, nblk_leaf = 0, nseg = 0, nbins = 0, passes = 0, Gs = 0, R0 = 0;
+  uint64_t tpb = 1, ntile0 = 1;
+  int cb = 0;
+  Geo g{};
+  uint32_t* ws_blk = nullptr;  // scan partial sums of this member
+  uint32_t* ws_f = nullptr;
+  uint32_t* ws_s = nullptr;
+  uint32_t* ws_sort = nullptr;
+  uint32_t* ws_reg = nullptr;
+};
+
+// Recorded launches of a step: run eagerly or captured into a graph.
+struct Ops {
+  std::vector<std::function<void(uint8_t*, hipStream_t)>> fn;
+  std::vector<uint64_t> sig;
+  template <class A, class K>
+  void add(uint32_t tag, const LaunchRec& r, K launch) {
+    if (r.grid == 0) return;
+    sig.push_back(((uint64_t)tag << 32) | r.grid);
+    sig.push_back((r.args << 32) ^ (r.prefix << 8) ^ r.group ^ r.count);
+    fn.push_back([r, launch](uint8_t* base, hipStream_t st) { launch(bind<A>(r, base), r.grid, st); });
+  }
+};
+
+}  // namespace
+
+static void prepare_member(Member& M, hipStream_t st) {
+  mtc_dedup* h = M.h;
   const uint64_t C = h->chunk_size;
-  const uint64_t n64 = (len + C - 1) / C;
+  const uint64_t n64 = (M.len + C - 1) / C;
   if (n64 > (1ull << 31)) throw Status{MTC_ERR_INVALID_ARG, "more than 2^31 chunks"};
-  const bool baseline = make_baseline || h->current_id == 0 || !h->has_state;
-  if (!baseline && len != h->data_len)
+  if (M.len == 0) throw Status{MTC_ERR_INVALID_ARG, "empty checkpoint buffer"};
+  M.baseline = M.baseline || h->current_id == 0 || !h->has_state;
+  if (!M.baseline && M.len != h->data_len)
     throw Status{MTC_ERR_STATE, "incremental checkpoint with a different length than the previous one"};
-  if (baseline) {
+  if (M.baseline) {
     h->n = (uint32_t)n64;
-    h->data_len = len;
+    h->data_len = M.len;
     plan_tree(h);
   }
   ensure_buffers(h, st);
-  h->ctr.ensure(CTR_KINDS * kStripes);
-  k_zero_ctl<<<1, kThreads, 0, st>>>(h->ctl_d, h->ctr.p);
-  ensure_table(h, baseline, st);
-  if (baseline) h->baseline_id = h->current_id;
+  prepare_table(h, M.baseline, st);
+  if (M.baseline) h->baseline_id = h->current_id;
   h->has_state = true;
-  const uint32_t n = h->n, cur = h->current_id;
-  const Geo g = make_geo(n);
+  M.n = h->n;
+  M.cur = h->current_id;
+  M.g = make_geo(M.n);
+  M.cb = staged_cb(h->chunk_size, (uintptr_t)M.data % 16 == 0);
+  const uint64_t ntiles = (M.n + kThreads - 1) / kThreads;
+  M.tpb = std::max<uint64_t>(1, h->tune.leaf_tiles_per_block);
+  M.nblk_leaf = (uint32_t)((ntiles + M.tpb - 1) / M.tpb);
+  M.nseg = h->seg_base_h.back();
+  M.nbins = M.cur - h->baseline_id + 1;
+  uint32_t bits = 0;
+  while ((1ull << bits) < M.nbins) bits++;
+  M.passes = (bits + 7) / 8;
+  M.Gs = sort_blocks(M.n);
+  M.R0 = h->stages.empty() ? 0 : h->stages[0].d_in - h->stages[0].d_out;
+  M.ntile0 = h->stages.empty() ? 1 : (1ull << h->stages[0].d_out);
+  uint32_t* w = h->scan_ws.p;
+  M.ws_blk = w;
+  w += scan_sums((M.n + kThreads - 1) / kThreads);
+  M.ws_f = w;
+  w += scan_sums(M.nseg);
+  M.ws_s = w;
+  w += scan_sums(M.nseg);
+  M.ws_sort = w;
+  w += scan_sums(256ull * M.Gs);
+  M.ws_reg = w;
   if (h->debug) {
-    h->dbg_tree.ensure(2ull * n - 1);
-    h->dbg_labels.ensure(2ull * n - 1);
-    k_fill_u64<<<grid_for(4ull * n, kThreads, 65536), kThreads, 0, st>>>(reinterpret_cast<uint64_t*>(h->dbg_tree.p),
-                                                                        2 * (2ull * n - 1), 0);
+    h->dbg_tree.ensure(2ull * M.n - 1);
+    h->dbg_labels.ensure(2ull * M.n + 3);  // whole words for the zero fill
   }
-  MTC_HIP(hipEventRecord(ev[1], st));
-  ht.mark("setup");
+}
 
-  // The leaf kernel runs one contiguous range of tiles per block (kLeafBlocks blocks).
-  const uint64_t ntiles_leaf = (n + kThreads - 1) / kThreads;
-  static const uint64_t leaf_blocks = [] {
-    const char* e = std::getenv("MTC_LEAF_BLOCKS");  // tuning knob for the persistent grid
-    return (e && std::atoll(e) > 0) ? (uint64_t)std::atoll(e) : kLeafBlocks;
-  }();
-  const uint64_t tpb = (ntiles_leaf + leaf_blocks - 1) / leaf_blocks;
-  const uint32_t nblk = (uint32_t)((ntiles_leaf + tpb - 1) / tpb);
-  LeafArgs la{data, len, n, h->chunk_size, h->leaf_dg.p, h->labels.p, baseline ? 1 : 0, h->list.p, h->blk_cnt.p, tpb};
-  const bool fast = (C % 16 == 0) && ((uintptr_t)data % 16 == 0);
-  if (h->hash_kind == HASH_MD5) launch_leaf<HASH_MD5>(la, fast, st);
-  else launch_leaf<HASH_MURMUR3>(la, fast, st);
-  MTC_HIP(hipGetLastError());
-  MTC_HIP(hipEventRecord(ev[4], st));
+static void add_scan(Ops& ops, ArenaImage& img, const std::vector<ScanSeg>& segs) {
+  if (segs.empty()) return;
+  std::vector<uint32_t> tiles, ones(segs.size(), 1);
+  for (const auto& s : segs) tiles.push_back(std::max<uint32_t>(1, std::min<uint32_t>(512, (s.count_max + kScanTile - 1) / kScanTile)));
+  const LaunchRec rt = img.add(segs, tiles), r1 = img.add(segs, ones);
+  ops.add<ScanSeg>(10, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_reduce<<<g, kThreads, 0, st>>>(L); });
+  ops.add<ScanSeg>(11, r1, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_top<<<g, kThreads, 0, st>>>(L); });
+  ops.add<ScanSeg>(12, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_down<<<g, kThreads, 0, st>>>(L); });
+}
 
-  // Dense table inserts over the changed chunks, then FIRST/SHIFT resolution.
-  exclusive_scan_u32(h->blk_cnt.p, h->blk_off.p, nblk, h->blk_off.p + nblk, h->ws, st);
-  k_compact<<<nblk, kThreads, 0, st>>>(h->list.p, h->blk_cnt.p, h->blk_off.p, tpb, h->dense.p);
-  MTC_HIP(hipGetLastError());
-  const uint32_t R0 = h->stages.empty() ? 0 : h->stages[0].d_in - h->stages[0].d_out;
-  const uint64_t ntile0 = h->stages.empty() ? 1 : (1ull << h->stages[0].d_out);
-  k_fill_u32<<<grid_for((2 * ntile0 + 3) / 4), kThreads, 0, st>>>(reinterpret_cast<uint32_t*>(h->tile_flags.p),
-                                                                 (2 * ntile0 + 3) / 4, 0);
-  DenseArgs ia{h->dense.p, h->blk_off.p + nblk, n, h->leaf_dg.p, h->leaf_slot.p, h->labels.p, h->table.p,
-               h->table_cap - 1, cur, h->ctl_d, h->ctr.p, g, R0, h->tile_flags.p, h->tile_flags.p + ntile0};
-  const uint32_t dense_grid = grid_for(n, kThreads, 16384);
-  if (!baseline) {  // a reset table holds nothing to find
-    k_lookup<<<dense_grid, kThreads, 0, st>>>(ia);
-    MTC_HIP(hipGetLastError());
-  }
-  k_insert<<<dense_grid, kThreads, 0, st>>>(ia);
-  MTC_HIP(hipGetLastError());
-  k_resolve<<<dense_grid, kThreads, 0, st>>>(ia);
-  MTC_HIP(hipGetLastError());
-
-  SweepArgs sa{};
-  sa.g = g;
-  sa.cur = cur;
-  sa.leaf_dg = h->leaf_dg.p;
-  sa.labels = h->labels.p;
-  sa.leaf_slot = h->leaf_slot.p;
-  sa.tab = h->table.p;
-  sa.mask = h->table_cap - 1;
-  sa.ctl = h->ctl_d;
-  sa.ctr = h->ctr.p;
-  sa.staging = h->staging.p;
-  sa.cnt_f = h->seg_cnt_f.p;
-  sa.cnt_s = h->seg_cnt_s.p;
-  sa.seg_base = h->seg_base.p;
-  sa.dbg_tree = h->debug ? h->dbg_tree.p : nullptr;
-  sa.dbg_labels = h->debug ? h->dbg_labels.p : nullptr;
-  sa.tile_first = h->tile_flags.p;
-  sa.tile_busy = h->tile_flags.p + ntile0;
-  if (h->hash_kind == HASH_MD5) run_sweeps<HASH_MD5>(h, sa, st);
-  else run_sweeps<HASH_MURMUR3>(h, sa, st);
-  ht.mark("sweeps launched");
-  if (h->debug) {  // leaf digests and labels into the debug views
-    MTC_HIP(hipMemcpyAsync(h->dbg_tree.p + (n - 1), h->leaf_dg.p, 16ull * n, hipMemcpyDeviceToDevice, st));
-    MTC_HIP(hipMemcpyAsync(h->dbg_labels.p + (n - 1), h->labels.p, n, hipMemcpyDeviceToDevice, st));
-  }
-
-  // Root totals, then order them: segments are (depth, block) in node-id order.
-  const uint32_t nseg = h->seg_base_h.back();
-  Ctl* cd = h->ctl_d;
-  exclusive_scan_u32(h->seg_cnt_f.p, h->seg_off_f.p, nseg, &cd->F, h->ws, st);
-  exclusive_scan_u32(h->seg_cnt_s.p, h->seg_off_s.p, nseg, &cd->S, h->ws, st);
-  MTC_HIP(hipEventRecord(ev[2], st));
-  sync_ctl(h, st);
-  ht.mark("sync 1");
-  const uint32_t F = h->ctl_h->F, S = h->ctl_h->S;
-  k_scatter_roots<<<grid_for(nseg), kThreads, 0, st>>>(h->staging.p, h->seg_cnt_f.p, h->seg_cnt_s.p, h->seg_off_f.p,
-                                                       h->seg_off_s.p, h->seg_base.p, g.D + 1, nseg,
-                                                       h->first_roots.p, h->shift_tmp.p);
-  MTC_HIP(hipGetLastError());
-
-  // Shifted entries grouped by source checkpoint (stable, so ascending node id inside).
-  const uint32_t nbins = cur - h->baseline_id + 1;
-  h->src_hist.ensure(nbins);
-  h->ptable.ensure(2ull * nbins);
-  k_fill_u32<<<grid_for(nbins), kThreads, 0, st>>>(h->src_hist.p, nbins, 0);
-  uint64_t* sorted = nullptr;
-  if (S > 0) {
-    for (int b = 0; b < 2; b++) {
-      h->sort_key[b].ensure(S);
-      h->sort_val[b].ensure(S);
+// Builds the whole step's launch sequence for the members.
+static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
+  const size_t B = ms.size();
+  std::vector<uint32_t> ones(B, 1);
+  // 1. per-step zero fills (counters, tile flags, root counts, histograms, debug views)
+  {
+    std::vector<FillSeg> fs;
+    std::vector<uint32_t> fb;
+    auto fill = [&](void* p, uint64_t words, uint32_t v) {
+      if (!words) return;
+      fs.push_back(FillSeg{reinterpret_cast<uint32_t*>(p), words, v});
+      fb.push_back(grid_for(words, kThreads * 4, 4096));
+    };
+    for (auto& M : ms) {
+      mtc_dedup* h = M.h;
+      fill(h->ctr.p, 2ull * CTR_KINDS * kStripes, 0);
+      fill(h->tile_flags.p, (2 * M.ntile0 + 3) / 4, 0);
+      fill(h->seg_cnt_f.p, M.nseg, 0);
+      fill(h->seg_cnt_s.p, M.nseg, 0);
+      fill(h->src_hist.p, M.nbins, 0);
+      if (h->debug) {
+        fill(h->dbg_tree.p, 4 * (2ull * M.n - 1), 0);
+        fill(h->dbg_labels.p, (2ull * M.n - 1 + 3) / 4, 0);
+      }
     }
-    k_shift_src<<<grid_for(S), kThreads, 0, st>>>(h->shift_tmp.p, S, h->table.p, h->table_cap - 1, h->baseline_id, cur,
-                                                 h->sort_key[0].p, h->sort_val[0].p, h->src_hist.p, nbins, cd);
-    MTC_HIP(hipGetLastError());
-    uint32_t bits = 0;
-    while ((1ull << bits) < nbins) bits++;
-    const uint32_t nsg = (S + kThreads - 1) / kThreads;
-    h->sort_hist.ensure(256ull * nsg);
-    int cur_buf = 0;
-    for (uint32_t shift = 0; shift < bits; shift += 8) {
-      const uint32_t db = std::min<uint32_t>(8, bits - shift);
-      const uint32_t nb = 1u << db;
-      k_sort_hist<<<nsg, kThreads, 0, st>>>(h->sort_key[cur_buf].p, S, shift, nb, nsg, h->sort_hist.p);
-      MTC_HIP(hipGetLastError());
-      exclusive_scan_u32(h->sort_hist.p, h->sort_hist.p, (uint64_t)nb * nsg, nullptr, h->ws, st);
-      k_sort_scatter<<<nsg, kThreads, 0, st>>>(h->sort_key[cur_buf].p, h->sort_val[cur_buf].p, S, shift, nb, nsg,
-                                              h->sort_hist.p, h->sort_key[1 - cur_buf].p, h->sort_val[1 - cur_buf].p);
-      MTC_HIP(hipGetLastError());
-      cur_buf = 1 - cur_buf;
+    ops.add<FillSeg>(1, img.add(fs, fb), [](Launch<FillSeg> L, uint32_t g, hipStream_t st) { k_fill<<<g, kThreads, 0, st>>>(L); });
+  }
+  // 2. leaf kernel, one launch per (hash, staged chunk size) group
+  {
+    std::map<std::pair<int, int>, std::vector<size_t>> groups;
+    for (size_t i = 0; i < B; i++) groups[{ms[i].h->hash_kind, ms[i].cb}].push_back(i);
+    for (auto& kv : groups) {
+      std::vector<LeafArgs> la;
+      std::vector<uint32_t> lb;
+      for (size_t i : kv.second) {
+        Member& M = ms[i];
+        mtc_dedup* h = M.h;
+        la.push_back(LeafArgs{M.data, M.len, M.n, h->chunk_size, h->leaf_dg.p, h->labels.p, M.baseline ? 1 : 0,
+                              h->list.p, h->blk_cnt.p, M.tpb});
+        lb.push_back(M.nblk_leaf);
+      }
+      const int hk = kv.first.first, cb = kv.first.second;
+      const uint32_t tag = 100 + (uint32_t)hk * 16 + (uint32_t)(cb ? __builtin_ctz((unsigned)cb) : 0);
+      ops.add<LeafArgs>(tag, img.add(la, lb), [hk, cb](Launch<LeafArgs> L, uint32_t g, hipStream_t st) {
+        if (hk == HASH_MD5) launch_leaf<HASH_MD5>(cb, L, g, st);
+        else launch_leaf<HASH_MURMUR3>(cb, L, g, st);
+      });
     }
-    sorted = h->sort_val[cur_buf].p;
   }
-  k_ptable<<<1, kThreads, 0, st>>>(h->src_hist.p, nbins, h->baseline_id, h->ptable.p, cd);
-  MTC_HIP(hipGetLastError());
+  // 3. dense table passes over the changed chunks
+  {
+    std::vector<ScanSeg> ss;
+    for (auto& M : ms)
+      ss.push_back(ScanSeg{M.h->blk_cnt.p, M.h->blk_off.p, nullptr, M.nblk_leaf, M.h->blk_off.p + M.nblk_leaf, M.ws_blk});
+    add_scan(ops, img, ss);
+    std::vector<CompactArgs> ca;
+    std::vector<uint32_t> cb;
+    std::vector<DenseArgs> da;
+    std::vector<uint32_t> db, dlook;
+    for (auto& M : ms) {
+      mtc_dedup* h = M.h;
+      ca.push_back(CompactArgs{h->list.p, h->blk_cnt.p, h->blk_off.p, M.tpb, h->dense.p, M.nblk_leaf});
+      cb.push_back((M.nblk_leaf + kCompactPer - 1) / kCompactPer);
+      da.push_back(DenseArgs{h->dense.p, h->blk_off.p + M.nblk_leaf, M.n, h->leaf_dg.p, h->leaf_slot.p, h->labels.p,
+                             h->table.p, h->table_cap - 1, M.cur, h->ctl_d, h->ctr.p, M.g, M.R0, h->tile_flags.p,
+                             h->tile_flags.p + M.ntile0});
+      db.push_back(grid_for(M.n, 8 * kThreads, 16384));
+      dlook.push_back(M.baseline ? 0u : db.back());  // a reset table holds nothing to find
+    }
+    ops.add<CompactArgs>(20, img.add(ca, cb), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
+    const LaunchRec rd = img.add(da, db);
+    ops.add<DenseArgs>(21, img.add(da, dlook), [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_lookup<<<g, kThreads, 0, st>>>(L); });
+    ops.add<DenseArgs>(22, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_insert<<<g, kThreads, 0, st>>>(L); });
+    ops.add<DenseArgs>(23, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_resolve<<<g, kThreads, 0, st>>>(L); });
+  }
+  // 4. sweeps: every stage of sweep A (and its inserts) before any stage of sweep B
+  {
+    std::vector<SweepArgs> base(B);
+    size_t max_stages = 0;
+    for (size_t i = 0; i < B; i++) {
+      Member& M = ms[i];
+      mtc_dedup* h = M.h;
+      SweepArgs& sa = base[i];
+      sa = SweepArgs{};
+      sa.g = M.g;
+      sa.cur = M.cur;
+      sa.leaf_dg = h->leaf_dg.p;
+      sa.labels = h->labels.p;
+      sa.leaf_slot = h->leaf_slot.p;
+      sa.tab = h->table.p;
+      sa.mask = h->table_cap - 1;
+      sa.ctl = h->ctl_d;
+      sa.ctr = h->ctr.p;
+      sa.staging = h->staging.p;
+      sa.cnt_f = h->seg_cnt_f.p;
+      sa.cnt_s = h->seg_cnt_s.p;
+      sa.seg_base = h->seg_base.p;
+      sa.dbg_tree = h->debug ? h->dbg_tree.p : nullptr;
+      sa.dbg_labels = h->debug ? h->dbg_labels.p : nullptr;
+      sa.tile_first = h->tile_flags.p;
+      sa.tile_busy = h->tile_flags.p + M.ntile0;
+      max_stages = std::max(max_stages, h->stages.size());
+    }
+    // single-chunk members
+    {
+      std::vector<uint32_t> sb(B, 0);
+      bool any = false;
+      for (size_t i = 0; i < B; i++)
+        if (ms[i].n == 1) sb[i] = 1, any = true;
+      if (any)
+        ops.add<SweepArgs>(30, img.add(base, sb), [](Launch<SweepArgs> L, uint32_t g, hipStream_t st) { k_single_leaf<<<g, 64, 0, st>>>(L); });
+    }
+    for (int pass = 0; pass < 2; pass++) {
+      for (size_t s = 0; s < max_stages; s++) {
+        std::map<int, std::pair<std::vector<SweepArgs>, std::vector<uint32_t>>> byhk;
+        for (size_t i = 0; i < B; i++) {
+          mtc_dedup* h = ms[i].h;
+          auto& slot = byhk[h->hash_kind];
+          SweepArgs a = base[i];
+          uint32_t blocks = 0;
+          if (s < h->stages.size()) {
+            a.d_in = h->stages[s].d_in;
+            a.d_out = h->stages[s].d_out;
+            a.stage0 = (s == 0);
+            const uint64_t half_cap = h->stages.size() > 1 ? (1ull << h->stages[0].d_out) : 1;
+            const uint64_t in_off = (s % 2 == 1) ? 0 : half_cap, out_off = (s % 2 == 0) ? 0 : half_cap;
+            a.in_lab = h->fr_lab.p + in_off;
+            a.in_dg = h->fr_dg.p + in_off;
+            a.in_slot = h->fr_slot.p + in_off;
+            a.out_lab = h->fr_lab.p + out_off;
+            a.out_dg = h->fr_dg.p + out_off;
+            a.out_slot = h->fr_slot.p + out_off;
+            blocks = (uint32_t)(((1ull << a.d_out) + 3) / 4);  // four wavefront tiles per block
+          }
+          slot.first.push_back(a);
+          slot.second.push_back(blocks);
+        }
+        for (auto& kv : byhk) {
+          const int hk = kv.first;
+          const uint32_t tag = 40 + 4 * (uint32_t)pass + (uint32_t)hk;
+          const LaunchRec r = img.add(kv.second.first, kv.second.second);
+          if (pass == 0)
+            ops.add<SweepArgs>(tag, r, [hk](Launch<SweepArgs> L, uint32_t g, hipStream_t st) {
+              if (hk == HASH_MD5) k_sweep_a<HASH_MD5><<<g, kThreads, 0, st>>>(L);
+              else k_sweep_a<HASH_MURMUR3><<<g, kThreads, 0, st>>>(L);
+            });
+          else
+            ops.add<SweepArgs>(tag, r, [hk](Launch<SweepArgs> L, uint32_t g, hipStream_t st) {
+              if (hk == HASH_MD5) k_sweep_b<HASH_MD5><<<g, kThreads, 0, st>>>(L);
+              else k_sweep_b<HASH_MURMUR3><<<g, kThreads, 0, st>>>(L);
+            });
+        }
+      }
+    }
+  }
+  // 5. root totals and compaction (segments are (depth, block) in node-id order)
+  {
+    std::vector<ScanSeg> ss;
+    for (auto& M : ms) {
+      mtc_dedup* h = M.h;
+      ss.push_back(ScanSeg{h->seg_cnt_f.p, h->seg_off_f.p, nullptr, M.nseg, &h->ctl_d->F, M.ws_f});
+      ss.push_back(ScanSeg{h->seg_cnt_s.p, h->seg_off_s.p, nullptr, M.nseg, &h->ctl_d->S, M.ws_s});
+    }
+    add_scan(ops, img, ss);
+    std::vector<RootArgs> ra;
+    std::vector<uint32_t> rb;
+    for (auto& M : ms) {
+      mtc_dedup* h = M.h;
+      ra.push_back(RootArgs{h->staging.p, h->seg_cnt_f.p, h->seg_cnt_s.p, h->seg_off_f.p, h->seg_off_s.p, h->seg_base.p,
+                            M.g.D + 1, M.nseg, h->first_roots.p, h->shift_tmp.p});
+      rb.push_back(grid_for(M.nseg));
+    }
+    ops.add<RootArgs>(50, img.add(ra, rb), [](Launch<RootArgs> L, uint32_t g, hipStream_t st) { k_scatter_roots<<<g, kThreads, 0, st>>>(L); });
+  }
+  // 6. shifted entries: source checkpoint, stable sort by source, (chkpt, count) table
+  std::vector<const uint64_t*> sorted(B);
+  {
+    std::vector<ShiftArgs> sa;
+    std::vector<uint32_t> sb;
+    uint32_t max_passes = 0;
+    for (size_t i = 0; i < B; i++) {
+      Member& M = ms[i];
+      mtc_dedup* h = M.h;
+      sa.push_back(ShiftArgs{h->shift_tmp.p, &h->ctl_d->S, h->table.p, h->table_cap - 1, h->baseline_id, M.cur,
+                             h->sort_key[0].p, h->sort_val[0].p, h->src_hist.p, M.nbins, h->ctl_d});
+      sb.push_back(M.Gs);
+      max_passes = std::max(max_passes, M.passes);
+      sorted[i] = h->sort_val[M.passes % 2].p;
+    }
+    ops.add<ShiftArgs>(60, img.add(sa, sb), [](Launch<ShiftArgs> L, uint32_t g, hipStream_t st) { k_shift_src<<<g, kThreads, 0, st>>>(L); });
+    for (uint32_t p = 0; p < max_passes; p++) {
+      std::vector<SortArgs> so;
+      std::vector<uint32_t> sblk;
+      std::vector<ScanSeg> hs;
+      for (auto& M : ms) {
+        mtc_dedup* h = M.h;
+        uint32_t bits = 0;
+        while ((1ull << bits) < M.nbins) bits++;
+        const uint32_t shift = 8 * p;
+        const uint32_t nb = (p < M.passes) ? (1u << std::min<uint32_t>(8, bits - shift)) : 1u;
+        so.push_back(SortArgs{h->sort_key[p % 2].p, h->sort_val[p % 2].p, h->sort_key[1 - p % 2].p,
+                              h->sort_val[1 - p % 2].p, &h->ctl_d->S, shift, nb, h->sort_hist.p});
+        sblk.push_back(p < M.passes ? M.Gs : 0u);
+        if (p < M.passes) hs.push_back(ScanSeg{h->sort_hist.p, h->sort_hist.p, nullptr, nb * M.Gs, nullptr, M.ws_sort});
+      }
+      const LaunchRec r = img.add(so, sblk);
+      ops.add<SortArgs>(61, r, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_count<<<g, kThreads, 0, st>>>(L); });
+      add_scan(ops, img, hs);
+      ops.add<SortArgs>(62, r, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_move<<<g, kThreads, 0, st>>>(L); });
+    }
+    std::vector<PtableArgs> pa;
+    for (auto& M : ms) pa.push_back(PtableArgs{M.h->src_hist.p, M.nbins, M.h->baseline_id, M.h->ptable.p, M.h->ctl_d});
+    ops.add<PtableArgs>(63, img.add(pa, ones), [](Launch<PtableArgs> L, uint32_t g, hipStream_t st) { k_ptable<<<g, kThreads, 0, st>>>(L); });
+  }
+  // 7. region sizes and offsets, metadata, gathered data, counters
+  {
+    std::vector<RegionArgs> ra;
+    std::vector<uint32_t> rb;
+    std::vector<ScanSeg> ss;
+    for (auto& M : ms) {
+      mtc_dedup* h = M.h;
+      ra.push_back(RegionArgs{h->first_roots.p, &h->ctl_d->F, M.g, h->region_off.p});
+      rb.push_back(grid_for(M.n, 16 * kThreads, 256));
+      ss.push_back(ScanSeg{h->region_off.p, h->region_off.p, &h->ctl_d->F, M.n, h->region_off.p + M.n, M.ws_reg});
+    }
+    ops.add<RegionArgs>(70, img.add(ra, rb), [](Launch<RegionArgs> L, uint32_t g, hipStream_t st) { k_region_sizes<<<g, kThreads, 0, st>>>(L); });
+    add_scan(ops, img, ss);
+    std::vector<DiffArgs> da;
+    std::vector<uint32_t> mb, gb;
+    for (size_t i = 0; i < B; i++) {
+      Member& M = ms[i];
+      mtc_dedup* h = M.h;
+      DiffArgs d{};
+      d.diff = h->diff.p;
+      d.first_roots = h->first_roots.p;
+      d.ptable = h->ptable.p;
+      d.shift_sorted = sorted[i];
+      d.region_off = h->region_off.p;
+      d.total_chunks = h->region_off.p + M.n;
+      d.data = M.data;
+      d.len = M.len;
+      d.n = M.n;
+      d.C = h->chunk_size;
+      d.ref_id = h->baseline_id;
+      d.chkpt_id = M.cur;
+      d.g = M.g;
+      d.ctl = h->ctl_d;
+      d.aligned = ((h->chunk_size % 16) == 0 && (uintptr_t)M.data % 16 == 0) ? 1 : 0;
+      da.push_back(d);
+      mb.push_back(grid_for(M.n, 16 * kThreads, 2048));
+      gb.push_back(grid_for(M.n, 8 * kThreads, 8192));
+    }
+    ops.add<DiffArgs>(71, img.add(da, mb), [](Launch<DiffArgs> L, uint32_t g, hipStream_t st) { k_diff_meta<<<g, kThreads, 0, st>>>(L); });
+    ops.add<DiffArgs>(72, img.add(da, gb), [](Launch<DiffArgs> L, uint32_t g, hipStream_t st) { k_gather<<<g, kThreads, 0, st>>>(L); });
+  }
+}
 
-  // Region sizes and data offsets.
-  if (F > 0) {
-    k_region_sizes<<<grid_for(F), kThreads, 0, st>>>(h->first_roots.p, F, g, h->region_off.p);
+void StepCtx::reset_graph() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  exec = nullptr;
+  sig.clear();
+}
+
+StepCtx::~StepCtx() {
+  reset_graph();
+  if (arena_h) (void)hipHostFree(arena_h);
+  if (pre_h) (void)hipHostFree(pre_h);
+  if (ctl_h) (void)hipHostFree(ctl_h);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+// Runs one checkpoint of every member.  use_graph: replay the step's launches from
+// a captured graph when their shape matches the previous step's.
+void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_graph) {
+  HostTrace ht;
+  const size_t B = ms.size();
+  if (!ctx.ev[0])
+    for (auto& e : ctx.ev) MTC_HIP(hipEventCreate(&e));
+  MTC_HIP(hipEventRecord(ctx.ev[0], st));
+  // Control blocks are cleared before any table work of the step (a rehash
+  // reports into them).
+  {
+    ArenaImage pre;
+    std::vector<FillSeg> fs;
+    std::vector<uint32_t> fb;
+    for (auto& M : ms) {
+      fs.push_back(FillSeg{reinterpret_cast<uint32_t*>(M.h->ctl_d), sizeof(Ctl) / 4, 0});
+      fb.push_back(1);
+    }
+    const LaunchRec r = pre.add(fs, fb);
+    ctx.pre_d.ensure(pre.bytes.size());
+    if (ctx.pre_h_cap < pre.bytes.size()) {
+      if (ctx.pre_h) (void)hipHostFree(ctx.pre_h);
+      ctx.pre_h = nullptr;
+      MTC_HIP(hipHostMalloc((void**)&ctx.pre_h, pre.bytes.size()));
+      ctx.pre_h_cap = pre.bytes.size();
+    }
+    MTC_HIP(hipStreamSynchronize(st));  // the pinned staging may still feed the previous copy
+    std::memcpy(ctx.pre_h, pre.bytes.data(), pre.bytes.size());
+    MTC_HIP(hipMemcpyAsync(ctx.pre_d.p, ctx.pre_h, pre.bytes.size(), hipMemcpyHostToDevice, st));
+    k_fill<<<r.grid, kThreads, 0, st>>>(bind<FillSeg>(r, ctx.pre_d.p));
     MTC_HIP(hipGetLastError());
   }
-  exclusive_scan_u32(h->region_off.p, h->region_off.p, F, h->region_off.p + n, h->ws, st);
-
-  DiffArgs da{};
-  da.diff = h->diff.p;
-  da.first_roots = h->first_roots.p;
-  da.ptable = h->ptable.p;
-  da.shift_sorted = sorted;
-  da.region_off = h->region_off.p;
-  da.total_chunks = h->region_off.p + n;
-  da.data = data;
-  da.len = len;
-  da.n = n;
-  da.C = h->chunk_size;
-  da.ref_id = h->baseline_id;
-  da.chkpt_id = cur;
-  da.g = g;
-  da.ctl = cd;
-  da.aligned = fast ? 1 : 0;
-  k_diff_meta<<<grid_for(std::max<uint64_t>(std::max(F, S), 2ull * nbins), kThreads, 4096), kThreads, 0, st>>>(da);
+  for (auto& M : ms) prepare_member(M, st);
+  // Reset tables of baseline members.
+  for (auto& M : ms)
+    if (M.baseline)
+      k_fill_u64<<<grid_for(4 * (M.h->table_cap + 1), kThreads, 65536), kThreads, 0, st>>>(
+          reinterpret_cast<uint64_t*>(M.h->table.p), 4 * (M.h->table_cap + 1), TEMPTY);
   MTC_HIP(hipGetLastError());
-  if (F > 0) {
-    k_gather<<<grid_for(n, kThreads, 8192), kThreads, 0, st>>>(da);
-    MTC_HIP(hipGetLastError());
+  MTC_HIP(hipEventRecord(ctx.ev[1], st));
+  ht.mark("prepare");
+
+  Ops ops;
+  ArenaImage img;
+  build_ops(ms, ops, img);
+  std::vector<CtrArgs> fa;
+  ctx.ctl_all.ensure(B);
+  for (size_t i = 0; i < B; i++)
+    fa.push_back(CtrArgs{ms[i].h->ctr.p, ms[i].h->blk_off.p + ms[i].nblk_leaf, ms[i].h->ctl_d, ctx.ctl_all.p + i});
+  ops.add<CtrArgs>(90, img.add(fa, std::vector<uint32_t>(B, 1)),
+                   [](Launch<CtrArgs> L, uint32_t g, hipStream_t st) { k_finish<<<g, kStripes, 0, st>>>(L); });
+  ht.mark("build");
+
+  const uint64_t bytes = img.bytes.size();
+  const uint8_t* old_arena = ctx.arena_d.p;
+  ctx.arena_d.ensure(bytes);
+  if (ctx.arena_h_cap < bytes) {
+    if (ctx.arena_h) (void)hipHostFree(ctx.arena_h);
+    ctx.arena_h = nullptr;
+    MTC_HIP(hipHostMalloc((void**)&ctx.arena_h, bytes));
+    ctx.arena_h_cap = bytes;
   }
-  k_reduce_ctr<<<1, kStripes, 0, st>>>(h->ctr.p, h->blk_off.p + nblk, cd);
-  MTC_HIP(hipEventRecord(ev[3], st));
-  sync_ctl(h, st);
-  ht.mark("final sync");
-  h->table_size += h->ctl_h->claims;
-  h->last_diff_size = h->ctl_h->diff_size;
-  h->last_header = mtc_header{h->baseline_id, cur, len, h->chunk_size, h->ctl_h->F, h->ctl_h->P, h->ctl_h->S};
-  MTC_HIP(hipEventElapsedTime(&h->last_ms[0], ev[0], ev[1]));
-  MTC_HIP(hipEventElapsedTime(&h->last_ms[1], ev[1], ev[2]));
-  MTC_HIP(hipEventElapsedTime(&h->last_ms[2], ev[2], ev[3]));
-  MTC_HIP(hipEventElapsedTime(&h->last_ms[4], ev[1], ev[4]));
-  h->last_ms[3] = 0;
-  ht.mark("epilogue");
+  std::memcpy(ctx.arena_h, img.bytes.data(), bytes);
+  MTC_HIP(hipMemcpyAsync(ctx.arena_d.p, ctx.arena_h, bytes, hipMemcpyHostToDevice, st));
+  uint8_t* base = ctx.arena_d.p;
+
+  const bool debug = std::any_of(ms.begin(), ms.end(), [](const Member& M) { return M.h->debug != 0; });
+  if (use_graph && !debug) {
+    ops.sig.push_back(reinterpret_cast<uint64_t>(base));
+    if (!ctx.exec || ctx.sig != ops.sig || old_arena != base) {
+      ctx.reset_graph();
+      hipGraph_t graph = nullptr;
+      MTC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      for (auto& f : ops.fn) f(base, st);
+      MTC_HIP(hipStreamEndCapture(st, &graph));
+      const hipError_t e = hipGraphInstantiate(&ctx.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      MTC_HIP(e);
+      ctx.sig = ops.sig;
+    }
+    MTC_HIP(hipGraphLaunch(ctx.exec, st));
+    MTC_HIP(hipEventRecord(ctx.ev[4], st));
+  } else {
+    // Eager: the leaf launches are timed on their own (ev[1] .. ev[4]).
+    for (size_t i = 0; i < ops.fn.size(); i++) {
+      ops.fn[i](base, st);
+      MTC_HIP(hipGetLastError());
+      const bool leaf = (ops.sig[2 * i] >> 32) >= 100;
+      if (leaf && (i + 1 == ops.fn.size() || (ops.sig[2 * i + 2] >> 32) < 100))
+        MTC_HIP(hipEventRecord(ctx.ev[4], st));
+      if (i == 0) MTC_HIP(hipEventRecord(ctx.ev[1], st));
+    }
+    if (debug)
+      for (auto& M : ms)
+        if (M.h->debug) {  // leaf digests and labels into the debug views
+          MTC_HIP(hipMemcpyAsync(M.h->dbg_tree.p + (M.n - 1), M.h->leaf_dg.p, 16ull * M.n, hipMemcpyDeviceToDevice, st));
+          MTC_HIP(hipMemcpyAsync(M.h->dbg_labels.p + (M.n - 1), M.h->labels.p, M.n, hipMemcpyDeviceToDevice, st));
+        }
+  }
+  ht.mark("launch");
+  if (ctx.ctl_h_cap < B) {
+    if (ctx.ctl_h) (void)hipHostFree(ctx.ctl_h);
+    ctx.ctl_h = nullptr;
+    MTC_HIP(hipHostMalloc((void**)&ctx.ctl_h, B * sizeof(Ctl)));
+    ctx.ctl_h_cap = (uint32_t)B;
+  }
+  MTC_HIP(hipMemcpyAsync(ctx.ctl_h, ctx.ctl_all.p, B * sizeof(Ctl), hipMemcpyDeviceToHost, st));
+  MTC_HIP(hipEventRecord(ctx.ev[3], st));
+  MTC_HIP(hipStreamSynchronize(st));
+  ht.mark("sync");
+  float ms_setup = 0, ms_total = 0, ms_leaf = 0;
+  MTC_HIP(hipEventElapsedTime(&ms_setup, ctx.ev[0], ctx.ev[1]));
+  MTC_HIP(hipEventElapsedTime(&ms_total, ctx.ev[1], ctx.ev[3]));
+  MTC_HIP(hipEventElapsedTime(&ms_leaf, ctx.ev[1], ctx.ev[4]));
+  std::string err;
+  for (size_t i = 0; i < B; i++) {
+    mtc_dedup* h = ms[i].h;
+    const Ctl& c = ctx.ctl_h[i];
+    if (c.err & (ERR_TABLE_FULL | ERR_TABLE_SPIN | ERR_OVERFLOW_SLOT)) {
+      h->has_state = false;  // the next checkpoint of this handle starts a new baseline
+      throw Status{MTC_ERR_TABLE_FULL, "member " + std::to_string(i) + ": digest table insert failed (err bits " +
+                                           std::to_string(c.err) + ")"};
+    }
+    if (c.err) {
+      h->has_state = false;
+      throw Status{MTC_ERR_STATE, "member " + std::to_string(i) + ": inconsistent table state (err bits " +
+                                      std::to_string(c.err) + ")"};
+    }
+    *h->ctl_h = c;
+    h->table_size += c.claims;
+    h->last_diff_size = c.diff_size;
+    h->last_header = mtc_header{h->baseline_id, ms[i].cur, ms[i].len, h->chunk_size, c.F, c.P, c.S};
+    h->last_ms[0] = ms_setup;
+    h->last_ms[1] = ms_total;
+    h->last_ms[2] = 0;
+    h->last_ms[3] = 0;
+    h->last_ms[4] = (use_graph && !debug) ? 0.f : ms_leaf;
+  }
   ht.print();
+}
+
+void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_baseline, hipStream_t st) {
+  std::vector<Member> ms{Member{h, data, len, make_baseline}};
+  run_step(h->step, ms, st, false);
+}
+
+void checkpoint_batch_impl(mtc_dedup* const* hs, const uint8_t* const* datas, const uint64_t* lens,
+                           const int* baselines, uint32_t count, StepCtx& ctx, hipStream_t st, bool use_graph) {
+  std::vector<Member> ms;
+  for (uint32_t i = 0; i < count; i++) {
+    for (uint32_t j = 0; j < i; j++)
+      if (hs[j] == hs[i]) throw Status{MTC_ERR_INVALID_ARG, "a handle appears twice in one batch"};
+    ms.push_back(Member{hs[i], datas[i], lens[i], baselines && baselines[i] != 0});
+  }
+  run_step(ctx, ms, st, use_graph);
 }
 
 }  // namespace mtc

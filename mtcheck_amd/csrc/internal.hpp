@@ -87,6 +87,35 @@ struct Workspace {
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t count, uint32_t* total, Workspace& ws,
                         hipStream_t st);
 
+// Launch state of one step (a checkpoint of one or many deduplicators): the
+// uploaded argument arena, the step's result array and, when steps are replayed
+// from a graph, the captured graph and the launch shape it was captured for.
+struct StepCtx {
+  DevBuf<uint8_t> arena_d, pre_d;
+  uint8_t* arena_h = nullptr;  // pinned staging of the arena
+  uint64_t arena_h_cap = 0;
+  uint8_t* pre_h = nullptr;
+  uint64_t pre_h_cap = 0;
+  DevBuf<Ctl> ctl_all;
+  Ctl* ctl_h = nullptr;  // pinned
+  uint32_t ctl_h_cap = 0;
+  hipEvent_t ev[5] = {};
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> sig;
+  StepCtx() = default;
+  StepCtx(const StepCtx&) = delete;
+  StepCtx& operator=(const StepCtx&) = delete;
+  ~StepCtx();
+  void reset_graph();
+};
+
+// Schedule parameters (set by mtc_set_tuning or the autotuner).
+struct Tune {
+  uint64_t leaf_tiles_per_block = 1;  // 256-chunk tiles each leaf block streams
+  double load_factor = 0.25;          // table entries per slot at a baseline (capacity = n / load_factor)
+  uint32_t fuse = 8;                  // tree levels per sweep stage (1..8)
+};
+
 }  // namespace mtc
 
 // Opaque handle behind include/mtcheck.h.
@@ -130,13 +159,15 @@ struct mtc_dedup {
   mtc::DevBuf<mtc::Digest> dbg_tree;  // debug: full tree, as the serial reference leaves it
   mtc::DevBuf<uint8_t> dbg_labels;    // debug: all node labels
   mtc::DevBuf<unsigned long long> ctr;  // striped per-checkpoint counters
+  mtc::DevBuf<uint32_t> scan_ws;      // partial sums of the step's scans
   mtc::Ctl* ctl_d = nullptr;
   mtc::Ctl* ctl_h = nullptr;          // pinned
   mtc::Workspace ws;
   std::vector<mtc::Stage> stages;
   std::vector<uint32_t> seg_base_h;   // first segment of each depth
   bool plan_dirty = false;            // seg_base_h not yet on the device
-  hipEvent_t ev[5] = {};              // phase timers, created on first use
+  mtc::StepCtx step;                  // launch state of single checkpoints
+  mtc::Tune tune;
   mtc_header last_header{};
   uint64_t last_diff_size = 0;
   float last_ms[5] = {0, 0, 0, 0, 0};  // setup, dedup, collect, copy, leaf kernel

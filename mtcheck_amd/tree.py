@@ -99,6 +99,10 @@ class TreeDeduplicator:
     def num_shift_dupl(self) -> int:  # include/tree_approach.hpp:57-59
         return self.stats().num_shift_dupl
 
+    def set_tuning(self, leaf_tiles_per_block: int = 1, load_factor: float = 0.25, fuse_levels: int = 8) -> None:
+        """Schedule parameters; load factor and fuse depth apply from the next baseline."""
+        check(lib().mtc_set_tuning(self._h, leaf_tiles_per_block, load_factor, fuse_levels), self._h)
+
     def set_debug(self, on: bool = True) -> None:
         check(lib().mtc_set_debug(self._h, 1 if on else 0), self._h)
 
@@ -119,6 +123,15 @@ class TreeDeduplicator:
         self.datasizes = (size.value - meta, meta)  # (:707-708) data size includes the header
         self.timers = [s.ms_setup / 1e3, s.ms_dedup / 1e3, s.ms_collect / 1e3, s.ms_copy / 1e3]
         return hdr, diff
+
+    def copy_last_diff(self) -> bytes:
+        """The last checkpoint's diff, copied from the device."""
+        p = ctypes.c_void_p()
+        size = ctypes.c_uint64()
+        check(lib().mtc_diff_device(self._h, ctypes.byref(p), ctypes.byref(size)), self._h)
+        buf = ctypes.create_string_buffer(size.value)
+        check(lib().mtc_copy_diff(self._h, buf, size.value, None), self._h)
+        return buf.raw
 
     def checkpoint_header(self, data, make_baseline: bool = False, stream=None):
         """checkpoint(header_t&, ptr, len, diff_h, bool): does not advance the id."""
@@ -193,6 +206,57 @@ class TreeDeduplicator:
     def write_restart_log(self, select_chkpt: int, logname: str) -> None:
         with open(logname, "a") as f:
             f.write(f"Tree,{select_chkpt},{self.chunk_size},{self.restart_timers[0]},{self.restart_timers[1]}\n")
+
+
+class Batch:
+    """Many deduplicators checkpointed together (mtc_checkpoint_batch): one sequence
+    of batched launches per step, replayed from a captured hipGraph when the step's
+    shape repeats.  Each member's diff equals what its own checkpoint() returns."""
+
+    def __init__(self, device: int = 0, use_graph: bool = True):
+        b = ctypes.c_void_p()
+        check(lib().mtc_batch_create(ctypes.byref(b), device, 1 if use_graph else 0))
+        self._b = b
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_b", None):
+            lib().mtc_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def checkpoint(self, dedups, datas, make_baseline=False, advance: bool = True, stream=None, copy: bool = False):
+        """One checkpoint of every deduplicator.  make_baseline: bool or one bool per
+        member.  Returns (headers, diff sizes) or, with copy, the host diffs."""
+        n = len(dedups)
+        if len(datas) != n:
+            raise ValueError("one buffer per deduplicator")
+        hs = (ctypes.c_void_p * n)(*[d._h.value for d in dedups])
+        ptrs = (ctypes.c_void_p * n)()
+        lens = (ctypes.c_uint64 * n)()
+        for i, x in enumerate(datas):
+            ptrs[i], lens[i] = _dev_ptr(x)
+        bl = make_baseline if isinstance(make_baseline, (list, tuple)) else [make_baseline] * n
+        base = (ctypes.c_int * n)(*[1 if b else 0 for b in bl])
+        hdrs = (Header * n)()
+        sizes = (ctypes.c_uint64 * n)()
+        rc = lib().mtc_checkpoint_batch(self._b, hs, ptrs, lens, base, n, 1 if advance else 0, _stream_ptr(stream),
+                                        hdrs, sizes)
+        if rc != _lib.MTC_OK:
+            raise _lib.MtcError(rc, (lib().mtc_batch_last_error(self._b) or b"").decode())
+        if not copy:
+            return list(hdrs), list(sizes)
+        out = []
+        for d, size in zip(dedups, sizes):
+            buf = ctypes.create_string_buffer(size)
+            check(lib().mtc_copy_diff(d._h, buf, size, None), d._h)
+            out.append(buf.raw)
+        return out
 
 
 def metadata_breakdown(header, diff: bytes, num_chkpts: int) -> str:

@@ -173,3 +173,37 @@ class CheckpointSeries:
             self.apply_device(buf, s)
         return buf
 
+
+
+# -- multi-buffer workloads ------------------------------------------------------------
+CONFIG3_SEED = 0x4D544333
+CONFIG4_SEED = 0x4D544334
+
+
+def _log_uniform(rng: np.random.Generator, lo: int, hi: int, size: int) -> np.ndarray:
+    return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), size))).astype(np.int64)
+
+
+def batch_members(count: int = 256, seed: int = CONFIG3_SEED, lo: int = 1 << 20, hi: int = 512 << 20,
+                  chunks=(64, 128, 512), change=(0.005, 0.05)):
+    """Independent buffers for batched deduplication: size log-uniform in [lo, hi]
+    bytes, chunk size uniform from `chunks`, change fraction uniform in `change`,
+    each with its own seed.  Returns a list of CheckpointSeries."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = _log_uniform(rng, lo, hi, count)
+    cs = rng.choice(np.array(chunks), size=count)
+    ch = rng.uniform(change[0], change[1], count)
+    seeds = rng.integers(1, 2**62, size=count)
+    return [CheckpointSeries.with_change(int(s), int(c), float(f), seed=int(sd))
+            for s, c, f, sd in zip(sizes, cs, ch, seeds)]
+
+
+def serve_clients(count: int = 64, seed: int = CONFIG4_SEED, lo: int = 4 << 20, hi: int = 2 << 30, chunk: int = 128,
+                  change=(0.001, 0.10)):
+    """Clients of the served workload: fixed submission size log-uniform in [lo, hi],
+    one chunk size, change fraction uniform in `change`, per-client seeds."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = _log_uniform(rng, lo, hi, count)
+    ch = rng.uniform(change[0], change[1], count)
+    seeds = rng.integers(1, 2**62, size=count)
+    return [CheckpointSeries.with_change(int(s), chunk, float(f), seed=int(sd)) for s, f, sd in zip(sizes, ch, seeds)]
