@@ -40,6 +40,12 @@ def parse():
     p.add_argument("--batch-count", type=int, default=256)
     p.add_argument("--batch-steps", type=int, default=10)
     p.add_argument("--no-autotune", action="store_true")
+    p.add_argument("--no-served", action="store_true", help="skip the served (request loop) measurement")
+    p.add_argument("--only-served", action="store_true", help="profiling aid: run the served measurement alone")
+    p.add_argument("--serve-clients", type=int, default=64)
+    p.add_argument("--serve-loads", default="50,100,200,350,500", help="offered loads, GB/s")
+    p.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per load")
+    p.add_argument("--serve-lanes", type=int, default=2)
     p.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                    help="only run a plain device copy of GIB GiB three times (scales the HBM byte counters)")
     return p.parse_args()
@@ -153,6 +159,7 @@ def batched_bench(args):
                        f"from {{64,128,512}}, change U[0.5%,5%] split 2:1, {args.batch_steps} checkpoints",
            "value": round(vb, 2), "unit": "GB/s", "per_buffer_value": round(vp, 2), "speedup": round(vb / vp, 3),
            "ms_per_step": round(sum(tb) / len(tb) * 1e3, 3), "per_buffer_ms_per_step": round(sum(tp) / len(tp) * 1e3, 3),
+           "step_ms": [round(t * 1e3, 2) for t in tb], "per_buffer_step_ms": [round(t * 1e3, 2) for t in tp],
            "bit_exact": bool(exact), "graph": True,
            "tuning": {str(c): list(t) for c, t in sorted(tuned.items())}}
     for d in A + P:
@@ -161,6 +168,100 @@ def batched_bench(args):
     del bufs
     torch.cuda.empty_cache()
     return res
+
+
+def served_bench(args):
+    """Clients with fixed buffer sizes submit checkpoints with Poisson arrivals at a
+    total offered load; the server batches whatever is queued.  A client's next
+    version is produced as soon as its previous checkpoint completes; latency runs
+    from the scheduled arrival to completion (queueing included)."""
+    import collections
+    import heapq
+
+    import numpy as np
+    import torch
+
+    from mtcheck_amd.serve import Server
+    from mtcheck_amd.workload import DevicePlan, serve_clients, stream_key
+    clients = serve_clients(args.serve_clients)
+    sizes = [c.nbytes for c in clients]
+    total = sum(sizes)
+    srv = Server(lanes=args.serve_lanes, max_batch=64, memory_budget=200 << 30)
+    cids = [srv.add_client(c.chunk_size, c.nbytes) for c in clients]
+    bufs = [torch.empty(n, dtype=torch.uint8, device="cuda:0") for n in sizes]
+    gen = torch.cuda.Stream()
+    K = 6  # uploaded plans per client, reused with fresh stream keys
+    plans = [[DevicePlan(c, c.plan(j + 1), "cuda:0") for j in range(K)] for c in clients]
+    step = [0] * len(clients)
+    for c, b in zip(clients, bufs):
+        c.apply_device(b, 0)
+    torch.cuda.synchronize()
+    for t in [srv.submit(ci, b, make_baseline=True) for ci, b in zip(cids, bufs)]:
+        srv.result(t)
+
+    def advance(i):
+        step[i] += 1
+        plans[i][(step[i] - 1) % K].apply(bufs[i], stream_key(clients[i].seed, step[i]), False, gen)
+
+    for i in range(len(clients)):
+        advance(i)
+    gen.synchronize()
+    levels = []
+    rng = np.random.default_rng(4)
+    for load in [float(x) for x in args.serve_loads.split(",")]:
+        lam = load * 1e9 / total  # requests per second per client
+        events = []
+        for i in range(len(clients)):
+            t = rng.exponential(1 / lam)
+            while t < args.serve_seconds:
+                events.append((t, i))
+                t += rng.exponential(1 / lam)
+        heapq.heapify(events)
+        n_events = len(events)
+        backlog = collections.defaultdict(collections.deque)
+        outstanding = {}
+        lat, done_bytes = [], 0
+        start = time.monotonic() + 0.01
+        last_done = start
+        while events or outstanding or any(backlog.values()):
+            now = time.monotonic()
+            busy = False
+            while events and start + events[0][0] <= now:
+                ta, i = heapq.heappop(events)
+                backlog[i].append(start + ta)
+            finished = []
+            for i, (tk, ta) in outstanding.items():
+                r = srv.result(tk, wait=False)
+                if r is not None:
+                    lat.append(r[4] - ta)
+                    done_bytes += sizes[i]
+                    last_done = max(last_done, r[4])
+                    finished.append(i)
+            for i in finished:
+                del outstanding[i]
+                advance(i)
+                busy = True
+            if finished:
+                gen.synchronize()
+            for i, q in backlog.items():
+                if q and i not in outstanding:
+                    outstanding[i] = (srv.submit(cids[i], bufs[i]), q.popleft())
+                    busy = True
+            if not busy:
+                time.sleep(0.0002)
+        lat_ms = np.array(lat) * 1e3
+        levels.append({"offered_gbs": load, "achieved_gbs": round(done_bytes / (last_done - start) / 1e9, 2),
+                       "requests": n_events, "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+                       "p99_ms": round(float(np.percentile(lat_ms, 99)), 3)})
+    st = srv.stats()
+    srv.close()
+    del bufs, plans
+    torch.cuda.empty_cache()
+    return {"workload": f"{len(clients)} clients, log-uniform 4 MiB-2 GiB ({total / 2**30:.1f} GiB), 128 B chunks, "
+                        f"murmur3, change U[0.1%,10%], Poisson arrivals, {args.serve_seconds:g} s per load, "
+                        f"{args.serve_lanes} lanes, budget 200 GiB",
+            "levels": levels, "steps": st["steps"], "served": st["served"],
+            "mean_batch": round(st["served"] / max(1, st["steps"]), 2)}
 
 
 def main():
@@ -174,6 +275,9 @@ def main():
         return
     if args.only_batched:
         print(json.dumps({"batched": batched_bench(args)}), flush=True)
+        return
+    if args.only_served:
+        print(json.dumps({"served": served_bench(args)}), flush=True)
         return
     import numpy as np
     import torch
@@ -245,6 +349,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_batched:
         out["batched"] = batched_bench(args)
+    if not args.no_served:
+        out["served"] = served_bench(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args, hk)
     print(json.dumps(out), flush=True)

@@ -28,7 +28,8 @@ typedef enum mtc_status {
   MTC_ERR_CORRUPT_DIFF = 5,
   MTC_ERR_HIP = 6,
   MTC_ERR_STATE = 7,
-  MTC_ERR_BUFFER_TOO_SMALL = 8
+  MTC_ERR_BUFFER_TOO_SMALL = 8,
+  MTC_PENDING = 9 /* a served request has not finished yet */
 } mtc_status;
 
 /* Approaches: the DedupMode enum, include/utils.hpp:39-49.  Every tree mode is
@@ -155,6 +156,34 @@ int mtc_checkpoint_batch(mtc_batch* b, mtc_dedup* const* handles, const void* co
                          const int* make_baseline, uint32_t count, int advance_id, void* stream,
                          mtc_header* headers_out, uint64_t* diff_sizes_out);
 const char* mtc_batch_last_error(const mtc_batch* b);
+
+/* Served checkpoints (no counterpart in the reference, which checkpoints inside
+ * the application's own loop).  A server keeps each client's tree and table
+ * resident on `device`, admits a client's first checkpoint only while the
+ * estimated resident state of all admitted clients stays within memory_budget
+ * bytes, and runs `lanes` worker threads, each with its own stream: a free lane
+ * takes every queued request it may run (at most one per client, in arrival
+ * order, at most max_batch) as one batched step, so requests arriving while others
+ * run join the next step and the steps of different lanes overlap on the device.
+ * A client's checkpoints run in submission order (the result equals its serial
+ * sequence); the diff of a request stays in the client's handle
+ * (mtc_server_client) until the client's next checkpoint runs.  mtc_server_submit
+ * records an event on `stream`, so the buffer may still be in production there. */
+typedef struct mtc_server mtc_server;
+int mtc_server_create(mtc_server** out, int device, uint64_t memory_budget, uint32_t lanes, uint32_t max_batch);
+int mtc_server_destroy(mtc_server* s);
+int mtc_server_add_client(mtc_server* s, uint32_t chunk_size, int hash_kind, uint64_t max_len, uint32_t* client);
+int mtc_server_client(mtc_server* s, uint32_t client, mtc_dedup** handle);
+int mtc_server_remove_client(mtc_server* s, uint32_t client);
+int mtc_server_submit(mtc_server* s, uint32_t client, const void* dev_data, uint64_t len, int make_baseline,
+                      void* stream, uint64_t* ticket);
+/* MTC_PENDING while unfinished (wait == 0); otherwise the request's status, and the
+ * record is released.  Times are CLOCK_MONOTONIC seconds. */
+int mtc_server_result(mtc_server* s, uint64_t ticket, int wait, mtc_header* header, uint64_t* diff_size,
+                      double* t_submit, double* t_start, double* t_done);
+int mtc_server_stats(mtc_server* s, uint64_t* steps, uint64_t* served, uint64_t* bytes, uint64_t* resident_bytes,
+                     uint32_t* queued);
+const char* mtc_server_last_error(mtc_server* s);
 
 /* Message for the last error on a handle (or the thread's last handle-less call). */
 const char* mtc_last_error(const mtc_dedup* h);

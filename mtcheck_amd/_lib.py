@@ -12,10 +12,11 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libmtcheck_amd.so")
 
 MTC_OK = 0
+MTC_PENDING = 9
 STATUS_NAMES = {
     0: "MTC_OK", 1: "MTC_ERR_INVALID_ARG", 2: "MTC_ERR_UNSUPPORTED_APPROACH", 3: "MTC_ERR_OUT_OF_MEMORY",
     4: "MTC_ERR_TABLE_FULL", 5: "MTC_ERR_CORRUPT_DIFF", 6: "MTC_ERR_HIP", 7: "MTC_ERR_STATE",
-    8: "MTC_ERR_BUFFER_TOO_SMALL",
+    8: "MTC_ERR_BUFFER_TOO_SMALL", 9: "MTC_PENDING",
 }
 
 # Every symbol the public headers declare (checked by the CPU test suite).
@@ -23,7 +24,9 @@ EXPORTS = [
     "mtc_create", "mtc_destroy", "mtc_checkpoint", "mtc_diff_device", "mtc_copy_diff", "mtc_get_stats",
     "mtc_restart", "mtc_digest_host", "mtc_hash_chunks", "mtc_set_debug", "mtc_copy_tree", "mtc_copy_labels",
     "mtc_last_error", "mtc_status_string", "mtc_set_tuning", "mtc_batch_create", "mtc_batch_destroy",
-    "mtc_checkpoint_batch", "mtc_batch_last_error",
+    "mtc_checkpoint_batch", "mtc_batch_last_error", "mtc_server_create", "mtc_server_destroy", "mtc_server_add_client",
+    "mtc_server_client", "mtc_server_remove_client", "mtc_server_submit", "mtc_server_result", "mtc_server_stats",
+    "mtc_server_last_error",
     "mtc_wl_fill_stream", "mtc_wl_fill_runs", "mtc_wl_copy_chunks",
 ]
 
@@ -84,11 +87,23 @@ def lib():
                                            ctypes.POINTER(u64)]
         L.mtc_batch_last_error.argtypes = [vp]
         L.mtc_batch_last_error.restype = ctypes.c_char_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.mtc_server_create.argtypes = [ctypes.POINTER(vp), i32, u64, u32, u32]
+        L.mtc_server_destroy.argtypes = [vp]
+        L.mtc_server_add_client.argtypes = [vp, u32, i32, u64, ctypes.POINTER(u32)]
+        L.mtc_server_client.argtypes = [vp, u32, ctypes.POINTER(vp)]
+        L.mtc_server_remove_client.argtypes = [vp, u32]
+        L.mtc_server_submit.argtypes = [vp, u32, vp, u64, i32, vp, ctypes.POINTER(u64)]
+        L.mtc_server_result.argtypes = [vp, u64, i32, ctypes.POINTER(Header), ctypes.POINTER(u64), dp, dp, dp]
+        L.mtc_server_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                       ctypes.POINTER(u64), ctypes.POINTER(u32)]
+        L.mtc_server_last_error.argtypes = [vp]
+        L.mtc_server_last_error.restype = ctypes.c_char_p
         L.mtc_wl_fill_stream.argtypes = [vp, u64, u64, u64, vp]
         L.mtc_wl_fill_runs.argtypes = [vp, u64, u32, vp, u32, u64, i32, vp]
         L.mtc_wl_copy_chunks.argtypes = [vp, u64, u32, vp, u32, vp, vp]
         for name in EXPORTS:
-            if name not in ("mtc_last_error", "mtc_status_string", "mtc_batch_last_error"):
+            if name not in ("mtc_last_error", "mtc_status_string", "mtc_batch_last_error", "mtc_server_last_error"):
                 getattr(L, name).restype = i32
         _lib = L
     return _lib

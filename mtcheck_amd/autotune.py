@@ -11,7 +11,10 @@ from .tree import MURMUR3, TreeDeduplicator
 from .workload import CheckpointSeries
 
 DEFAULT = (1, 0.25, 8)
-CANDIDATES = list(itertools.product((1, 2, 4, 8), (0.25, 0.4), (6, 8)))
+# The table must hold a whole checkpoint's inserts next to the baseline's, so its
+# load factor stays at 0.25 (a fuller table is rehashed on the next checkpoint).
+CANDIDATES = list(itertools.product((1, 2, 4, 8), (0.25,), (6, 7, 8)))
+MARGIN = 0.03  # a schedule replaces the default only when it is this much faster
 
 
 def time_schedule(series: CheckpointSeries, bufs, hash_kind: int, tuning, reps: int = 2) -> float:
@@ -48,7 +51,9 @@ def autotune(chunk_sizes, hash_kind: int = MURMUR3, sample_bytes: int = 256 << 2
             series.apply_device(v, k)
             bufs.append(v)
         timings = {t: time_schedule(series, bufs, hash_kind, t) for t in (candidates or CANDIDATES)}
-        out[C] = min(timings, key=timings.get)
+        timings[DEFAULT] = min(timings.get(DEFAULT, float("inf")), time_schedule(series, bufs, hash_kind, DEFAULT))
+        best = min(timings, key=timings.get)
+        out[C] = best if timings[best] < (1 - MARGIN) * timings[DEFAULT] else DEFAULT
         del bufs
         torch.cuda.empty_cache()
     return out

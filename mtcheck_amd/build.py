@@ -15,7 +15,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmtcheck_amd.so")
-SOURCES = ["api.hip", "dedup.hip", "restart.hip", "scan.hip", "workload.hip"]
+SOURCES = ["api.hip", "dedup.hip", "restart.hip", "scan.hip", "serve.hip", "workload.hip"]
 CLI = ["dedup_chkpt_files", "restart_chkpt_files"]
 BIN = os.path.join(PKG, "bin")
 ARCH = os.environ.get("MTC_OFFLOAD_ARCH", "gfx950")

@@ -325,6 +325,7 @@ const char* mtc_status_string(int status) {
     case MTC_ERR_HIP: return "HIP runtime error";
     case MTC_ERR_STATE: return "invalid state";
     case MTC_ERR_BUFFER_TOO_SMALL: return "buffer too small";
+    case MTC_PENDING: return "pending";
     default: return "unknown status";
   }
 }

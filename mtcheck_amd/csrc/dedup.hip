@@ -1641,8 +1641,8 @@ struct HostTrace {
     line += std::string(what) + "=" + std::to_string(std::chrono::duration<double, std::micro>(now - last).count()) + "us ";
     last = now;
   }
-  void print() {
-    if (on) std::fprintf(stderr, "[mtc host] %stotal=%.1fus\n", line.c_str(),
+  void print(size_t members) {
+    if (on) std::fprintf(stderr, "[mtc host] members=%zu %stotal=%.1fus\n", members, line.c_str(),
                          std::chrono::duration<double, std::micro>(last - t0).count());
   }
 };
@@ -2195,7 +2195,7 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
     h->last_ms[3] = 0;
     h->last_ms[4] = (use_graph && !debug) ? 0.f : ms_leaf;
   }
-  ht.print();
+  ht.print(B);
 }
 
 void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_baseline, hipStream_t st) {

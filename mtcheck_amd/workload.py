@@ -83,6 +83,44 @@ class StepPlan:
     pairs: np.ndarray     # (m, 2) uint32: destination chunk, source chunk (distinct destinations)
 
 
+class DevicePlan:
+    """A step plan's run and copy lists uploaded to the device."""
+
+    def __init__(self, series: "CheckpointSeries", p: StepPlan, device, stream=None):
+        import torch
+        self.series, self.zero_runs = series, p.zero_runs
+        ctx = torch.cuda.stream(stream) if stream is not None else _nullcontext()
+        with ctx:
+            self.runs = torch.from_numpy(np.ascontiguousarray(p.runs)).to(device) if len(p.runs) else None
+            self.nruns = len(p.runs)
+            self.pairs = torch.from_numpy(np.ascontiguousarray(p.pairs)).to(device) if len(p.pairs) else None
+            self.npairs = len(p.pairs)
+            self.tmp = (torch.empty(self.npairs * series.chunk_size, dtype=torch.uint8, device=device)
+                        if self.npairs else None)
+
+    def apply(self, buf, key: int, fill_all: bool, stream=None) -> None:
+        from ._lib import check, lib
+        s = self.series
+        st = None if stream is None else int(getattr(stream, "cuda_stream", stream))
+        ptr = buf.data_ptr()
+        if fill_all:
+            check(lib().mtc_wl_fill_stream(ptr, 0, s.nbytes, key, st))
+        if self.nruns:
+            check(lib().mtc_wl_fill_runs(ptr, s.nbytes, s.chunk_size, self.runs.data_ptr(), self.nruns, key,
+                                         1 if self.zero_runs else 0, st))
+        if self.npairs:
+            check(lib().mtc_wl_copy_chunks(ptr, s.nbytes, s.chunk_size, self.pairs.data_ptr(), self.npairs,
+                                           self.tmp.data_ptr(), st))
+
+
+class _nullcontext:
+    def __enter__(self):
+        return None
+
+    def __exit__(self, *a):
+        return False
+
+
 class CheckpointSeries:
     def __init__(self, nbytes: int, chunk_size: int, seed: int = CONFIG0_SEED, zero_frac: float = 0.10,
                  zero_max: int = 32, fresh_frac: float = 0.02, fresh_max: int = 64, copy_frac: float = 0.01):
@@ -145,26 +183,17 @@ class CheckpointSeries:
         return buf
 
     # -- device ------------------------------------------------------------------------
-    def apply_device(self, buf, step: int, stream=None) -> None:
-        """Mutate a uint8 cuda tensor in place (kernels of libmtcheck_amd.so)."""
+    def apply_device(self, buf, step: int, stream=None, plan: "DevicePlan | None" = None) -> None:
+        """Mutate a uint8 cuda tensor in place (kernels of libmtcheck_amd.so).  With a
+        stream, the work is queued there and only that stream is waited for.  `plan`
+        reuses an uploaded plan's positions with this step's stream key."""
         import torch
-
-        from ._lib import check, lib
-        p = self.plan(step)
-        st = None if stream is None else int(getattr(stream, "cuda_stream", stream))
-        ptr = buf.data_ptr()
-        if step == 0:
-            check(lib().mtc_wl_fill_stream(ptr, 0, self.nbytes, p.key, st))
-        if len(p.runs):
-            runs = torch.from_numpy(np.ascontiguousarray(p.runs)).to(buf.device)
-            check(lib().mtc_wl_fill_runs(ptr, self.nbytes, self.chunk_size, runs.data_ptr(), len(p.runs), p.key,
-                                         1 if p.zero_runs else 0, st))
-        if len(p.pairs):
-            pairs = torch.from_numpy(np.ascontiguousarray(p.pairs)).to(buf.device)
-            tmp = torch.empty(len(p.pairs) * self.chunk_size, dtype=torch.uint8, device=buf.device)
-            check(lib().mtc_wl_copy_chunks(ptr, self.nbytes, self.chunk_size, pairs.data_ptr(), len(p.pairs),
-                                           tmp.data_ptr(), st))
-        torch.cuda.synchronize(buf.device)
+        dp = plan if plan is not None else DevicePlan(self, self.plan(step), buf.device, stream)
+        dp.apply(buf, stream_key(self.seed, step), step == 0, stream)
+        if stream is None:
+            torch.cuda.synchronize(buf.device)
+        else:
+            stream.synchronize()
 
     def device_buffer(self, step: int, device="cuda:0"):
         import torch
