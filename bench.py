@@ -40,6 +40,10 @@ def parse():
     p.add_argument("--batch-count", type=int, default=256)
     p.add_argument("--batch-steps", type=int, default=10)
     p.add_argument("--no-autotune", action="store_true")
+    p.add_argument("--no-restart", action="store_true", help="skip the restart measurement")
+    p.add_argument("--only-restart", action="store_true", help="profiling aid: run the restart measurement alone")
+    p.add_argument("--restart-gib", type=float, default=16.0)
+    p.add_argument("--restart-steps", type=int, default=20)
     p.add_argument("--no-served", action="store_true", help="skip the served (request loop) measurement")
     p.add_argument("--only-served", action="store_true", help="profiling aid: run the served measurement alone")
     p.add_argument("--serve-clients", type=int, default=64)
@@ -170,6 +174,65 @@ def batched_bench(args):
     return res
 
 
+def state_checksum(raw, M):
+    """sha256 over the Murmur3 digests of every 4 KiB of a device buffer."""
+    import hashlib
+    dg = M.hash_chunks(raw, 4096, M.MURMUR3)
+    return hashlib.sha256(dg.cpu().numpy().tobytes()).hexdigest()
+
+
+def restart_bench(args):
+    """A float64 simulation state (N(0,1), seeded) checkpointed `restart_steps` times
+    with 1% of the values perturbed per step; the diffs stay in HBM.  Every
+    checkpoint k is then rebuilt from diffs 0..k and compared with the generated
+    state (checksum of 4 KiB digests; the last one byte for byte).  Restart GB/s =
+    state bytes / restart time."""
+    import numpy as np
+    import torch
+
+    import mtcheck_amd as M
+    from mtcheck_amd.workload import FloatStateSeries
+    nvals = int(args.restart_gib * (1 << 30)) // 8
+    fs = FloatStateSeries(nvals)
+    vals = fs.initial()
+    raw = vals.view(torch.uint8)
+    d = M.TreeDeduplicator(128)
+    diffs, sums, ck_ms = [], [], []
+    for k in range(args.restart_steps):
+        if k:
+            fs.perturb(vals, k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d.checkpoint_device(raw, make_baseline=(k == 0))
+        ck_ms.append((time.perf_counter() - t0) * 1e3)
+        diffs.append(d.copy_last_diff_device().clone())
+        sums.append(state_checksum(raw, M))
+    d.close()
+    out = torch.empty_like(raw)
+    rs_ms, ok = [], True
+    for k in range(args.restart_steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        M.restart_into(out, diffs[:k + 1], k)
+        torch.cuda.synchronize()
+        rs_ms.append((time.perf_counter() - t0) * 1e3)
+        ok &= state_checksum(out, M) == sums[k]
+    ok &= bool(torch.equal(out, raw))
+    nbytes = raw.numel()
+    diff_bytes = sum(x.numel() for x in diffs)
+    meta = sum(8 * (M.parse_header(x[:32].cpu().numpy().tobytes())["num_shift_dupl"]) for x in diffs)
+    res = {"workload": f"float64 state of {nbytes / 2**30:g} GiB ({nvals} values, N(0,1)), 128 B chunks, murmur3, "
+                       f"{args.restart_steps} checkpoints perturbing 1% of values in runs of 8-4096 values "
+                       "by N(0,1e-3); diffs resident in HBM",
+           "value": round(nbytes / (float(np.mean(rs_ms[1:])) / 1e3) / 1e9, 2), "unit": "GB/s",
+           "restart_ms": [round(x, 3) for x in rs_ms], "checkpoint_ms_mean": round(float(np.mean(ck_ms[1:])), 3),
+           "diff_gib_total": round(diff_bytes / 2**30, 3), "byte_identical": bool(ok),
+           "hbm_frac": round((2 * nbytes + meta) / (float(np.mean(rs_ms[1:])) / 1e3) / HBM_PEAK, 4)}
+    del diffs, out, vals, raw
+    torch.cuda.empty_cache()
+    return res
+
+
 def served_bench(args):
     """Clients with fixed buffer sizes submit checkpoints with Poisson arrivals at a
     total offered load; the server batches whatever is queued.  A client's next
@@ -276,6 +339,9 @@ def main():
     if args.only_batched:
         print(json.dumps({"batched": batched_bench(args)}), flush=True)
         return
+    if args.only_restart:
+        print(json.dumps({"restart": restart_bench(args)}), flush=True)
+        return
     if args.only_served:
         print(json.dumps({"served": served_bench(args)}), flush=True)
         return
@@ -349,6 +415,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_batched:
         out["batched"] = batched_bench(args)
+    if not args.no_restart:
+        out["restart"] = restart_bench(args)
     if not args.no_served:
         out["served"] = served_bench(args)
     if not args.no_cpu_baseline:

@@ -103,7 +103,8 @@ int mtc_checkpoint(mtc_dedup* h, const void* dev_data, uint64_t len, int make_ba
 /* The device diff of the last checkpoint (valid until the next checkpoint). */
 int mtc_diff_device(mtc_dedup* h, const void** dev_diff, uint64_t* size);
 
-/* Device-to-host copy of the last diff, src/tree_approach.cpp:1678-1684.
+/* Copy of the last diff, src/tree_approach.cpp:1678-1684: to host memory (the
+ * reference's D2H copy) or to device memory (keeping a chain resident in HBM).
  * `capacity` must be at least the diff size. */
 int mtc_copy_diff(mtc_dedup* h, void* host_dst, uint64_t capacity, void* stream);
 

@@ -145,7 +145,8 @@ int mtc_copy_diff(mtc_dedup* h, void* host_dst, uint64_t capacity, void* stream)
     MTC_HIP(hipEventCreate(&a));
     MTC_HIP(hipEventCreate(&b));
     MTC_HIP(hipEventRecord(a, st));
-    MTC_HIP(hipMemcpyAsync(host_dst, h->diff.p, h->last_diff_size, hipMemcpyDeviceToHost, st));
+    // host or device destination (unified addressing decides the direction)
+    MTC_HIP(hipMemcpyAsync(host_dst, h->diff.p, h->last_diff_size, hipMemcpyDefault, st));
     MTC_HIP(hipEventRecord(b, st));
     MTC_HIP(hipEventSynchronize(b));
     (void)hipEventElapsedTime(&h->last_ms[3], a, b);

@@ -8,6 +8,8 @@
 // once into a per-chunk (diff, data slot) cover array, then one scatter copy runs.
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "internal.hpp"
 
@@ -235,6 +237,30 @@ __global__ __launch_bounds__(kRThreads) void k_r_copy(const uint64_t* cover, uin
   }
 }
 
+struct RestartScratch {
+  Workspace ws;
+  DevBuf<uint32_t> err, region_off, set_vals;
+  DevBuf<uint64_t> set_keys, cover, meta;
+  uint8_t* stage = nullptr;  // pinned
+  uint64_t stage_cap = 0;
+  uint32_t* totals = nullptr;  // pinned
+  uint32_t totals_cap = 0;
+  std::vector<uint32_t> totals_h;
+};
+
+static std::mutex& scratch_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+// Scratch of one device, kept until the process ends.
+static RestartScratch& scratch_for(int dev) {
+  static std::map<int, RestartScratch*> all;
+  RestartScratch*& r = all[dev];
+  if (!r) r = new RestartScratch();
+  return *r;
+}
+
 static void check_err(uint32_t* err_d, hipStream_t st) {
   uint32_t e = 0;
   MTC_HIP(hipMemcpyAsync(&e, err_d, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -303,23 +329,31 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
     sumF += d.h.num_first_ocur;
   }
 
-  Workspace ws;
-  DevBuf<uint32_t> err, region_off, cum;
-  DevBuf<uint64_t> set_keys, cover, meta;
-  DevBuf<uint32_t> set_vals;
-  err.ensure(1);
-  k_r_fill32<<<1, kRThreads, 0, st>>>(err.p, 1, 0);
-  region_off.ensure(sumF + L);
+  // Scratch buffers are kept per device between restarts (grow-only); one restart
+  // at a time uses them.
+  int dev = 0;
+  MTC_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> guard(scratch_mutex());
+  RestartScratch& rs = scratch_for(dev);
+  Workspace& ws = rs.ws;
+  rs.err.ensure(1);
+  k_r_fill32<<<1, kRThreads, 0, st>>>(rs.err.p, 1, 0);
+  rs.region_off.ensure(sumF + L);
   uint64_t set_cap = 16;
   while (set_cap < 2 * sumF) set_cap <<= 1;
-  set_keys.ensure(set_cap);
-  set_vals.ensure(set_cap);
-  k_r_fill64<<<rgrid(set_cap, 65536), kRThreads, 0, st>>>(set_keys.p, set_cap, kNoCover);
-  cover.ensure(n);
-  k_r_fill64<<<rgrid(n, 65536), kRThreads, 0, st>>>(cover.p, n, kNoCover);
+  rs.set_keys.ensure(set_cap);
+  rs.set_vals.ensure(set_cap);
+  k_r_fill64<<<rgrid(set_cap, 65536), kRThreads, 0, st>>>(rs.set_keys.p, set_cap, kNoCover);
+  rs.cover.ensure(n);
+  k_r_fill64<<<rgrid(n, 65536), kRThreads, 0, st>>>(rs.cover.p, n, kNoCover);
   MTC_HIP(hipGetLastError());
+  uint32_t* err_p = rs.err.p;
+  uint32_t* region_p = rs.region_off.p;
+  uint64_t* cover_p = rs.cover.p;
 
-  // Per-position tables for the kernels: F-root address, region base, data address, data chunks.
+  // Per-position tables for the kernels: F-root address, region base, data address,
+  // data chunks; then every position's running count of shifted entries by source
+  // checkpoint (ascending id, :872-883).
   std::vector<uint64_t> meta_h(4ull * L);
   for (uint32_t p = 0; p < L; p++) {
     meta_h[p] = reinterpret_cast<uint64_t>(ch[p].base + ch[p].off_first);
@@ -327,65 +361,88 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
     meta_h[2 * L + p] = reinterpret_cast<uint64_t>(ch[p].base + ch[p].off_data);
     meta_h[3 * L + p] = ch[p].data_chunks;
   }
-  meta.ensure(4ull * L);
-  MTC_HIP(hipMemcpyAsync(meta.p, meta_h.data(), 8 * meta_h.size(), hipMemcpyHostToDevice, st));
+  std::vector<uint32_t> cum_h;
+  std::vector<uint64_t> cum_off(L, 0);
+  for (uint32_t p = 0; p < L; p++) {
+    const uint32_t idx = ref + p, S = ch[p].h.num_shift_dupl;
+    cum_off[p] = cum_h.size();
+    std::vector<uint32_t> cnt(idx - ref + 1, 0);
+    for (uint32_t t = 0; t < ch[p].h.num_prior_chkpts; t++) {
+      const uint32_t c = ptables[p][2 * t], m = ptables[p][2 * t + 1];
+      if (c < ref || c > idx) throw Status{MTC_ERR_CORRUPT_DIFF, "shifted entries name a checkpoint off the chain"};
+      cnt[c - ref] += m;
+    }
+    for (size_t t = 1; t < cnt.size(); t++) cnt[t] += cnt[t - 1];
+    if (cnt.back() != S) throw Status{MTC_ERR_CORRUPT_DIFF, "prior-checkpoint table does not sum to S"};
+    cum_h.insert(cum_h.end(), cnt.begin(), cnt.end());
+  }
+  // meta and cum share one upload from pinned staging
+  const uint64_t meta_bytes = 8 * meta_h.size(), cum_bytes = 4 * cum_h.size();
+  rs.meta.ensure(meta_h.size() + (cum_h.size() + 1) / 2);
+  if (rs.stage_cap < meta_bytes + cum_bytes) {
+    if (rs.stage) (void)hipHostFree(rs.stage);
+    rs.stage = nullptr;
+    MTC_HIP(hipHostMalloc((void**)&rs.stage, meta_bytes + cum_bytes));
+    rs.stage_cap = meta_bytes + cum_bytes;
+  }
+  MTC_HIP(hipStreamSynchronize(st));  // the staging may still feed the previous restart's copy
+  std::memcpy(rs.stage, meta_h.data(), meta_bytes);
+  std::memcpy(rs.stage + meta_bytes, cum_h.data(), cum_bytes);
+  MTC_HIP(hipMemcpyAsync(rs.meta.p, rs.stage, meta_bytes + cum_bytes, hipMemcpyHostToDevice, st));
+  const uint64_t* meta_p = rs.meta.p;
+  const uint32_t* cum_p = reinterpret_cast<const uint32_t*>(rs.meta.p + meta_h.size());
 
   // Region offsets and the (position, root) set.
   for (uint32_t p = 0; p < L; p++) {
     const ChainDiff& d = ch[p];
     const uint32_t F = d.h.num_first_ocur;
-    uint32_t* ro = region_off.p + d.region_base;
+    uint32_t* ro = region_p + d.region_base;
     if (F) {
-      k_r_sizes<<<rgrid(F), kRThreads, 0, st>>>(d.base + d.off_first, F, g, ro, err.p);
-      k_r_set_insert<<<rgrid(F), kRThreads, 0, st>>>(d.base + d.off_first, F, p, set_keys.p, set_vals.p, set_cap - 1,
-                                                     err.p);
+      k_r_sizes<<<rgrid(F), kRThreads, 0, st>>>(d.base + d.off_first, F, g, ro, err_p);
+      k_r_set_insert<<<rgrid(F), kRThreads, 0, st>>>(d.base + d.off_first, F, p, rs.set_keys.p, rs.set_vals.p,
+                                                     set_cap - 1, err_p);
       MTC_HIP(hipGetLastError());
     }
     exclusive_scan_u32(ro, ro, F, ro + F, ws, st);
   }
-  std::vector<uint32_t> totals(L);
+  rs.totals_h.resize(L);
+  if (rs.totals_cap < L) {
+    if (rs.totals) (void)hipHostFree(rs.totals);
+    rs.totals = nullptr;
+    MTC_HIP(hipHostMalloc((void**)&rs.totals, 4ull * L));
+    rs.totals_cap = L;
+  }
   for (uint32_t p = 0; p < L; p++)
-    MTC_HIP(hipMemcpyAsync(&totals[p], region_off.p + ch[p].region_base + ch[p].h.num_first_ocur, sizeof(uint32_t),
+    MTC_HIP(hipMemcpyAsync(rs.totals + p, region_p + ch[p].region_base + ch[p].h.num_first_ocur, sizeof(uint32_t),
                            hipMemcpyDeviceToHost, st));
-  check_err(err.p, st);
-  for (uint32_t p = 0; p < L; p++)
-    if (totals[p] > ch[p].data_chunks) throw Status{MTC_ERR_CORRUPT_DIFF, "data section shorter than its regions"};
+  check_err(err_p, st);
+  for (uint32_t p = 0; p < L; p++) {
+    rs.totals_h[p] = rs.totals[p];
+    if (rs.totals_h[p] > ch[p].data_chunks) throw Status{MTC_ERR_CORRUPT_DIFF, "data section shorter than its regions"};
+  }
 
   // Cover chunks, oldest diff first so newer diffs overwrite.
   for (uint32_t p = 0; p < L; p++) {
     const ChainDiff& d = ch[p];
     const uint32_t F = d.h.num_first_ocur, S = d.h.num_shift_dupl;
     if (F) {
-      k_r_cover_first<<<rgrid(totals[p], 65536), kRThreads, 0, st>>>(d.base + d.off_first, F,
-                                                                    region_off.p + d.region_base, totals[p], g, p,
-                                                                    cover.p);
+      k_r_cover_first<<<rgrid(rs.totals_h[p], 65536), kRThreads, 0, st>>>(d.base + d.off_first, F,
+                                                                         region_p + d.region_base, rs.totals_h[p], g, p,
+                                                                         cover_p);
       MTC_HIP(hipGetLastError());
     }
     if (S) {
-      // Running count of shifted entries by source checkpoint (ascending id), :872-883.
-      const uint32_t idx = ref + p;
-      std::vector<uint32_t> cnt(idx - ref + 1, 0);
-      for (uint32_t t = 0; t < d.h.num_prior_chkpts; t++) {
-        const uint32_t c = ptables[p][2 * t], m = ptables[p][2 * t + 1];
-        if (c < ref || c > idx) throw Status{MTC_ERR_CORRUPT_DIFF, "shifted entries name a checkpoint off the chain"};
-        cnt[c - ref] += m;
-      }
-      for (size_t t = 1; t < cnt.size(); t++) cnt[t] += cnt[t - 1];
-      if (cnt.back() != S) throw Status{MTC_ERR_CORRUPT_DIFF, "prior-checkpoint table does not sum to S"};
-      cum.ensure(cnt.size());
-      MTC_HIP(hipMemcpyAsync(cum.p, cnt.data(), 4 * cnt.size(), hipMemcpyHostToDevice, st));
-      ShiftArgs sa{d.base + d.off_shift, S, cum.p, (uint32_t)cnt.size(), ref, p, g, set_keys.p, set_vals.p,
-                   set_cap - 1, meta.p, meta.p + L, region_off.p, cover.p, err.p};
+      ShiftArgs sa{d.base + d.off_shift, S, cum_p + cum_off[p], p + 1, ref, p, g, rs.set_keys.p, rs.set_vals.p,
+                   set_cap - 1, meta_p, meta_p + L, region_p, cover_p, err_p};
       k_r_cover_shift<<<rgrid(S), kRThreads, 0, st>>>(sa);
       MTC_HIP(hipGetLastError());
-      MTC_HIP(hipStreamSynchronize(st));  // cnt (host) and cum are reused per position
     }
   }
   const int vec = (C % 16 == 0) && ((uintptr_t)out % 16 == 0);
-  k_r_copy<<<rgrid(n, 8192), kRThreads, 0, st>>>(cover.p, n, (uint32_t)C, hk.datalen, meta.p + 2 * L, meta.p + 3 * L,
-                                                 out, vec, err.p);
+  k_r_copy<<<rgrid(n, 8192), kRThreads, 0, st>>>(cover_p, n, (uint32_t)C, hk.datalen, meta_p + 2 * L, meta_p + 3 * L,
+                                                 out, vec, err_p);
   MTC_HIP(hipGetLastError());
-  check_err(err.p, st);
+  check_err(err_p, st);
 }
 
 }  // namespace mtc

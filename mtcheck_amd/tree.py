@@ -124,6 +124,15 @@ class TreeDeduplicator:
         self.timers = [s.ms_setup / 1e3, s.ms_dedup / 1e3, s.ms_collect / 1e3, s.ms_copy / 1e3]
         return hdr, diff
 
+    def copy_last_diff_device(self, out=None):
+        """The last diff copied into a (new) uint8 cuda tensor."""
+        import torch
+        size = self.stats().diff_size
+        if out is None:
+            out = torch.empty(size, dtype=torch.uint8, device=f"cuda:{self.device}")
+        check(lib().mtc_copy_diff(self._h, out.data_ptr(), out.numel(), None), self._h)
+        return out[:size]
+
     def copy_last_diff(self) -> bytes:
         """The last checkpoint's diff, copied from the device."""
         p = ctypes.c_void_p()

@@ -236,3 +236,46 @@ def serve_clients(count: int = 64, seed: int = CONFIG4_SEED, lo: int = 4 << 20, 
     ch = rng.uniform(change[0], change[1], count)
     seeds = rng.integers(1, 2**62, size=count)
     return [CheckpointSeries.with_change(int(s), chunk, float(f), seed=int(sd)) for s, f, sd in zip(sizes, ch, seeds)]
+
+
+# -- simulation-state workload (float64 values; bytes treated opaquely) -----------------
+CONFIG2_SEED = 0x4D544332
+
+
+class FloatStateSeries:
+    """A float64 array initialised from N(0, 1) (torch's device generator, seeded);
+    each later step adds N(0, sigma) noise to `frac` of the values in contiguous runs
+    of U{run_min..run_max} values at uniform positions (the last run is cut so the
+    run lengths sum to frac * n; a value hit by two runs gets the noise once).  Run positions come from numpy's PCG64 seeded with (seed, step)."""
+
+    def __init__(self, nvalues: int, seed: int = CONFIG2_SEED, frac: float = 0.01, run_min: int = 8,
+                 run_max: int = 4096, sigma: float = 1e-3):
+        self.n, self.seed, self.frac = nvalues, seed, frac
+        self.run_min, self.run_max, self.sigma = run_min, run_max, sigma
+
+    def initial(self, device="cuda:0"):
+        import torch
+        g = torch.Generator(device=device)
+        g.manual_seed(self.seed)
+        return torch.randn(self.n, dtype=torch.float64, device=device, generator=g)
+
+    def perturb(self, values, step: int) -> None:
+        import torch
+        rng = np.random.Generator(np.random.PCG64([self.seed, step]))
+        target = int(round(self.frac * self.n))
+        starts, lens, total = [], [], 0
+        while total < target:
+            ln = int(rng.integers(self.run_min, min(self.run_max, self.n) + 1))
+            ln = min(ln, target - total)  # the last run is cut so the lengths sum to the target
+            st = int(rng.integers(0, self.n - ln + 1))
+            starts.append(st)
+            lens.append(ln)
+            total += ln
+        dev = values.device
+        st = torch.tensor(starts, dtype=torch.int64, device=dev)
+        ln = torch.tensor(lens, dtype=torch.int64, device=dev)
+        base = torch.repeat_interleave(st - (torch.cumsum(ln, 0) - ln), ln)
+        idx = torch.unique(base + torch.arange(int(ln.sum()), device=dev))
+        g = torch.Generator(device=dev)
+        g.manual_seed(self.seed * 1000003 + step)
+        values[idx] += self.sigma * torch.randn(idx.numel(), dtype=torch.float64, device=dev, generator=g)
