@@ -35,6 +35,8 @@ def parse():
     p.add_argument("--cpu-sample-mib", type=int, default=1024)
     p.add_argument("--cpu-steps", type=int, default=4)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--sweep", action="store_true",
+                   help="chunk sizes {64,128,512,4096} x {murmur3,md5} x change {0.1,2,20}%%, one JSON line each")
     p.add_argument("--no-batched", action="store_true", help="skip the batched multi-buffer measurement")
     p.add_argument("--only-batched", action="store_true", help="profiling aid: run the batched measurement alone")
     p.add_argument("--batch-count", type=int, default=256)
@@ -327,6 +329,82 @@ def served_bench(args):
             "mean_batch": round(st["served"] / max(1, st["steps"]), 2)}
 
 
+def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, warmup: int) -> dict:
+    """Incremental checkpoints of one buffer resident in HBM; every version is
+    generated before the timed region (version k = version k-1 + step k)."""
+    import numpy as np
+    import torch
+
+    import mtcheck_amd as M
+    from mtcheck_amd.workload import CheckpointSeries
+    hk = M.MD5 if hash_name == "md5" else M.MURMUR3
+    n = (nbytes + C - 1) // C
+    total = warmup + steps
+    series = CheckpointSeries.with_change(nbytes, C, change)
+    dev = torch.device("cuda:0")
+    versions = []
+    for k in range(total):
+        v = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        if k:
+            v.copy_(versions[-1])
+        series.apply_device(v, k)
+        versions.append(v)
+    torch.cuda.synchronize()
+    d = M.TreeDeduplicator(C, hash_kind=hk)
+    for k in range(warmup):
+        d.checkpoint_device(versions[k], make_baseline=(k == 0))
+    torch.cuda.synchronize()
+    lat, leaf_ms, changed, balg, dsize = [], [], [], [], []
+    t0 = time.perf_counter()
+    for k in range(warmup, total):
+        s0 = time.perf_counter()
+        hdr, _, size = d.checkpoint_device(versions[k])
+        lat.append(time.perf_counter() - s0)
+        st = d.stats()
+        leaf_ms.append(st.ms_leaf)
+        changed.append(st.changed_chunks)
+        dsize.append(size)
+        meta = 32 + 4 * hdr.num_first_ocur + 8 * hdr.num_prior_chkpts + 8 * hdr.num_shift_dupl
+        balg.append(nbytes + 32 * n + 2 * (size - meta) + meta)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    # Leaf kernel: C B read + 16 B previous digest read + 1 B label per chunk,
+    # 16 B digest + 4 B slot written per changed chunk.
+    leaf_bytes = [nbytes + 17 * n + 20 * c for c in changed]
+    leaf_avg_ms = float(np.mean(leaf_ms))
+    achieved = float(np.mean(leaf_bytes)) / (leaf_avg_ms / 1e3)
+    del versions
+    d.close()
+    torch.cuda.empty_cache()
+    return {
+        "value": round(nbytes * steps / elapsed / 1e9, 3), "ms_per_step": round(elapsed / steps * 1e3, 4),
+        "config": {"workload": f"plain single-buffer dedup, {nbytes / 2**30:g} GiB uint8, {C} B chunks, "
+                               f"{hash_name}, {change * 100:g}% change per checkpoint (2:1 fresh runs:copies)",
+                   "size_bytes": nbytes, "chunk_size": C, "hash": hash_name, "change": change},
+        "hbm_frac_checkpoint": round(float(np.mean(balg)) / (elapsed / steps) / HBM_PEAK, 4),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+        "diff_mib_mean": round(float(np.mean(dsize)) / (1 << 20), 2),
+        "roofline": {"kernel": "k_leaf (hash + fixed test + table insert)", "bound": "hbm",
+                     "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": leaf_traffic(nbytes, C, hash_name, change),
+                     "avg_ms": round(leaf_avg_ms, 4)},
+    }
+
+
+def sweep(args):
+    """Chunk sizes x hashes x change rates on one buffer; one JSON line each."""
+    nbytes = int(args.size_gib * (1 << 30))
+    for C in (64, 128, 512, 4096):
+        for h in ("murmur3", "md5"):
+            for ch in (0.001, 0.02, 0.2):
+                r = plain_run(nbytes, C, h, ch, args.steps, args.warmup)
+                print(json.dumps({"config": r["config"], "value": r["value"], "unit": "GB/s",
+                                  "ms_per_step": r["ms_per_step"], "hbm_frac_checkpoint": r["hbm_frac_checkpoint"],
+                                  "leaf_gbs": r["roofline"]["achieved"], "leaf_frac": r["roofline"]["frac"],
+                                  "leaf_ms": r["roofline"]["avg_ms"], "diff_mib_mean": r["diff_mib_mean"]}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -345,74 +423,19 @@ def main():
     if args.only_served:
         print(json.dumps({"served": served_bench(args)}), flush=True)
         return
-    import numpy as np
-    import torch
-
     import mtcheck_amd as M
-    from mtcheck_amd.workload import CheckpointSeries
-
-    hk = M.MD5 if args.hash == "md5" else M.MURMUR3
-    nbytes = int(args.size_gib * (1 << 30))
-    C = args.chunk
-    n = (nbytes + C - 1) // C
-    total = args.warmup + args.steps
-    series = CheckpointSeries.with_change(nbytes, C, args.change)
-    dev = torch.device("cuda:0")
-    # All versions resident before timing: version k = version k-1 + step k.
-    versions = []
-    for k in range(total):
-        v = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-        if k:
-            v.copy_(versions[-1])
-        series.apply_device(v, k)
-        versions.append(v)
-    torch.cuda.synchronize()
-    d = M.TreeDeduplicator(C, hash_kind=hk)
-    for k in range(args.warmup):
-        d.checkpoint_device(versions[k], make_baseline=(k == 0))
-    torch.cuda.synchronize()
-    lat, leaf_ms, changed, balg, dsize = [], [], [], [], []
-    t0 = time.perf_counter()
-    for k in range(args.warmup, total):
-        s0 = time.perf_counter()
-        hdr, _, size = d.checkpoint_device(versions[k])
-        lat.append(time.perf_counter() - s0)
-        st = d.stats()
-        leaf_ms.append(st.ms_leaf)
-        changed.append(st.changed_chunks)
-        dsize.append(size)
-        meta = 32 + 4 * hdr.num_first_ocur + 8 * hdr.num_prior_chkpts + 8 * hdr.num_shift_dupl
-        balg.append(nbytes + 32 * n + 2 * (size - meta) + meta)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    value = nbytes * args.steps / elapsed / 1e9
-    ms_step = elapsed / args.steps * 1e3
-    # Leaf kernel: C B read + 16 B previous digest read + 1 B label per chunk,
-    # 16 B digest + 4 B slot written per changed chunk.
-    leaf_bytes = [nbytes + 17 * n + 20 * c for c in changed]
-    leaf_avg_ms = float(np.mean(leaf_ms))
-    achieved = float(np.mean(leaf_bytes)) / (leaf_avg_ms / 1e3)
-    ckpt_frac = float(np.mean(balg)) / (elapsed / args.steps) / HBM_PEAK
+    if args.sweep:
+        sweep(args)
+        return
+    r = plain_run(int(args.size_gib * (1 << 30)), args.chunk, args.hash, args.change, args.steps, args.warmup)
     out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
-        "config": {"workload": f"plain single-buffer dedup, {args.size_gib:g} GiB uint8, {C} B chunks, "
-                               f"{args.hash}, {args.change * 100:g}% change per checkpoint (2:1 fresh runs:copies)",
-                   "size_bytes": nbytes, "chunk_size": C, "hash": args.hash, "change": args.change},
-        "hbm_frac_checkpoint": round(ckpt_frac, 4),
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
-                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
-        "diff_mib_mean": round(float(np.mean(dsize)) / (1 << 20), 2),
-        "roofline": {"kernel": "k_leaf (hash + fixed test + table insert)", "bound": "hbm",
-                     "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": leaf_traffic(nbytes, C, args.hash, args.change),
-                     "avg_ms": round(leaf_avg_ms, 4)},
+        "metric": METRIC, "value": r["value"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic", "config": r["config"],
+        "hbm_frac_checkpoint": r["hbm_frac_checkpoint"], "latency_ms": r["latency_ms"],
+        "diff_mib_mean": r["diff_mib_mean"], "roofline": r["roofline"],
     }
-    del versions
-    d.close()
-    torch.cuda.empty_cache()
+    hk = M.MD5 if args.hash == "md5" else M.MURMUR3
     if not args.no_batched:
         out["batched"] = batched_bench(args)
     if not args.no_restart:

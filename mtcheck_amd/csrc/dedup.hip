@@ -1597,25 +1597,7 @@ static void ensure_buffers(mtc_dedup* h, hipStream_t st) {
   (void)g;
 }
 
-// The table is reset at a baseline (filled by the step's first launch) and grown
-// by a rehash when the next checkpoint could pass the load limit.
-static void prepare_table(mtc_dedup* h, bool reset, hipStream_t st) {
-  const uint64_t nodes = 2ull * h->n - 1;
-  const double lf = h->tune.load_factor > 0.05 ? h->tune.load_factor : 0.25;
-  if (reset) {
-    const uint64_t cap = next_pow2(std::max<uint64_t>((uint64_t)(h->n / lf), 1024));
-    h->table.ensure(cap + 1);
-    h->table_cap = cap;
-    h->table_size = 0;
-    return;
-  }
-  // Every checkpoint can add at most 2n-1 entries; keep room for all of them, so
-  // probing always ends (the reference rehashes to size + num_nodes,
-  // src/tree_approach.cpp:1622-1623), and grow early once the table is 80% full.
-  auto fits = [&](uint64_t cap) { return h->table_size + nodes < cap && h->table_size <= cap / 10 * 8; };
-  if (fits(h->table_cap)) return;
-  uint64_t cap = h->table_cap;
-  while (!fits(cap)) cap *= 2;
+static void rehash_to(mtc_dedup* h, uint64_t cap, hipStream_t st) {
   if (cap > (1ull << 31)) throw Status{MTC_ERR_TABLE_FULL, "table capacity above 2^31 slots"};
   h->table_tmp.ensure(cap + 1);
   k_fill_u64<<<grid_for(4 * (cap + 1), kThreads, 65536), kThreads, 0, st>>>(reinterpret_cast<uint64_t*>(h->table_tmp.p),
@@ -1627,6 +1609,34 @@ static void prepare_table(mtc_dedup* h, bool reset, hipStream_t st) {
   std::swap(h->table, h->table_tmp);
   h->table_tmp.release();
   h->table_cap = cap;
+}
+
+// The table is reset at a baseline (filled by the step's first launch).  Probing
+// always ends while one slot stays empty, so a checkpoint may start only if the
+// table holds all its possible inserts (the reference rehashes to size +
+// num_nodes, src/tree_approach.cpp:1622-1623).  Returns false when the worst case
+// (2n-1 new entries) fits and the table is at most 80% full: the step runs in one
+// piece.  Otherwise the step pauses after the leaf kernel, when an exact bound is
+// known (twice the chunks to insert: every new internal node has two new leaves
+// below it), and grow_exact rehashes only if that bound does not fit.
+static bool prepare_table(mtc_dedup* h, bool reset, hipStream_t st) {
+  const uint64_t nodes = 2ull * h->n - 1;
+  const double lf = h->tune.load_factor > 0.05 ? h->tune.load_factor : 0.25;
+  (void)st;
+  if (reset) {
+    const uint64_t cap = next_pow2(std::max<uint64_t>((uint64_t)(h->n / lf), 1024));
+    h->table.ensure(cap + 1);
+    h->table_cap = cap;
+    h->table_size = 0;
+  }
+  return !(h->table_size + nodes < h->table_cap && h->table_size <= h->table_cap / 10 * 8);
+}
+
+static void grow_exact(mtc_dedup* h, uint64_t to_insert, hipStream_t st) {
+  const uint64_t need = h->table_size + 2 * to_insert + 1;
+  uint64_t cap = h->table_cap;
+  while (need > cap / 10 * 9 || h->table_size > cap / 10 * 8) cap *= 2;
+  if (cap != h->table_cap) rehash_to(h, cap, st);
 }
 
 // Host-side phase timer, printed per step when MTC_HOST_TRACE is set.
@@ -1703,6 +1713,7 @@ struct Member {
   uint32_t n = 0, cur = 0, nblk_leaf = 0, nseg = 0, nbins = 0, passes = 0, Gs = 0, R0 = 0;
   uint64_t tpb = 1, ntile0 = 1;
   int cb = 0;
+  bool split = false;  // the table bound is checked after the leaf kernel
   Geo g{};
   uint32_t* ws_blk = nullptr;  // scan partial sums of this member
   uint32_t* ws_f = nullptr;
@@ -1741,7 +1752,7 @@ static void prepare_member(Member& M, hipStream_t st) {
     plan_tree(h);
   }
   ensure_buffers(h, st);
-  prepare_table(h, M.baseline, st);
+  M.split = prepare_table(h, M.baseline, st);
   if (M.baseline) h->baseline_id = h->current_id;
   h->has_state = true;
   M.n = h->n;
@@ -1785,10 +1796,12 @@ static void add_scan(Ops& ops, ArenaImage& img, const std::vector<ScanSeg>& segs
   ops.add<ScanSeg>(12, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_down<<<g, kThreads, 0, st>>>(L); });
 }
 
-// Builds the whole step's launch sequence for the members.
-static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
+// Builds the step's launch sequence for the members: phase 1 (fills, leaf kernel,
+// scan of the leaf blocks' counts), phase 2 (the rest), or both (phase 0).
+static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int phase) {
   const size_t B = ms.size();
   std::vector<uint32_t> ones(B, 1);
+  if (phase != 2) {
   // 1. per-step zero fills (counters, tile flags, root counts, histograms, debug views)
   {
     std::vector<FillSeg> fs;
@@ -1834,12 +1847,16 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       });
     }
   }
-  // 3. dense table passes over the changed chunks
   {
     std::vector<ScanSeg> ss;
     for (auto& M : ms)
       ss.push_back(ScanSeg{M.h->blk_cnt.p, M.h->blk_off.p, nullptr, M.nblk_leaf, M.h->blk_off.p + M.nblk_leaf, M.ws_blk});
     add_scan(ops, img, ss);
+  }
+  }
+  if (phase == 1) return;
+  // 3. dense table passes over the changed chunks
+  {
     std::vector<CompactArgs> ca;
     std::vector<uint32_t> cb;
     std::vector<DenseArgs> da;
@@ -2053,6 +2070,7 @@ StepCtx::~StepCtx() {
   if (arena_h) (void)hipHostFree(arena_h);
   if (pre_h) (void)hipHostFree(pre_h);
   if (ctl_h) (void)hipHostFree(ctl_h);
+  if (cnt_h) (void)hipHostFree(cnt_h);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
 }
@@ -2099,63 +2117,95 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
   MTC_HIP(hipEventRecord(ctx.ev[1], st));
   ht.mark("prepare");
 
-  Ops ops;
-  ArenaImage img;
-  build_ops(ms, ops, img);
-  std::vector<CtrArgs> fa;
-  ctx.ctl_all.ensure(B);
-  for (size_t i = 0; i < B; i++)
-    fa.push_back(CtrArgs{ms[i].h->ctr.p, ms[i].h->blk_off.p + ms[i].nblk_leaf, ms[i].h->ctl_d, ctx.ctl_all.p + i});
-  ops.add<CtrArgs>(90, img.add(fa, std::vector<uint32_t>(B, 1)),
-                   [](Launch<CtrArgs> L, uint32_t g, hipStream_t st) { k_finish<<<g, kStripes, 0, st>>>(L); });
-  ht.mark("build");
-
-  const uint64_t bytes = img.bytes.size();
-  const uint8_t* old_arena = ctx.arena_d.p;
-  ctx.arena_d.ensure(bytes);
-  if (ctx.arena_h_cap < bytes) {
-    if (ctx.arena_h) (void)hipHostFree(ctx.arena_h);
-    ctx.arena_h = nullptr;
-    MTC_HIP(hipHostMalloc((void**)&ctx.arena_h, bytes));
-    ctx.arena_h_cap = bytes;
-  }
-  std::memcpy(ctx.arena_h, img.bytes.data(), bytes);
-  MTC_HIP(hipMemcpyAsync(ctx.arena_d.p, ctx.arena_h, bytes, hipMemcpyHostToDevice, st));
-  uint8_t* base = ctx.arena_d.p;
-
   const bool debug = std::any_of(ms.begin(), ms.end(), [](const Member& M) { return M.h->debug != 0; });
-  if (use_graph && !debug) {
-    ops.sig.push_back(reinterpret_cast<uint64_t>(base));
-    if (!ctx.exec || ctx.sig != ops.sig || old_arena != base) {
-      ctx.reset_graph();
-      hipGraph_t graph = nullptr;
-      MTC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      for (auto& f : ops.fn) f(base, st);
-      MTC_HIP(hipStreamEndCapture(st, &graph));
-      const hipError_t e = hipGraphInstantiate(&ctx.exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      MTC_HIP(e);
-      ctx.sig = ops.sig;
+  const bool split = std::any_of(ms.begin(), ms.end(), [](const Member& M) { return M.split; });
+  ctx.ctl_all.ensure(B);
+  auto add_finish = [&](Ops& ops, ArenaImage& img) {
+    std::vector<CtrArgs> fa;
+    for (size_t i = 0; i < B; i++)
+      fa.push_back(CtrArgs{ms[i].h->ctr.p, ms[i].h->blk_off.p + ms[i].nblk_leaf, ms[i].h->ctl_d, ctx.ctl_all.p + i});
+    ops.add<CtrArgs>(90, img.add(fa, std::vector<uint32_t>(B, 1)),
+                     [](Launch<CtrArgs> L, uint32_t g, hipStream_t st) { k_finish<<<g, kStripes, 0, st>>>(L); });
+  };
+  // Uploads the arena image and runs the launches, replayed from a graph or eagerly
+  // (eager runs time the leaf launches on their own: ev[1] .. ev[4]).
+  auto run_ops = [&](Ops& ops, ArenaImage& img, bool graph) {
+    const uint64_t bytes = img.bytes.size();
+    const uint8_t* old_arena = ctx.arena_d.p;
+    ctx.arena_d.ensure(bytes);
+    if (ctx.arena_h_cap < bytes) {
+      if (ctx.arena_h) (void)hipHostFree(ctx.arena_h);
+      ctx.arena_h = nullptr;
+      MTC_HIP(hipHostMalloc((void**)&ctx.arena_h, bytes));
+      ctx.arena_h_cap = bytes;
     }
-    MTC_HIP(hipGraphLaunch(ctx.exec, st));
-    MTC_HIP(hipEventRecord(ctx.ev[4], st));
-  } else {
-    // Eager: the leaf launches are timed on their own (ev[1] .. ev[4]).
+    std::memcpy(ctx.arena_h, img.bytes.data(), bytes);
+    MTC_HIP(hipMemcpyAsync(ctx.arena_d.p, ctx.arena_h, bytes, hipMemcpyHostToDevice, st));
+    uint8_t* base = ctx.arena_d.p;
+    if (graph) {
+      ops.sig.push_back(reinterpret_cast<uint64_t>(base));
+      if (!ctx.exec || ctx.sig != ops.sig || old_arena != base) {
+        ctx.reset_graph();
+        hipGraph_t g = nullptr;
+        MTC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (auto& f : ops.fn) f(base, st);
+        MTC_HIP(hipStreamEndCapture(st, &g));
+        const hipError_t e = hipGraphInstantiate(&ctx.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        MTC_HIP(e);
+        ctx.sig = ops.sig;
+      }
+      MTC_HIP(hipGraphLaunch(ctx.exec, st));
+      MTC_HIP(hipEventRecord(ctx.ev[4], st));
+      return;
+    }
     for (size_t i = 0; i < ops.fn.size(); i++) {
       ops.fn[i](base, st);
       MTC_HIP(hipGetLastError());
       const bool leaf = (ops.sig[2 * i] >> 32) >= 100;
       if (leaf && (i + 1 == ops.fn.size() || (ops.sig[2 * i + 2] >> 32) < 100))
         MTC_HIP(hipEventRecord(ctx.ev[4], st));
-      if (i == 0) MTC_HIP(hipEventRecord(ctx.ev[1], st));
+      if (i == 0 && (ops.sig[0] >> 32) == 1) MTC_HIP(hipEventRecord(ctx.ev[1], st));
     }
-    if (debug)
-      for (auto& M : ms)
-        if (M.h->debug) {  // leaf digests and labels into the debug views
-          MTC_HIP(hipMemcpyAsync(M.h->dbg_tree.p + (M.n - 1), M.h->leaf_dg.p, 16ull * M.n, hipMemcpyDeviceToDevice, st));
-          MTC_HIP(hipMemcpyAsync(M.h->dbg_labels.p + (M.n - 1), M.h->labels.p, M.n, hipMemcpyDeviceToDevice, st));
-        }
+  };
+  if (!split) {
+    Ops ops;
+    ArenaImage img;
+    build_ops(ms, ops, img, 0);
+    add_finish(ops, img);
+    ht.mark("build");
+    run_ops(ops, img, use_graph && !debug);
+  } else {
+    // Phase 1, then the exact table bound of every member, then phase 2.
+    Ops ops1;
+    ArenaImage img1;
+    build_ops(ms, ops1, img1, 1);
+    run_ops(ops1, img1, false);
+    if (ctx.cnt_h_cap < B) {
+      if (ctx.cnt_h) (void)hipHostFree(ctx.cnt_h);
+      ctx.cnt_h = nullptr;
+      MTC_HIP(hipHostMalloc((void**)&ctx.cnt_h, B * sizeof(uint32_t)));
+      ctx.cnt_h_cap = (uint32_t)B;
+    }
+    for (size_t i = 0; i < B; i++)
+      MTC_HIP(hipMemcpyAsync(ctx.cnt_h + i, ms[i].h->blk_off.p + ms[i].nblk_leaf, sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, st));
+    MTC_HIP(hipStreamSynchronize(st));
+    ht.mark("phase 1");
+    for (size_t i = 0; i < B; i++)
+      if (ms[i].split) grow_exact(ms[i].h, ctx.cnt_h[i], st);
+    Ops ops2;
+    ArenaImage img2;
+    build_ops(ms, ops2, img2, 2);
+    add_finish(ops2, img2);
+    run_ops(ops2, img2, false);
   }
+  if (debug)
+    for (auto& M : ms)
+      if (M.h->debug) {  // leaf digests and labels into the debug views
+        MTC_HIP(hipMemcpyAsync(M.h->dbg_tree.p + (M.n - 1), M.h->leaf_dg.p, 16ull * M.n, hipMemcpyDeviceToDevice, st));
+        MTC_HIP(hipMemcpyAsync(M.h->dbg_labels.p + (M.n - 1), M.h->labels.p, M.n, hipMemcpyDeviceToDevice, st));
+      }
   ht.mark("launch");
   if (ctx.ctl_h_cap < B) {
     if (ctx.ctl_h) (void)hipHostFree(ctx.ctl_h);

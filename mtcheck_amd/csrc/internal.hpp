@@ -99,6 +99,8 @@ struct StepCtx {
   DevBuf<Ctl> ctl_all;
   Ctl* ctl_h = nullptr;  // pinned
   uint32_t ctl_h_cap = 0;
+  uint32_t* cnt_h = nullptr;  // pinned: chunks to insert per member (steps that pause after the leaf kernel)
+  uint32_t cnt_h_cap = 0;
   hipEvent_t ev[5] = {};
   hipGraphExec_t exec = nullptr;
   std::vector<uint64_t> sig;

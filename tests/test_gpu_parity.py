@@ -253,3 +253,30 @@ def test_golden_fixtures():
             diff = d.checkpoint(gbuf, make_baseline=(k == 0))
             assert M.parse_header(diff) == rec["header"], (name, k)
             assert hashlib.sha256(diff).hexdigest() == rec["sha256"], (name, k)
+
+
+def test_table_growth_during_long_heavy_series():
+    """12 checkpoints at 30% change: the table outgrows its baseline capacity, so
+    checkpoints pause after the leaf kernel for the exact bound and rehash; every
+    diff stays bit-exact and restarts."""
+    diffs = run_series(1 << 19, 64, 12, fresh_frac=0.2, copy_frac=0.1)
+    assert len(diffs) == 12
+
+
+def test_table_growth_in_batch():
+    """The same growth inside a batched step, next to members that need none."""
+    sizes = [(1 << 19, 64, 0.3), (1 << 20, 128, 0.01), (300_001, 64, 0.25)]
+    series = [CheckpointSeries.with_change(nb, C, ch, seed=70 + i) for i, (nb, C, ch) in enumerate(sizes)]
+    bufs = [torch.empty(nb, dtype=torch.uint8, device="cuda:0") for nb, _, _ in sizes]
+    mates = [M.TreeDeduplicator(C) for _, C, _ in sizes]
+    orc = [O.OracleDedup(C) for _, C, _ in sizes]
+    batch = M.Batch()
+    for k in range(10):
+        for s, b in zip(series, bufs):
+            s.apply_device(b, k)
+        say(f"batched growth step {k}")
+        got = batch.checkpoint(mates, bufs, make_baseline=(k == 0), copy=True)
+        for i, g in enumerate(got):
+            assert g == orc[i].checkpoint(bufs[i].cpu().numpy(), make_baseline=(k == 0)), (k, i)
+    caps = [m.stats().table_capacity for m in mates]
+    assert caps[0] > 4 * (sizes[0][0] // 64)  # grew past the baseline capacity
