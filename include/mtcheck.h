@@ -138,7 +138,7 @@ int mtc_copy_labels(const mtc_dedup* h, void* host_dst, uint64_t capacity);
 
 /* Schedule parameters of a handle (no counterpart in the reference, whose Kokkos
  * policies are fixed): 256-chunk tiles streamed per leaf-kernel block (1..4096),
- * table load factor at a baseline (capacity = chunks / load_factor, in (0.05, 0.7]),
+ * table load factor at a baseline (capacity = chunks / load_factor, in [0.05, 0.7]; default 0.125),
  * and tree levels fused per sweep stage (1..8).  The load factor and fuse depth
  * take effect at the next baseline.  Results do not depend on them. */
 int mtc_set_tuning(mtc_dedup* h, uint32_t leaf_tiles_per_block, float load_factor, uint32_t fuse_levels);

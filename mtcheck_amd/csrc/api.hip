@@ -241,9 +241,9 @@ int mtc_copy_labels(const mtc_dedup* h, void* host_dst, uint64_t capacity) {
 int mtc_set_tuning(mtc_dedup* h, uint32_t leaf_tiles_per_block, float load_factor, uint32_t fuse_levels) {
   if (!h) return MTC_ERR_INVALID_ARG;
   return guarded(h, [&] {
-    if (leaf_tiles_per_block == 0 || leaf_tiles_per_block > 4096 || !(load_factor > 0.05f && load_factor <= 0.7f) ||
+    if (leaf_tiles_per_block == 0 || leaf_tiles_per_block > 4096 || !(load_factor >= 0.05f && load_factor <= 0.7f) ||
         fuse_levels == 0 || fuse_levels > 8)
-      throw Status{MTC_ERR_INVALID_ARG, "tuning out of range (tiles 1..4096, load factor (0.05, 0.7], fuse 1..8)"};
+      throw Status{MTC_ERR_INVALID_ARG, "tuning out of range (tiles 1..4096, load factor [0.05, 0.7], fuse 1..8)"};
     h->tune.leaf_tiles_per_block = leaf_tiles_per_block;
     h->tune.fuse = fuse_levels;
     if (h->tune.load_factor != load_factor || h->has_state) {

@@ -579,10 +579,7 @@ __global__ __launch_bounds__(kThreads) void k_lookup(Launch<DenseArgs> L) {
     const uint32_t s = table_find(a.tab, a.mask, a.leaf_dg[c]);
     if (s == ~0u) continue;
     a.leaf_slot[c] = s;
-    if (a.labels[c] == L_PENDING) {
-      a.labels[c] = L_SHIFT;
-      if (a.R0) a.tile_busy[chunk_tile(a.g, a.R0, c)] = 1;
-    }
+    if (a.labels[c] == L_PENDING) a.labels[c] = L_SHIFT;
   }
 }
 
@@ -638,11 +635,7 @@ __global__ __launch_bounds__(kThreads) void k_resolve(Launch<DenseArgs> L) {
     uint8_t l = L_SHIFT;
     if (s <= a.mask + 1) l = (a.tab[s].val == (((uint64_t)a.cur << 32) | (uint32_t)(a.n - 1 + c))) ? L_FIRST : L_SHIFT;
     a.labels[c] = l;
-    if (a.R0) {
-      const uint64_t t = chunk_tile(a.g, a.R0, c);
-      a.tile_busy[t] = 1;
-      if (l == L_FIRST) a.tile_first[t] = 1;
-    }
+    if (a.R0 && l == L_FIRST) a.tile_first[chunk_tile(a.g, a.R0, c)] = 1;
   }
 }
 
@@ -969,37 +962,27 @@ __global__ __launch_bounds__(kThreads) void k_sweep_b(Launch<SweepArgs> L) {
   if (tile >= (1ull << a.d_out)) return;  // whole wavefront
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
   const uint64_t base = tile << R;
-  if (a.stage0 && !a.tile_busy[tile]) {
-    fixed_tile(a, tile, R, T, base, lane);
-    return;
+  // This lane's four positions: their labels in one 32-bit load when whole and aligned.
+  const uint32_t lp0 = 4 * lane;
+  const uint64_t c0 = a.stage0 ? bottom_chunk(a.g, base + lp0) : base + lp0;  // index into labels / in_lab
+  const uint8_t* lsrc = a.stage0 ? a.labels + c0 : a.in_lab + c0;
+  uint32_t nval = lp0 < T ? min(4u, T - lp0) : 0u;
+  if (a.stage0) nval = (base + lp0 >= a.g.m) ? 0u : (uint32_t)min<uint64_t>(nval, a.g.m - (base + lp0));
+  uint8_t l0[4] = {L_NONE, L_NONE, L_NONE, L_NONE};
+  if (nval == 4 && ((uintptr_t)lsrc & 3) == 0) {
+    const uint32_t w4 = *reinterpret_cast<const uint32_t*>(lsrc);
+#pragma unroll
+    for (int i = 0; i < 4; i++) l0[i] = (uint8_t)(w4 >> (8 * i));
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      if ((uint32_t)i < nval) l0[i] = lsrc[i];
   }
-  uint8_t l0[4];
   Digest g0[4];
   uint32_t s0[4];
   bool busy = false;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const uint32_t lp = 4 * lane + i;
-    l0[i] = L_NONE;
-    if (lp >= T) continue;
-    const uint64_t p = base + lp;
-    if (a.stage0) {
-      if (p >= a.g.m) continue;
-      const uint64_t c = bottom_chunk(a.g, p);
-      l0[i] = a.labels[c];
-      if (l0[i] == L_SHIFT) {
-        g0[i] = a.leaf_dg[c];
-        s0[i] = a.leaf_slot[c];
-      }
-    } else {
-      l0[i] = a.in_lab[p];
-      if (l0[i] == L_SHIFT) {
-        g0[i] = a.in_dg[p];
-        s0[i] = a.in_slot[p];
-      }
-    }
-    busy |= (l0[i] != L_FIXED);
-  }
+  for (int i = 0; i < 4; i++) busy |= ((uint32_t)i < nval && l0[i] != L_FIXED);
   uint8_t lf[2] = {L_NONE, L_NONE};
   Digest gf[2];
   uint32_t sf[2];
@@ -1022,6 +1005,36 @@ __global__ __launch_bounds__(kThreads) void k_sweep_b(Launch<SweepArgs> L) {
     fixed_tile(a, tile, R, T, base, lane);
     return;
   }
+  // Sparse tile: no two siblings carry the same region label, so every region leaf
+  // is a root and every ancestor of a non-fixed leaf is done (the levels below
+  // would derive exactly this); emit the roots without walking the levels.
+  if (a.stage0 && !a.dbg_labels && base + T <= a.g.m) {
+    bool pair = false;
+#pragma unroll
+    for (int k = 0; k < 2; k++) pair |= (l0[2 * k] == l0[2 * k + 1] && is_region(l0[2 * k]));
+    if (!__ballot(pair)) {
+      SweepB<HK> s{a};
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        if (is_region(l0[i])) s.emit(l0[i], l0[i] == L_SHIFT ? a.leaf_slot[c0 + i] : 0u, heap_id(a.d_in, base + lp0 + i));
+      s.flush(a.d_in, tile);
+      if (lane == 0) {
+        a.out_lab[tile] = L_DONE;
+        if (a.d_out == 0) {
+          a.cnt_f[a.seg_base[0]] = 0;
+          a.cnt_s[a.seg_base[0]] = 0;
+        }
+      }
+      return;
+    }
+  }
+  // Digests and table slots of the shifted positions.
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if ((uint32_t)i < nval && l0[i] == L_SHIFT) {
+      g0[i] = a.stage0 ? a.leaf_dg[c0 + i] : a.in_dg[c0 + i];
+      s0[i] = a.stage0 ? a.leaf_slot[c0 + i] : a.in_slot[c0 + i];
+    }
   SweepB<HK> s{a};
   uint8_t l1[2] = {L_NONE, L_NONE};
   Digest g1[2];
@@ -1621,7 +1634,7 @@ static void rehash_to(mtc_dedup* h, uint64_t cap, hipStream_t st) {
 // below it), and grow_exact rehashes only if that bound does not fit.
 static bool prepare_table(mtc_dedup* h, bool reset, hipStream_t st) {
   const uint64_t nodes = 2ull * h->n - 1;
-  const double lf = h->tune.load_factor > 0.05 ? h->tune.load_factor : 0.25;
+  const double lf = h->tune.load_factor > 0.05 ? h->tune.load_factor : 0.125;
   (void)st;
   if (reset) {
     const uint64_t cap = next_pow2(std::max<uint64_t>((uint64_t)(h->n / lf), 1024));

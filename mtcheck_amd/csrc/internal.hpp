@@ -114,7 +114,7 @@ struct StepCtx {
 // Schedule parameters (set by mtc_set_tuning or the autotuner).
 struct Tune {
   uint64_t leaf_tiles_per_block = 1;  // 256-chunk tiles each leaf block streams
-  double load_factor = 0.25;          // table entries per slot at a baseline (capacity = n / load_factor)
+  double load_factor = 0.125;         // table entries per slot at a baseline (capacity = n / load_factor)
   uint32_t fuse = 8;                  // tree levels per sweep stage (1..8)
 };
 

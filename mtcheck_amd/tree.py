@@ -99,7 +99,7 @@ class TreeDeduplicator:
     def num_shift_dupl(self) -> int:  # include/tree_approach.hpp:57-59
         return self.stats().num_shift_dupl
 
-    def set_tuning(self, leaf_tiles_per_block: int = 1, load_factor: float = 0.25, fuse_levels: int = 8) -> None:
+    def set_tuning(self, leaf_tiles_per_block: int = 1, load_factor: float = 0.125, fuse_levels: int = 8) -> None:
         """Schedule parameters; load factor and fuse depth apply from the next baseline."""
         check(lib().mtc_set_tuning(self._h, leaf_tiles_per_block, load_factor, fuse_levels), self._h)
 
