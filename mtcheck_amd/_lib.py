@@ -9,7 +9,8 @@ import ctypes
 import os
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libmtcheck_amd.so")
+# MTC_LIBRARY selects another build of the library (a schedule experiment, say).
+LIB_PATH = os.environ.get("MTC_LIBRARY") or os.path.join(PKG, "libmtcheck_amd.so")
 
 MTC_OK = 0
 MTC_PENDING = 9

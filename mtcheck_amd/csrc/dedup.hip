@@ -264,9 +264,14 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(Launch<ScanSeg> L) {
 // from LDS.  Murmur3 keeps (h1, h2) across segments; MD5 keeps its state and
 // compresses every 64 bytes.
 // ---------------------------------------------------------------------------
+#ifndef MTC_SEG_MAX
+#define MTC_SEG_MAX 128
+#endif
+constexpr int kSegMax = MTC_SEG_MAX;  // most bytes of a chunk staged per pass (LDS per block: 256 * (SEG + 16))
+
 template <int CB>
 struct TileShape {
-  static constexpr int SEG = CB < 128 ? CB : 128;  // bytes of each chunk per pass
+  static constexpr int SEG = CB < kSegMax ? CB : kSegMax;  // bytes of each chunk per pass
   static constexpr int PITCH = SEG + 16;           // LDS bytes per chunk
   static constexpr int LOADS = SEG / 16;           // 16-byte loads per thread per pass
   static constexpr int PASSES = CB / SEG;
