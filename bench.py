@@ -36,6 +36,7 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=4)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--load-factor", type=float, default=0.125, help="digest-table entries per slot at a baseline")
+    p.add_argument("--tiles-per-block", type=int, default=0, help="leaf-kernel tiles per block (0: the library default)")
     p.add_argument("--sweep", action="store_true",
                    help="chunk sizes {64,128,512,4096} x {murmur3,md5} x change {0.1,2,20}%%, one JSON line each")
     p.add_argument("--no-batched", action="store_true", help="skip the batched multi-buffer measurement")
@@ -331,7 +332,7 @@ def served_bench(args):
 
 
 def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, warmup: int,
-              load_factor: float = 0.125) -> dict:
+              load_factor: float = 0.125, tiles_per_block: int = 0) -> dict:
     """Incremental checkpoints of one buffer resident in HBM; every version is
     generated before the timed region (version k = version k-1 + step k)."""
     import numpy as np
@@ -353,7 +354,8 @@ def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, wa
         versions.append(v)
     torch.cuda.synchronize()
     d = M.TreeDeduplicator(C, hash_kind=hk)
-    d.set_tuning(1, load_factor, 8)
+    if tiles_per_block or load_factor != 0.125:
+        d.set_tuning(tiles_per_block or 1, load_factor, 8)
     for k in range(warmup):
         d.checkpoint_device(versions[k], make_baseline=(k == 0))
     torch.cuda.synchronize()
@@ -431,7 +433,7 @@ def main():
         sweep(args)
         return
     r = plain_run(int(args.size_gib * (1 << 30)), args.chunk, args.hash, args.change, args.steps, args.warmup,
-                  args.load_factor)
+                  args.load_factor, args.tiles_per_block)
     out = {
         "metric": METRIC, "value": r["value"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",

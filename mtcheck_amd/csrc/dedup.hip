@@ -527,16 +527,35 @@ struct CompactArgs {
   }
 };
 
-constexpr uint32_t kCompactPer = 8;  // leaf blocks per compaction block
+constexpr uint32_t kCompactPer = 32;  // leaf blocks per compaction block
 
+// The regions of kCompactPer consecutive leaf blocks land contiguously in the dense
+// list (their offsets are a prefix sum), so each thread copies entries by their
+// position in the group and finds the source region by a search in LDS.
 __global__ __launch_bounds__(kThreads) void k_compact(Launch<CompactArgs> L) {
+  __shared__ uint32_t s_pre[kCompactPer + 1];
   const Where w = where(L);
   const CompactArgs a = L.args[w.m].global();
-  for (uint32_t b = w.blk * kCompactPer; b < min(a.nblk, (w.blk + 1) * kCompactPer); b++) {
-    const uint32_t cnt = a.blk_cnt[b];
-    const uint32_t* src = a.list + (uint64_t)b * a.tiles_per_block * kThreads;
-    uint32_t* dst = a.dense + a.blk_off[b];
-    for (uint32_t i = threadIdx.x; i < cnt; i += kThreads) dst[i] = src[i];
+  const uint32_t b0 = w.blk * kCompactPer, nb = min(kCompactPer, a.nblk - b0);
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (uint32_t i = 0; i < nb; i++) {
+      s_pre[i] = run;
+      run += a.blk_cnt[b0 + i];
+    }
+    s_pre[nb] = run;
+  }
+  __syncthreads();
+  const uint32_t total = s_pre[nb];
+  uint32_t* dst = a.dense + a.blk_off[b0];
+  for (uint32_t e = threadIdx.x; e < total; e += kThreads) {
+    uint32_t lo = 0, hi = nb;  // last region whose start <= e
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_pre[mid] <= e) lo = mid;
+      else hi = mid;
+    }
+    dst[e] = a.list[(uint64_t)(b0 + lo) * a.tiles_per_block * kThreads + (e - s_pre[lo])];
   }
 }
 
