@@ -30,6 +30,10 @@ constexpr uint32_t kFuse = 8;                  // most tree levels fused per swe
 constexpr uint64_t kFirstTag = 0xFFFFFFFFull;  // staging tag of a first-occurrence root
 constexpr uint32_t kScanTile = 8 * kThreads;   // elements per block-tile of the batched scan
 constexpr uint32_t kSortBlocks = 1024;         // most blocks per member of the stable sort
+#ifndef MTC_TPW_A
+#define MTC_TPW_A 8
+#endif
+constexpr uint32_t kTilesPerWaveA = MTC_TPW_A;  // stage-0 tiles per wavefront, sweep A (<= 64)
 
 // ---------------------------------------------------------------------------
 // Batched launches
@@ -707,6 +711,7 @@ struct SweepArgs {
   uint8_t* dbg_labels;
   const uint8_t* tile_first;  // stage 0: tile holds a first-occurrence leaf
   const uint8_t* tile_busy;   // stage 0: tile holds a leaf that is not a fixed duplicate
+  uint32_t tiles_per_wave;     // sweep A stage 0: consecutive tiles one wavefront walks (<= 64)
   __device__ SweepArgs global() const {
     SweepArgs c = *this;
     MTC_GLOBAL(leaf_dg); MTC_GLOBAL(labels); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(in_lab); MTC_GLOBAL(in_dg); MTC_GLOBAL(in_slot); MTC_GLOBAL(out_lab); MTC_GLOBAL(out_dg); MTC_GLOBAL(out_slot); MTC_GLOBAL(tab); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(staging); MTC_GLOBAL(cnt_f); MTC_GLOBAL(cnt_s); MTC_GLOBAL(seg_base); MTC_GLOBAL(dbg_tree); MTC_GLOBAL(dbg_labels); MTC_GLOBAL(tile_first); MTC_GLOBAL(tile_busy);
@@ -789,19 +794,12 @@ struct SweepA {
   }
 };
 
+// One tile of sweep A (the whole wavefront).
 template <int HK>
-__global__ __launch_bounds__(kThreads) void k_sweep_a(Launch<SweepArgs> L) {
-  const Where wh = where(L);
-  const SweepArgs a = L.args[wh.m].global();
-  const uint32_t lane = lane_id();
-  const uint64_t tile = (uint64_t)wh.blk * (kThreads / 64) + (threadIdx.x >> 6);
-  if (tile >= (1ull << a.d_out)) return;  // whole wavefront
+__device__ __forceinline__ void sweep_a_tile(const SweepArgs& a, uint64_t tile, uint32_t lane, Digest* rec_dg,
+                                             uint32_t* rec_node) {
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
   const uint64_t base = tile << R;
-  if (a.stage0 && !a.tile_first[tile]) {  // no first-occurrence leaf below this tile
-    if (lane == 0) a.out_lab[tile] = L_NONE;
-    return;
-  }
   uint8_t l0[4];
   Digest g0[4];
   bool any = false;
@@ -841,10 +839,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_a(Launch<SweepArgs> L) {
     if (lane == 0) a.out_lab[tile] = L_NONE;
     return;
   }
-  __shared__ Digest rec_dg[kThreads / 64][256];
-  __shared__ uint32_t rec_node[kThreads / 64][256];
-  const uint32_t w = threadIdx.x >> 6;
-  SweepA<HK> s{a, rec_dg[w], rec_node[w]};
+  SweepA<HK> s{a, rec_dg, rec_node};
   // level 1 (two parents per lane)
   uint8_t l1[2] = {L_NONE, L_NONE};
   Digest g1[2];
@@ -892,12 +887,41 @@ __global__ __launch_bounds__(kThreads) void k_sweep_a(Launch<SweepArgs> L) {
   uint32_t claims = 0;
   for (uint32_t i = lane; i < s.count; i += 64) {
     bool cl = false;
-    table_insert(a.tab, a.mask, rec_dg[w][i], ((uint64_t)a.cur << 32) | rec_node[w][i], false, &a.ctl->err, &cl,
+    table_insert(a.tab, a.mask, rec_dg[i], ((uint64_t)a.cur << 32) | rec_node[i], false, &a.ctl->err, &cl,
                  true);
     claims += cl;
   }
   const uint32_t wc = wave_sum(claims);
   if (lane == 0 && wc) atomicAdd(stripe(a.ctr, CTR_CLAIMS), (unsigned long long)wc);
+}
+
+// Stage 0: a wavefront owns tiles_per_wave consecutive tiles, reads their
+// first-occurrence flags with one load per lane and walks only the flagged ones.
+template <int HK>
+__global__ __launch_bounds__(kThreads) void k_sweep_a(Launch<SweepArgs> L) {
+  __shared__ Digest rec_dg[kThreads / 64][256];
+  __shared__ uint32_t rec_node[kThreads / 64][256];
+  const Where wh = where(L);
+  const SweepArgs a = L.args[wh.m].global();
+  const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  const uint64_t ntiles = 1ull << a.d_out;
+  const uint32_t tpw = a.stage0 ? a.tiles_per_wave : 1u;
+  const uint64_t t0 = ((uint64_t)wh.blk * (kThreads / 64) + w) * tpw;
+  if (t0 >= ntiles) return;  // whole wavefront
+  const uint64_t t1 = min(t0 + tpw, ntiles);
+  if (!a.stage0) {
+    for (uint64_t t = t0; t < t1; t++) sweep_a_tile<HK>(a, t, lane, rec_dg[w], rec_node[w]);
+    return;
+  }
+  const uint64_t mine = t0 + lane;
+  const bool has = mine < t1 && a.tile_first[mine];
+  if (mine < t1 && !has) a.out_lab[mine] = L_NONE;  // no first-occurrence leaf below this tile
+  uint64_t todo = __ballot(has);
+  while (todo) {
+    const int i = __ffsll((long long)todo) - 1;
+    todo &= todo - 1;
+    sweep_a_tile<HK>(a, t0 + i, lane, rec_dg[w], rec_node[w]);
+  }
 }
 
 // Sweep B: shifted-duplicate forest and region roots (include/reference_impl.hpp
@@ -977,31 +1001,45 @@ __device__ void fixed_tile(const SweepArgs& a, uint64_t tile, uint32_t R, uint32
       }
 }
 
-template <int HK>
-__global__ __launch_bounds__(kThreads) void k_sweep_b(Launch<SweepArgs> L) {
-  const Where wh = where(L);
-  const SweepArgs a = L.args[wh.m].global();
-  const uint32_t lane = lane_id();
-  const uint64_t tile = (uint64_t)wh.blk * (kThreads / 64) + (threadIdx.x >> 6);
-  if (tile >= (1ull << a.d_out)) return;  // whole wavefront
-  const uint32_t R = a.d_in - a.d_out, T = 1u << R;
-  const uint64_t base = tile << R;
-  // This lane's four positions: their labels in one 32-bit load when whole and aligned.
+// Labels of a tile's positions, four per lane: one 32-bit load when whole and aligned.
+struct Lab4 {
+  uint8_t l[4];
+  uint32_t nval;  // valid positions of this lane
+  uint64_t c0;    // index of the first one in labels (stage 0) or in_lab
+};
+
+__device__ __forceinline__ Lab4 load_lab4(const SweepArgs& a, uint64_t base, uint32_t T, uint32_t lane) {
+  Lab4 r;
   const uint32_t lp0 = 4 * lane;
-  const uint64_t c0 = a.stage0 ? bottom_chunk(a.g, base + lp0) : base + lp0;  // index into labels / in_lab
-  const uint8_t* lsrc = a.stage0 ? a.labels + c0 : a.in_lab + c0;
-  uint32_t nval = lp0 < T ? min(4u, T - lp0) : 0u;
-  if (a.stage0) nval = (base + lp0 >= a.g.m) ? 0u : (uint32_t)min<uint64_t>(nval, a.g.m - (base + lp0));
-  uint8_t l0[4] = {L_NONE, L_NONE, L_NONE, L_NONE};
-  if (nval == 4 && ((uintptr_t)lsrc & 3) == 0) {
+  r.c0 = a.stage0 ? bottom_chunk(a.g, base + lp0) : base + lp0;
+  const uint8_t* lsrc = a.stage0 ? a.labels + r.c0 : a.in_lab + r.c0;
+  r.nval = lp0 < T ? min(4u, T - lp0) : 0u;
+  if (a.stage0) r.nval = (base + lp0 >= a.g.m) ? 0u : (uint32_t)min<uint64_t>(r.nval, a.g.m - (base + lp0));
+#pragma unroll
+  for (int i = 0; i < 4; i++) r.l[i] = L_NONE;
+  if (r.nval == 4 && ((uintptr_t)lsrc & 3) == 0) {
     const uint32_t w4 = *reinterpret_cast<const uint32_t*>(lsrc);
 #pragma unroll
-    for (int i = 0; i < 4; i++) l0[i] = (uint8_t)(w4 >> (8 * i));
+    for (int i = 0; i < 4; i++) r.l[i] = (uint8_t)(w4 >> (8 * i));
   } else {
 #pragma unroll
     for (int i = 0; i < 4; i++)
-      if ((uint32_t)i < nval) l0[i] = lsrc[i];
+      if ((uint32_t)i < r.nval) r.l[i] = lsrc[i];
   }
+  return r;
+}
+
+// One tile of sweep B (the whole wavefront), its labels already loaded.
+template <int HK>
+__device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, const Lab4& lb, uint32_t lane) {
+  const uint32_t R = a.d_in - a.d_out, T = 1u << R;
+  const uint64_t base = tile << R;
+  const uint32_t lp0 = 4 * lane;
+  const uint64_t c0 = lb.c0;
+  const uint32_t nval = lb.nval;
+  uint8_t l0[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) l0[i] = lb.l[i];
   Digest g0[4];
   uint32_t s0[4];
   bool busy = false;
@@ -1119,6 +1157,18 @@ __global__ __launch_bounds__(kThreads) void k_sweep_b(Launch<SweepArgs> L) {
       a.cnt_s[a.seg_base[0]] = (root && lc == L_SHIFT) ? 1u : 0u;
     }
   }
+}
+
+// One wavefront per tile.
+template <int HK>
+__global__ __launch_bounds__(kThreads) void k_sweep_b(Launch<SweepArgs> L) {
+  const Where wh = where(L);
+  const SweepArgs a = L.args[wh.m].global();
+  const uint32_t lane = lane_id();
+  const uint64_t tile = (uint64_t)wh.blk * (kThreads / 64) + (threadIdx.x >> 6);
+  if (tile >= (1ull << a.d_out)) return;  // whole wavefront
+  const uint32_t R = a.d_in - a.d_out, T = 1u << R;
+  sweep_b_tile<HK>(a, tile, load_lab4(a, tile << R, T, lane), lane);
 }
 
 
@@ -1971,7 +2021,11 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
             a.out_lab = h->fr_lab.p + out_off;
             a.out_dg = h->fr_dg.p + out_off;
             a.out_slot = h->fr_slot.p + out_off;
-            blocks = (uint32_t)(((1ull << a.d_out) + 3) / 4);  // four wavefront tiles per block
+            // stage 0 of sweep A: several tiles per wavefront (their flags are read at
+            // once); four wavefronts per block
+            a.tiles_per_wave = (a.stage0 && pass == 0) ? kTilesPerWaveA : 1u;
+            const uint64_t waves = ((1ull << a.d_out) + a.tiles_per_wave - 1) / a.tiles_per_wave;
+            blocks = (uint32_t)((waves + 3) / 4);
           }
           slot.first.push_back(a);
           slot.second.push_back(blocks);
