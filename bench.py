@@ -239,11 +239,14 @@ def restart_bench(args):
 
 def served_bench(args):
     """Clients with fixed buffer sizes submit checkpoints with Poisson arrivals at a
-    total offered load; the server batches whatever is queued.  A client's next
-    version is produced as soon as its previous checkpoint completes; latency runs
-    from the scheduled arrival to completion (queueing included)."""
+    total offered load; the server batches whatever is queued.  A generator thread
+    (its own stream) produces a client's next version as soon as its previous
+    checkpoint completes; latency runs from the scheduled arrival to completion
+    (queueing included)."""
     import collections
     import heapq
+    import queue
+    import threading
 
     import numpy as np
     import torch
@@ -256,7 +259,6 @@ def served_bench(args):
     srv = Server(lanes=args.serve_lanes, max_batch=64, memory_budget=200 << 30)
     cids = [srv.add_client(c.chunk_size, c.nbytes) for c in clients]
     bufs = [torch.empty(n, dtype=torch.uint8, device="cuda:0") for n in sizes]
-    gen = torch.cuda.Stream()
     K = 6  # uploaded plans per client, reused with fresh stream keys
     plans = [[DevicePlan(c, c.plan(j + 1), "cuda:0") for j in range(K)] for c in clients]
     step = [0] * len(clients)
@@ -266,13 +268,26 @@ def served_bench(args):
     for t in [srv.submit(ci, b, make_baseline=True) for ci, b in zip(cids, bufs)]:
         srv.result(t)
 
-    def advance(i):
-        step[i] += 1
-        plans[i][(step[i] - 1) % K].apply(bufs[i], stream_key(clients[i].seed, step[i]), False, gen)
+    ready = [threading.Event() for _ in clients]
+    todo = queue.Queue()
 
+    def generator():
+        gen = torch.cuda.Stream()
+        while True:
+            i = todo.get()
+            if i is None:
+                return
+            step[i] += 1
+            plans[i][(step[i] - 1) % K].apply(bufs[i], stream_key(clients[i].seed, step[i]), False, gen)
+            gen.synchronize()
+            ready[i].set()
+
+    worker = threading.Thread(target=generator, daemon=True)
+    worker.start()
     for i in range(len(clients)):
-        advance(i)
-    gen.synchronize()
+        todo.put(i)
+    for e in ready:
+        e.wait()
     levels = []
     rng = np.random.default_rng(4)
     for load in [float(x) for x in args.serve_loads.split(",")]:
@@ -306,20 +321,23 @@ def served_bench(args):
                     finished.append(i)
             for i in finished:
                 del outstanding[i]
-                advance(i)
+                ready[i].clear()
+                todo.put(i)  # the client produces its next version
                 busy = True
-            if finished:
-                gen.synchronize()
             for i, q in backlog.items():
-                if q and i not in outstanding:
+                if q and i not in outstanding and ready[i].is_set():
                     outstanding[i] = (srv.submit(cids[i], bufs[i]), q.popleft())
                     busy = True
             if not busy:
-                time.sleep(0.0002)
+                time.sleep(0.0001)
         lat_ms = np.array(lat) * 1e3
         levels.append({"offered_gbs": load, "achieved_gbs": round(done_bytes / (last_done - start) / 1e9, 2),
                        "requests": n_events, "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
                        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3)})
+        for e in ready:
+            e.wait()
+    todo.put(None)
+    worker.join()
     st = srv.stats()
     srv.close()
     del bufs, plans
