@@ -303,6 +303,8 @@ def served_bench(args):
         backlog = collections.defaultdict(collections.deque)
         outstanding = {}
         lat, done_bytes = [], 0
+        parts = []  # (submit - arrival, start - submit, done - start) per request
+        cap0 = [srv.client(c).stats().table_capacity for c in cids]
         start = time.monotonic() + 0.01
         last_done = start
         while events or outstanding or any(backlog.values()):
@@ -316,6 +318,7 @@ def served_bench(args):
                 r = srv.result(tk, wait=False)
                 if r is not None:
                     lat.append(r[4] - ta)
+                    parts.append((r[2] - ta, r[3] - r[2], r[4] - r[3]))
                     done_bytes += sizes[i]
                     last_done = max(last_done, r[4])
                     finished.append(i)
@@ -331,9 +334,14 @@ def served_bench(args):
             if not busy:
                 time.sleep(0.0001)
         lat_ms = np.array(lat) * 1e3
+        pm = np.array(parts) * 1e3
+        grown = sum(srv.client(c).stats().table_capacity != c0 for c, c0 in zip(cids, cap0))
         levels.append({"offered_gbs": load, "achieved_gbs": round(done_bytes / (last_done - start) / 1e9, 2),
                        "requests": n_events, "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
-                       "p99_ms": round(float(np.percentile(lat_ms, 99)), 3)})
+                       "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+                       "p99_parts_ms": {k: round(float(np.percentile(pm[:, j], 99)), 3)
+                                        for j, k in enumerate(("to_submit", "queued", "service"))},
+                       "max_service_ms": round(float(pm[:, 2].max()), 3), "tables_grown": int(grown)})
         for e in ready:
             e.wait()
     todo.put(None)

@@ -30,6 +30,12 @@ class ClientHandle:
         _lib.check(lib().mtc_copy_diff(self._h, buf, size.value, None), self._h)
         return buf.raw
 
+    def stats(self):
+        """The client's deduplicator statistics (read between its checkpoints)."""
+        s = _lib.Stats()
+        _lib.check(lib().mtc_get_stats(self._h, ctypes.byref(s)), self._h)
+        return s
+
 
 class Server:
     def __init__(self, device: int = 0, memory_budget: int = 200 << 30, lanes: int = 2, max_batch: int = 64):
