@@ -1542,6 +1542,82 @@ __global__ __launch_bounds__(kThreads) void k_rehash(const Slot* old, uint64_t o
   }
 }
 
+// Windowed rehash into a table 2^g times larger (g = 1, 2).  An entry's new home
+// keeps its old home as low bits (both are the same hash masked), so the entries
+// whose old home lies in window [b*W, (b+1)*W) are exactly the ones whose new home
+// lies in one of the 2^g windows [j*old_cap + b*W, ...+W).  Block b collects them
+// (from its own old slots and from the cluster running past its window end),
+// places them by linear probing inside its new windows in LDS, and writes those
+// windows out whole, empty slots included, so no fill of the new table is needed.
+// An entry that probes past its window's end is left for k_rehash_left, which
+// inserts it with the ordinary atomic probe once every window is written.  Any
+// insertion order gives a valid linear-probing table, and entries are distinct,
+// so lookups find exactly what the old table held.
+constexpr uint32_t kRehashW = 1024;
+constexpr uint32_t kRehashMaxParts = 4;
+constexpr uint32_t kOwnEmpty = ~0u;
+
+__global__ __launch_bounds__(kThreads) void k_rehash_windows(const Slot* old, uint64_t old_cap, Slot* tab,
+                                                             uint64_t new_mask, uint32_t cap_log, uint32_t* left,
+                                                             uint32_t probe_limit) {
+  __shared__ uint32_t own[kRehashMaxParts * kRehashW];
+  const uint32_t parts = (uint32_t)((new_mask + 1) >> cap_log);
+  const uint64_t old_mask = old_cap - 1;
+  const uint64_t base = (uint64_t)blockIdx.x * kRehashW;
+  for (uint32_t i = threadIdx.x; i < parts * kRehashW; i += kThreads) own[i] = kOwnEmpty;
+  if (blockIdx.x == 0 && threadIdx.x == 0) tab[new_mask + 1] = old[old_cap];  // the overflow slot
+  __syncthreads();
+  auto place = [&](uint64_t p, uint64_t k0, uint64_t k1) {
+    const uint64_t h = slot_home(Digest{k0, k1}, new_mask);
+    const uint64_t off0 = (h & old_mask) - base;  // wraps above W when the home is elsewhere
+    if (off0 >= kRehashW) return;
+    uint32_t* w = own + (uint32_t)(h >> cap_log) * kRehashW;
+    const uint32_t end = (uint32_t)std::min<uint64_t>(kRehashW, off0 + probe_limit);
+    for (uint32_t off = (uint32_t)off0; off < end; off++)
+      if (atomicCAS(&w[off], kOwnEmpty, (uint32_t)p) == kOwnEmpty) return;
+    left[1 + atomicAdd(left, 1u)] = (uint32_t)p;  // left[0] counts; at most one entry per old slot
+  };
+  for (uint32_t i = threadIdx.x; i < kRehashW; i += kThreads) {
+    const ulonglong2 k = *reinterpret_cast<const ulonglong2*>(&old[base + i].k0);
+    if (k.x != TEMPTY) place(base + i, k.x, k.y);
+  }
+  // The cluster that runs past the window end, up to its first empty slot.
+  if (threadIdx.x < 64) {
+    const uint32_t lane = threadIdx.x;
+    for (uint64_t q = base + kRehashW, seen = 0; seen + kRehashW < old_cap; q += 64, seen += 64) {  // never its own window
+      const uint64_t p = q & old_mask;  // old_cap is a multiple of 64
+      const ulonglong2 k = *reinterpret_cast<const ulonglong2*>(&old[p + lane].k0);
+      const uint64_t empty = __ballot(k.x == TEMPTY);
+      const uint32_t first = empty ? (uint32_t)__ffsll((long long)empty) - 1 : 64u;
+      if (lane < first) place(p + lane, k.x, k.y);
+      if (empty) break;
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < parts * kRehashW; i += kThreads) {
+    const uint32_t j = i / kRehashW, r = i % kRehashW;
+    Slot* dst = tab + ((uint64_t)j << cap_log) + base + r;
+    const uint32_t o = own[i];
+    ulonglong2 a{TEMPTY, TEMPTY}, b{TEMPTY, TEMPTY};
+    if (o != kOwnEmpty) {
+      a = *reinterpret_cast<const ulonglong2*>(&old[o].k0);
+      b = *reinterpret_cast<const ulonglong2*>(&old[o].val);
+    }
+    *reinterpret_cast<ulonglong2*>(&dst->k0) = a;
+    *reinterpret_cast<ulonglong2*>(&dst->val) = b;
+  }
+}
+
+// The entries that k_rehash_windows could not place inside their windows.
+__global__ __launch_bounds__(kThreads) void k_rehash_left(const Slot* old, const uint32_t* left, uint32_t count,
+                                                          Slot* tab, uint64_t mask, Ctl* ctl) {
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < count; i += gridDim.x * kThreads) {
+    const Slot s = old[left[1 + i]];
+    bool cl;
+    table_insert(tab, mask, Digest{s.k0, s.k1}, s.val, true, &ctl->err, &cl);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -1684,15 +1760,44 @@ static void ensure_buffers(mtc_dedup* h, hipStream_t st) {
   (void)g;
 }
 
+// MTC_REHASH=atomic sends every entry through the atomic probe, and
+// MTC_REHASH_PROBE_LIMIT=k ends the in-window probe after k slots; both exist to
+// test the less used paths (results never depend on them).
 static void rehash_to(mtc_dedup* h, uint64_t cap, hipStream_t st) {
   if (cap > (1ull << 31)) throw Status{MTC_ERR_TABLE_FULL, "table capacity above 2^31 slots"};
+  const uint64_t old_cap = h->table_cap;
   h->table_tmp.ensure(cap + 1);
-  k_fill_u64<<<grid_for(4 * (cap + 1), kThreads, 65536), kThreads, 0, st>>>(reinterpret_cast<uint64_t*>(h->table_tmp.p),
-                                                                            4 * (cap + 1), TEMPTY);
-  MTC_HIP(hipGetLastError());
-  k_rehash<<<grid_for(h->table_cap + 1, kThreads, 65536), kThreads, 0, st>>>(h->table.p, h->table_cap + 1,
-                                                                             h->table_tmp.p, cap - 1, h->ctl_d);
-  MTC_HIP(hipGetLastError());
+  uint32_t cap_log = 0;
+  while ((1ull << cap_log) < old_cap) cap_log++;
+  const char* mode = std::getenv("MTC_REHASH");
+  const char* lim = std::getenv("MTC_REHASH_PROBE_LIMIT");
+  const uint32_t probe_limit = lim ? (uint32_t)std::max(1L, std::atol(lim)) : kRehashW;
+  if (old_cap % kRehashW == 0 && cap / old_cap <= kRehashMaxParts && !(mode && std::strcmp(mode, "atomic") == 0)) {
+    // Windowed rehash; the entries that overflow a window go through k_rehash_left.
+    DevBuf<uint32_t> left_buf;
+    left_buf.ensure(old_cap + 1);
+    k_fill_u64<<<1, 64, 0, st>>>(reinterpret_cast<uint64_t*>(left_buf.p), 1, 0);  // the count (and left[1])
+    MTC_HIP(hipGetLastError());
+    k_rehash_windows<<<(uint32_t)(old_cap / kRehashW), kThreads, 0, st>>>(h->table.p, old_cap, h->table_tmp.p, cap - 1,
+                                                                         cap_log, left_buf.p, probe_limit);
+    MTC_HIP(hipGetLastError());
+    uint32_t left = 0;
+    MTC_HIP(hipMemcpyAsync(&left, left_buf.p, sizeof(left), hipMemcpyDeviceToHost, st));
+    MTC_HIP(hipStreamSynchronize(st));
+    if (left) {
+      k_rehash_left<<<grid_for(left, kThreads, 65536), kThreads, 0, st>>>(h->table.p, left_buf.p, left, h->table_tmp.p,
+                                                                          cap - 1, h->ctl_d);
+      MTC_HIP(hipGetLastError());
+      MTC_HIP(hipStreamSynchronize(st));  // before left_buf is freed
+    }
+  } else {  // every entry through the atomic probe
+    k_fill_u64<<<grid_for(4 * (cap + 1), kThreads, 65536), kThreads, 0, st>>>(
+        reinterpret_cast<uint64_t*>(h->table_tmp.p), 4 * (cap + 1), TEMPTY);
+    MTC_HIP(hipGetLastError());
+    k_rehash<<<grid_for(old_cap + 1, kThreads, 65536), kThreads, 0, st>>>(h->table.p, old_cap + 1, h->table_tmp.p,
+                                                                          cap - 1, h->ctl_d);
+    MTC_HIP(hipGetLastError());
+  }
   std::swap(h->table, h->table_tmp);
   h->table_tmp.release();
   h->table_cap = cap;

@@ -320,10 +320,28 @@ enum : uint32_t { ERR_TABLE_FULL = 1u, ERR_TABLE_SPIN = 2u, ERR_CORRUPT = 4u, ER
 
 constexpr uint32_t kMaxSpin = 1u << 20;
 
+// Probing runs in line order: slots are 32 bytes and the table is 128-byte aligned,
+// so slots 4k .. 4k+3 share one cache line.  Both routines read every slot from the
+// probe position to the end of its line at once (one memory round trip per line
+// instead of one per slot); the probe sequence itself is plain linear probing.
+constexpr uint32_t kLineSlots = 4;
+
+// The value of a matched entry only ever decreases (use_min) or is set once (insert-
+// if-absent).  Skipping the atomic when it cannot change keeps lanes that repeat a
+// popular digest (zero chunks) from serialising on one address.
+__device__ __forceinline__ void table_update_val(Slot* s, uint64_t v, bool use_min) {
+  const uint64_t now = ld_relaxed(&s->val);
+  if (use_min) {
+    if (now > v) min_relaxed(&s->val, v);
+  } else if (now == TEMPTY) {
+    cas_relaxed(&s->val, TEMPTY, v);
+  }
+}
+
 // Insert d with value v.  use_min: val = min(val, v) (leaf inserts); otherwise
 // val is set only if still empty (insert-if-absent, first-occurrence nodes).
-// cas_first: the digest is expected to be new, so each probe starts with the claim
-// instead of a read.  Returns the slot index, or ~0u on error (bit set in *err).
+// cas_first: the digest is expected to be new, so the home slot is claimed without
+// reading it first.  Returns the slot index, or ~0u on error (bit set in *err).
 // *claimed = 1 when this call created the entry.
 __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& d, uint64_t v, bool use_min,
                                         uint32_t* err, bool* claimed, bool cas_first = false) {
@@ -340,34 +358,53 @@ __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& 
     else cas_relaxed(&s->val, TEMPTY, v);
     return (uint32_t)(mask + 1);
   }
+  // One slot per iteration.  A claim's stores sit in a plain if-block that every
+  // lane leaves before any lane can exit or loop again, and a lane that waits for
+  // k1 does so by repeating the loop: a claimer in the same wavefront has then
+  // always published k1 (an inner wait loop could run while the claimer is masked
+  // off before its store).
   uint64_t i = slot_home(d, mask);
-  uint32_t probes = 0, spins = 0;
+  uint64_t probes = 0;
+  uint32_t spins = 0;
+  uint64_t kl0 = 0, kl1 = 0, kl2 = 0, kl3 = 0;  // k0 of the current line's slots
+  bool loaded = false;
+  bool assume_empty = cas_first;
   while (true) {
     Slot* s = tab + i;
-    uint64_t k0 = cas_first ? TEMPTY : ld_relaxed(&s->k0);
+    uint64_t k0 = TEMPTY;
+    if (!assume_empty) {
+      if (!loaded) {
+        const Slot* line = tab + (i & ~(uint64_t)(kLineSlots - 1));
+        kl0 = ld_relaxed(&line[0].k0);
+        kl1 = ld_relaxed(&line[1].k0);
+        kl2 = ld_relaxed(&line[2].k0);
+        kl3 = ld_relaxed(&line[3].k0);
+        loaded = true;
+      }
+      const uint32_t q = (uint32_t)(i & (kLineSlots - 1));
+      k0 = q == 0 ? kl0 : q == 1 ? kl1 : q == 2 ? kl2 : kl3;
+    }
+    assume_empty = false;
+    bool mine = false;
     if (k0 == TEMPTY) {
-      uint64_t old = cas_relaxed(&s->k0, TEMPTY, d.lo);
+      const uint64_t old = cas_relaxed(&s->k0, TEMPTY, d.lo);
       if (old == TEMPTY) {
         st_relaxed(&s->k1, d.hi);
         if (use_min) min_relaxed(&s->val, v);
         else cas_relaxed(&s->val, TEMPTY, v);
-        *claimed = true;
-        return (uint32_t)i;
+        mine = true;
+      } else {
+        k0 = old;
       }
-      k0 = old;
+    }
+    if (mine) {
+      *claimed = true;
+      return (uint32_t)i;
     }
     if (k0 == d.lo) {
-      uint64_t k1 = ld_relaxed(&s->k1);
+      const uint64_t k1 = ld_relaxed(&s->k1);
       if (k1 == d.hi) {
-        // The value only ever decreases: skip the atomic when it cannot win.  This
-        // keeps lanes that repeat a popular digest (zero chunks) from serialising
-        // on one address.
-        const uint64_t now = ld_relaxed(&s->val);
-        if (use_min) {
-          if (now > v) min_relaxed(&s->val, v);
-        } else if (now == TEMPTY) {
-          cas_relaxed(&s->val, TEMPTY, v);
-        }
+        table_update_val(s, v, use_min);
         return (uint32_t)i;
       }
       if (k1 == TEMPTY) {  // claimer has not published k1 yet: look again
@@ -378,27 +415,37 @@ __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& 
         continue;
       }
     }
-    i = (i + 1) & mask;
     if (++probes > mask) {
       atomicOr(err, ERR_TABLE_FULL);
       return ~0u;
     }
+    i = (i + 1) & mask;
+    if ((i & (kLineSlots - 1)) == 0) loaded = false;
   }
 }
 
-// Read-only lookup, valid once the inserting kernel has completed.
+// Read-only lookup, valid once the inserting kernel has completed.  Reads the
+// (k0, k1) pairs of the probe position's whole line with 16-byte loads.
 __device__ inline uint32_t table_find(const Slot* tab, uint64_t mask, const Digest& d) {
   if (d.lo == TEMPTY) {
     const Slot* s = tab + mask + 1;
     return (s->k1 == d.hi && s->val != TEMPTY) ? (uint32_t)(mask + 1) : ~0u;
   }
   uint64_t i = slot_home(d, mask);
-  for (uint64_t probes = 0; probes <= mask; probes++) {
-    const Slot* s = tab + i;
-    uint64_t k0 = s->k0;
-    if (k0 == TEMPTY) return ~0u;
-    if (k0 == d.lo && s->k1 == d.hi) return (uint32_t)i;
-    i = (i + 1) & mask;
+  for (uint64_t probes = 0; probes <= mask;) {
+    const Slot* line = tab + (i & ~(uint64_t)(kLineSlots - 1));
+    ulonglong2 k[kLineSlots];
+#pragma unroll
+    for (uint32_t q = 0; q < kLineSlots; q++) k[q] = *reinterpret_cast<const ulonglong2*>(&line[q].k0);
+    const uint32_t j0 = (uint32_t)(i & (kLineSlots - 1));
+#pragma unroll
+    for (uint32_t q = 0; q < kLineSlots; q++) {
+      if (q < j0) continue;
+      if (k[q].x == TEMPTY) return ~0u;
+      if (k[q].x == d.lo && k[q].y == d.hi) return (uint32_t)(line + q - tab);
+      probes++;
+    }
+    i = ((i | (kLineSlots - 1)) + 1) & mask;
   }
   return ~0u;
 }

@@ -75,9 +75,11 @@ def test_reference_case(case):
     assert sorted(p["shift"][:, 0].tolist()) == shift
 
 
-def run_series(nbytes, C, steps, hk=M.MURMUR3, check_tree=True, seed=3, tuning=None, **kw):
+def run_series(nbytes, C, steps, hk=M.MURMUR3, check_tree=True, seed=3, tuning=None, handles=None, **kw):
     series = CheckpointSeries(nbytes, C, seed=seed, **kw)
     d = M.TreeDeduplicator(C, hash_kind=hk)
+    if handles is not None:
+        handles.append(d)
     if tuning:
         d.set_tuning(*tuning)
     d.set_debug(check_tree)
@@ -257,17 +259,25 @@ def test_golden_fixtures():
             assert hashlib.sha256(diff).hexdigest() == rec["sha256"], (name, k)
 
 
-def test_table_growth_during_long_heavy_series():
+@pytest.mark.parametrize("path,env", [("windowed", {}), ("leftovers", {"MTC_REHASH_PROBE_LIMIT": "1"}),
+                                      ("atomic", {"MTC_REHASH": "atomic"})])
+def test_table_growth_during_long_heavy_series(monkeypatch, path, env):
     """12 checkpoints at 30% change: the table outgrows its baseline capacity, so
     checkpoints pause after the leaf kernel for the exact bound and rehash; every
-    diff stays bit-exact and restarts."""
-    diffs = run_series(1 << 19, 64, 12, fresh_frac=0.2, copy_frac=0.1, tuning=(1, 0.5, 8))
+    diff stays bit-exact and restarts.  Each rehash path is run: windowed (the
+    default), windowed with most entries left for the atomic pass, and atomic only."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    hs = []
+    diffs = run_series(1 << 19, 64, 12, fresh_frac=0.2, copy_frac=0.1, tuning=(1, 0.5, 8), handles=hs)
     assert len(diffs) == 12
+    assert hs[0].stats().table_capacity >= 4 * ((1 << 19) // 64)  # grew past the baseline capacity
 
 
 def test_table_growth_in_batch():
     """The same growth inside a batched step, next to members that need none."""
-    sizes = [(1 << 19, 64, 0.3), (1 << 20, 128, 0.01), (300_001, 64, 0.25)]
+    # the last member's table starts at the minimum 1024 slots: one rehash window
+    sizes = [(1 << 19, 64, 0.3), (1 << 20, 128, 0.01), (300_001, 64, 0.25), (30_000, 64, 0.3)]
     series = [CheckpointSeries.with_change(nb, C, ch, seed=70 + i) for i, (nb, C, ch) in enumerate(sizes)]
     bufs = [torch.empty(nb, dtype=torch.uint8, device="cuda:0") for nb, _, _ in sizes]
     mates = [M.TreeDeduplicator(C) for _, C, _ in sizes]
@@ -284,3 +294,4 @@ def test_table_growth_in_batch():
             assert g == orc[i].checkpoint(bufs[i].cpu().numpy(), make_baseline=(k == 0)), (k, i)
     caps = [m.stats().table_capacity for m in mates]
     assert caps[0] > 2 * (sizes[0][0] // 64)  # grew past the baseline capacity
+    assert caps[3] > 1024
