@@ -320,11 +320,12 @@ enum : uint32_t { ERR_TABLE_FULL = 1u, ERR_TABLE_SPIN = 2u, ERR_CORRUPT = 4u, ER
 
 constexpr uint32_t kMaxSpin = 1u << 20;
 
-// Probing runs in line order: slots are 32 bytes and the table is 128-byte aligned,
-// so slots 4k .. 4k+3 share one cache line.  Both routines read every slot from the
-// probe position to the end of its line at once (one memory round trip per line
-// instead of one per slot); the probe sequence itself is plain linear probing.
-constexpr uint32_t kLineSlots = 4;
+// Probing runs in sector order: slots are 32 bytes and HBM is read in 64-byte
+// sectors, so slots 2k and 2k+1 arrive together.  Both routines read the keys of
+// the probe position's whole sector at once (one access per sector instead of one
+// per slot; reading a whole 128-byte line costs two accesses and mostly goes
+// unused at low load).  The probe sequence itself is plain linear probing.
+constexpr uint32_t kLineSlots = 2;
 
 // The value of a matched entry only ever decreases (use_min) or is set once (insert-
 // if-absent).  Skipping the atomic when it cannot change keeps lanes that repeat a
@@ -366,7 +367,7 @@ __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& 
   uint64_t i = slot_home(d, mask);
   uint64_t probes = 0;
   uint32_t spins = 0;
-  uint64_t kl0 = 0, kl1 = 0, kl2 = 0, kl3 = 0;  // k0 of the current line's slots
+  uint64_t kl0 = 0, kl1 = 0;  // k0 of the current sector's slots
   bool loaded = false;
   bool assume_empty = cas_first;
   while (true) {
@@ -377,12 +378,9 @@ __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& 
         const Slot* line = tab + (i & ~(uint64_t)(kLineSlots - 1));
         kl0 = ld_relaxed(&line[0].k0);
         kl1 = ld_relaxed(&line[1].k0);
-        kl2 = ld_relaxed(&line[2].k0);
-        kl3 = ld_relaxed(&line[3].k0);
         loaded = true;
       }
-      const uint32_t q = (uint32_t)(i & (kLineSlots - 1));
-      k0 = q == 0 ? kl0 : q == 1 ? kl1 : q == 2 ? kl2 : kl3;
+      k0 = (i & 1) ? kl1 : kl0;
     }
     assume_empty = false;
     bool mine = false;
@@ -425,7 +423,7 @@ __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& 
 }
 
 // Read-only lookup, valid once the inserting kernel has completed.  Reads the
-// (k0, k1) pairs of the probe position's whole line with 16-byte loads.
+// (k0, k1) pairs of the probe position's whole sector with 16-byte loads.
 __device__ inline uint32_t table_find(const Slot* tab, uint64_t mask, const Digest& d) {
   if (d.lo == TEMPTY) {
     const Slot* s = tab + mask + 1;
