@@ -34,6 +34,8 @@ def parse():
     p.add_argument("--change", type=float, default=0.02)
     p.add_argument("--cpu-sample-mib", type=int, default=1024)
     p.add_argument("--cpu-steps", type=int, default=4)
+    p.add_argument("--cpu-par-sample-mib", type=int, default=2048)
+    p.add_argument("--cpu-par-steps", type=int, default=6)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--load-factor", type=float, default=0.125, help="digest-table entries per slot at a baseline")
     p.add_argument("--tiles-per-block", type=int, default=0, help="leaf-kernel tiles per block (0: the library default)")
@@ -60,33 +62,43 @@ def parse():
 
 
 def cpu_baseline(args, hk):
-    """The CPU oracle (serial restatement of the reference, one core) on a bounded
-    sample of the same workload: a series of cpu-sample-mib MiB, same generator,
-    chunk size, hash and change rate; baseline untimed, incrementals timed."""
+    """The CPU oracle on bounded samples of the same workload (same generator, chunk
+    size, hash and change rate; baseline untimed, incrementals timed): its OpenMP
+    checkpoint on every host thread this job may use (the value), and its serial
+    checkpoint on one core beside it."""
     import numpy as np
     import torch
 
     from mtcheck_amd.workload import CheckpointSeries
     from oracle import oracle as O
-    nbytes = args.cpu_sample_mib << 20
-    series = CheckpointSeries.with_change(nbytes, args.chunk, args.change)
-    buf = torch.empty(nbytes, dtype=torch.uint8, device="cuda:0")
-    orc = O.OracleDedup(args.chunk, hk)
-    series.apply_device(buf, 0)
-    orc.checkpoint(buf.cpu().numpy(), make_baseline=True)
-    t = 0.0
-    for k in range(1, args.cpu_steps + 1):
-        series.apply_device(buf, k)
-        host = buf.cpu().numpy()
-        t0 = time.perf_counter()
-        orc.checkpoint(host)
-        t += time.perf_counter() - t0
-    del buf
-    torch.cuda.empty_cache()
-    return {"value": round(nbytes * args.cpu_steps / t / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"{args.cpu_sample_mib} MiB series, {args.chunk} B chunks, {args.hash}, "
-                      f"{args.change * 100:g}% change; {args.cpu_steps} incremental checkpoints timed "
-                      f"({t:.1f} s) with the serial oracle"}
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+
+    def timed(mib, steps, nthreads):
+        nbytes = mib << 20
+        series = CheckpointSeries.with_change(nbytes, args.chunk, args.change)
+        buf = torch.empty(nbytes, dtype=torch.uint8, device="cuda:0")
+        orc = O.OracleDedup(args.chunk, hk)
+        series.apply_device(buf, 0)
+        orc.checkpoint(buf.cpu().numpy(), make_baseline=True, threads=nthreads)
+        t = 0.0
+        for k in range(1, steps + 1):
+            series.apply_device(buf, k)
+            host = buf.cpu().numpy()
+            t0 = time.perf_counter()
+            orc.checkpoint(host, threads=nthreads)
+            t += time.perf_counter() - t0
+        del buf, orc
+        torch.cuda.empty_cache()
+        return nbytes * steps / t / 1e9, t
+
+    par, tp = timed(args.cpu_par_sample_mib, args.cpu_par_steps, threads)
+    ser, ts = timed(args.cpu_sample_mib, args.cpu_steps, 0)
+    return {"value": round(par, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"{args.chunk} B chunks, {args.hash}, {args.change * 100:g}% change, same generator: "
+                      f"OpenMP oracle on {threads} threads, {args.cpu_par_sample_mib} MiB series, "
+                      f"{args.cpu_par_steps} incremental checkpoints timed ({tp:.1f} s); serial oracle on one core, "
+                      f"{args.cpu_sample_mib} MiB series, {args.cpu_steps} checkpoints ({ts:.1f} s)",
+            "serial_1core": round(ser, 4)}
 
 
 def copy_calibration(gib: float):

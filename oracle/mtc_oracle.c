@@ -533,6 +533,160 @@ int orc_dedup_checkpoint(orc_dedup* d, const uint8_t* data, uint64_t len, int ma
   return rc;
 }
 
+/* ========================================================================== */
+/* Multi-threaded checkpoint (OpenMP): the CPU baseline on all host cores.     */
+/* Same results as orc_dedup_checkpoint, byte for byte:                        */
+/*  - leaf hashes and the fixed test run in parallel; table inserts of the    */
+/*    leaves stay serial, in ascending leaf order.  A fixed leaf's insert is   */
+/*    skipped after the baseline's own checkpoint: its digest is the previous */
+/*    checkpoint's leaf, which that checkpoint inserted, so the serial insert */
+/*    finds it and changes nothing;                                            */
+/*  - both forests run one depth at a time, deepest first, nodes of a depth   */
+/*    in parallel (a node reads only its children, one depth below).  The    */
+/*    first-occurrence inserts are made serially afterwards in descending node */
+/*    order, the serial loop's order; no forest label depends on them, and    */
+/*    the shifted-duplicate forest (which reads the table) runs after them, as */
+/*    in reference_impl.hpp:74-119.                                            */
+/* ========================================================================== */
+
+/* Internal nodes at heap depth k: [2^k - 1, min(2^(k+1) - 2, n - 2)]. */
+static void depth_range(uint32_t n, uint32_t k, int64_t* lo, int64_t* hi) {
+  *lo = (int64_t)((1ull << k) - 1);
+  int64_t h = (int64_t)((1ull << (k + 1)) - 2);
+  *hi = h < (int64_t)n - 2 ? h : (int64_t)n - 2;
+}
+
+static int label_low_offset_par(orc_dedup* d, const uint8_t* data, uint64_t len, uint8_t* cur_tree, int threads) {
+  const uint32_t n = d->n, num_nodes = d->num_nodes, cur = d->current_id;
+  const uint64_t C = d->chunk_size;
+  const int first_after_reset = (cur == d->baseline_id);
+  uint8_t* labels = d->labels;
+  memset(labels, ORC_DONE, num_nodes);
+#pragma omp parallel for num_threads(threads) schedule(static)
+  for (int64_t leaf = (int64_t)n - 1; leaf < (int64_t)num_nodes; leaf++) {
+    uint64_t off = (uint64_t)(leaf - ((int64_t)n - 1)) * C;
+    uint64_t nb = (leaf == (int64_t)num_nodes - 1) ? len - off : C;
+    orc_hash(d->hash_kind, data + off, nb, DG(cur_tree, leaf));
+    labels[leaf] = memcmp(DG(d->tree, leaf), DG(cur_tree, leaf), 16) == 0 ? ORC_FIXED_DUPL : ORC_DONE;
+  }
+  for (uint64_t leaf = n - 1; leaf < num_nodes; leaf++) { /* :44-71, ascending */
+    if (labels[leaf] == ORC_FIXED_DUPL) {
+      if (first_after_reset) map_insert(&d->map, DG(cur_tree, leaf), (uint32_t)leaf, cur, NULL);
+      continue;
+    }
+    orc_slot* s = NULL;
+    int ok = map_insert(&d->map, DG(cur_tree, leaf), (uint32_t)leaf, cur, &s);
+    if (ok) {
+      labels[leaf] = ORC_FIRST_OCUR;
+    } else if (s->tree == cur && s->node > leaf) { /* unreachable with ascending leaves */
+      labels[leaf] = ORC_FIRST_OCUR;
+      labels[s->node] = ORC_SHIFT_DUPL;
+      s->node = (uint32_t)leaf;
+    } else {
+      labels[leaf] = ORC_SHIFT_DUPL;
+    }
+  }
+  if (n < 2) return ORC_OK;
+  const uint32_t depth_int = ilog2_u64((uint64_t)n - 1); /* depth of node n - 2 */
+  for (int64_t k = depth_int; k >= 0; k--) { /* first-occurrence forest, :74-82 */
+    int64_t lo, hi;
+    depth_range(n, (uint32_t)k, &lo, &hi);
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t node = lo; node <= hi; node++) {
+      uint64_t l = 2 * (uint64_t)node + 1, r = l + 1;
+      if (labels[l] == ORC_FIRST_OCUR && labels[r] == ORC_FIRST_OCUR) {
+        labels[node] = ORC_FIRST_OCUR;
+        orc_hash(d->hash_kind, DG(cur_tree, l), 32, DG(cur_tree, node));
+      }
+    }
+  }
+  for (int64_t node = (int64_t)n - 2; node >= 0; node--)
+    if (labels[node] == ORC_FIRST_OCUR) map_insert(&d->map, DG(cur_tree, node), (uint32_t)node, cur, NULL);
+  return ORC_OK;
+}
+
+static void forest_roots_par(orc_dedup* d, uint8_t* cur_tree, uint8_t* is_root, int threads) {
+  const uint32_t n = d->n;
+  uint8_t* labels = d->labels;
+  if (n >= 2) {
+    const uint32_t depth_int = ilog2_u64((uint64_t)n - 1);
+    for (int64_t k = depth_int; k >= 0; k--) { /* :83-104, a depth at a time */
+      int64_t lo, hi;
+      depth_range(n, (uint32_t)k, &lo, &hi);
+#pragma omp parallel for num_threads(threads) schedule(static)
+      for (int64_t node = lo; node <= hi; node++) {
+        uint64_t l = 2 * (uint64_t)node + 1, r = l + 1;
+        uint8_t a = labels[l], b = labels[r];
+        if (a != b) {
+          labels[node] = ORC_DONE;
+          if (a != ORC_FIXED_DUPL && a != ORC_DONE) is_root[l] = 1;
+          if (b != ORC_FIXED_DUPL && b != ORC_DONE) is_root[r] = 1;
+        } else if (a == ORC_FIXED_DUPL) {
+          labels[node] = ORC_FIXED_DUPL;
+        } else if (a == ORC_SHIFT_DUPL) {
+          orc_hash(d->hash_kind, DG(cur_tree, l), 32, DG(cur_tree, node));
+          if (map_find(&d->map, DG(cur_tree, node))) {
+            labels[node] = ORC_SHIFT_DUPL;
+          } else {
+            labels[node] = ORC_DONE;
+            is_root[l] = 1;
+            is_root[r] = 1;
+          }
+        }
+      }
+    }
+  }
+  if (labels[0] == ORC_FIRST_OCUR) is_root[0] = 1;
+  if (!d->compat && labels[0] == ORC_SHIFT_DUPL) is_root[0] = 1; /* D2 */
+}
+
+int orc_dedup_checkpoint_par(orc_dedup* d, const uint8_t* data, uint64_t len, int make_baseline, int threads) {
+  if (!d || !data || len == 0 || threads < 1) return ORC_ERR_ARG;
+  uint64_t n64 = (len + d->chunk_size - 1) / d->chunk_size;
+  if (n64 > (1ull << 31)) return ORC_ERR_ARG;
+  uint32_t n = (uint32_t)n64, num_nodes = 2 * n - 1;
+  int baseline = make_baseline || d->current_id == 0;
+  if (!baseline && n != d->n) return ORC_ERR_STATE;
+  if (baseline) {
+    free(d->tree);
+    free(d->labels);
+    map_free(&d->map);
+    d->tree = (uint8_t*)calloc(num_nodes, 16);
+    d->labels = (uint8_t*)malloc(num_nodes);
+    if (!d->tree || !d->labels || map_init(&d->map, 2ull * num_nodes) != ORC_OK) return ORC_ERR_NOMEM;
+    d->n = n;
+    d->num_nodes = num_nodes;
+    d->baseline_id = d->current_id;
+  }
+  uint8_t* cur_tree = (uint8_t*)calloc(num_nodes, 16);
+  uint8_t* is_root = (uint8_t*)calloc(num_nodes, 1);
+  if (!cur_tree || !is_root) { free(cur_tree); free(is_root); return ORC_ERR_NOMEM; }
+  label_low_offset_par(d, data, len, cur_tree, threads);
+  forest_roots_par(d, cur_tree, is_root, threads);
+  free(d->tree);
+  d->tree = cur_tree;
+  free(d->first);
+  free(d->shift);
+  uint32_t F = 0, S = 0;
+  for (uint32_t v = 0; v < num_nodes; v++) {
+    if (!is_root[v]) continue;
+    if (d->labels[v] == ORC_FIRST_OCUR) F++;
+    else if (d->labels[v] == ORC_SHIFT_DUPL) S++;
+  }
+  d->first = (uint32_t*)malloc(sizeof(uint32_t) * (F ? F : 1));
+  d->shift = (uint32_t*)malloc(sizeof(uint32_t) * (S ? S : 1));
+  d->nfirst = d->nshift = 0;
+  for (uint32_t v = 0; v < num_nodes; v++) {
+    if (!is_root[v]) continue;
+    if (d->labels[v] == ORC_FIRST_OCUR) d->first[d->nfirst++] = v;
+    else if (d->labels[v] == ORC_SHIFT_DUPL) d->shift[d->nshift++] = v;
+  }
+  free(is_root);
+  int rc = collect_diff(d, data, len);
+  d->current_id += 1;
+  return rc;
+}
+
 uint64_t orc_dedup_diff_size(const orc_dedup* d) { return d->diff_size; }
 void orc_dedup_copy_diff(const orc_dedup* d, uint8_t* dst) { memcpy(dst, d->diff, d->diff_size); }
 uint32_t orc_dedup_num_first_ocur(const orc_dedup* d) { return d->nfirst; }

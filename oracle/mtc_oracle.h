@@ -66,6 +66,8 @@ orc_dedup* orc_dedup_create(uint32_t chunk_size, int hash_kind, int compat_ref);
 void orc_dedup_destroy(orc_dedup* d);
 /* One checkpoint; advances current_id like src/tree_approach.cpp:1740-1748. */
 int orc_dedup_checkpoint(orc_dedup* d, const uint8_t* data, uint64_t len, int make_baseline);
+/* The same checkpoint with OpenMP on `threads` threads (identical results). */
+int orc_dedup_checkpoint_par(orc_dedup* d, const uint8_t* data, uint64_t len, int make_baseline, int threads);
 uint64_t orc_dedup_diff_size(const orc_dedup* d);
 void orc_dedup_copy_diff(const orc_dedup* d, uint8_t* dst);
 uint32_t orc_dedup_num_first_ocur(const orc_dedup* d);

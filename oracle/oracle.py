@@ -30,7 +30,7 @@ def build(force: bool = False) -> str:
     """Compile the oracle with gcc (the recipe build() in __graft_entry__ runs)."""
     if force or not os.path.exists(LIB_PATH):
         tmp = LIB_PATH + ".tmp"
-        subprocess.check_call(["gcc", "-O2", "-std=c11", "-Wall", "-shared", "-fPIC",
+        subprocess.check_call(["gcc", "-O2", "-std=c11", "-Wall", "-fopenmp", "-shared", "-fPIC",
                                "-o", tmp, SRC_PATH])
         os.replace(tmp, LIB_PATH)
     return LIB_PATH
@@ -61,6 +61,8 @@ def lib():
         L.orc_dedup_destroy.argtypes = [vp]
         L.orc_dedup_checkpoint.argtypes = [vp, vp, u64, i32]
         L.orc_dedup_checkpoint.restype = i32
+        L.orc_dedup_checkpoint_par.argtypes = [vp, vp, u64, i32, i32]
+        L.orc_dedup_checkpoint_par.restype = i32
         for name in ("orc_dedup_diff_size", "orc_dedup_table_size"):
             getattr(L, name).argtypes = [vp]
             getattr(L, name).restype = u64
@@ -145,9 +147,13 @@ class OracleDedup:
             lib().orc_dedup_destroy(h)
             self._h = None
 
-    def checkpoint(self, data, make_baseline: bool = False) -> bytes:
+    def checkpoint(self, data, make_baseline: bool = False, threads: int = 0) -> bytes:
+        """threads > 0: the OpenMP checkpoint on that many threads (same bytes)."""
         a = _as_u8(data)
-        rc = lib().orc_dedup_checkpoint(self._h, _ptr(a), a.size, 1 if make_baseline else 0)
+        if threads > 0:
+            rc = lib().orc_dedup_checkpoint_par(self._h, _ptr(a), a.size, 1 if make_baseline else 0, threads)
+        else:
+            rc = lib().orc_dedup_checkpoint(self._h, _ptr(a), a.size, 1 if make_baseline else 0)
         if rc != 0:
             raise RuntimeError(f"oracle checkpoint failed: {rc}")
         out = np.empty(lib().orc_dedup_diff_size(self._h), np.uint8)
