@@ -390,12 +390,18 @@ __device__ __forceinline__ Digest tile_digest(const uint8_t* data, uint64_t len,
 
 // Fixed test, label and packing of one tile's chunks (chunk t*256 + threadIdx.x
 // has digest d).  Chunks to insert go to this block's region, ascending.
-__device__ __forceinline__ void leaf_epilogue(const LeafArgs& a, uint64_t t, const Digest& d, uint32_t* region,
-                                              uint32_t& run, uint32_t* s_wcnt) {
+// The previous checkpoint's digest of this lane's chunk in tile t (zero after a
+// reset).  Callers issue it with the tile's data loads so that its latency hides
+// behind theirs.
+__device__ __forceinline__ Digest leaf_prev(const LeafArgs& a, uint64_t t) {
+  const uint64_t c = t * kThreads + threadIdx.x;
+  return (c < a.n && !a.baseline) ? a.leaf_dg[c] : Digest{0, 0};
+}
+
+__device__ __forceinline__ void leaf_epilogue(const LeafArgs& a, uint64_t t, const Digest& d, const Digest& prev,
+                                              uint32_t* region, uint32_t& run, uint32_t* s_wcnt) {
   const uint64_t c = t * kThreads + threadIdx.x;
   const bool valid = c < a.n;
-  Digest prev{0, 0};
-  if (valid && !a.baseline) prev = a.leaf_dg[c];
   const bool fixed = valid && d == prev;
   // A fixed leaf's digest was inserted by an earlier checkpoint of this baseline,
   // so its insert is a no-op except right after a reset, where every leaf inserts.
@@ -434,8 +440,10 @@ __global__ __launch_bounds__(kThreads) void k_leaf(Launch<LeafArgs> L) {
   uint32_t* region = a.list + t0 * kThreads;
   uint32_t run = 0;
   if constexpr (CB == 0) {
-    for (uint64_t t = t0; t < t1; t++)
-      leaf_epilogue(a, t, tile_digest<HK, 0>(a.data, a.len, a.n, a.C, t), region, run, s_wcnt);
+    for (uint64_t t = t0; t < t1; t++) {
+      const Digest pv = leaf_prev(a, t);
+      leaf_epilogue(a, t, tile_digest<HK, 0>(a.data, a.len, a.n, a.C, t), pv, region, run, s_wcnt);
+    }
   } else {
     using S = TileShape<CB>;
     __shared__ __align__(16) uint8_t lds[S::LDS];
@@ -455,12 +463,20 @@ __global__ __launch_bounds__(kThreads) void k_leaf(Launch<LeafArgs> L) {
       }
     };
     uint4 v[S::LOADS];
-    if (steps) load(0, v);
+    Digest pv{0, 0};  // previous digest of this lane's chunk in the current tile
+    if (steps) {
+      load(0, v);
+      pv = leaf_prev(a, t0);
+    }
     uint64_t h1 = 0, h2 = 0;
     uint32_t st[4], X[16];
     for (uint64_t step = 0; step < steps; step++) {
       uint4 nv[S::LOADS];
-      if (step + 1 < steps) load(step + 1, nv);
+      Digest npv = pv;
+      if (step + 1 < steps) {
+        load(step + 1, nv);
+        if ((step + 1) % S::PASSES == 0) npv = leaf_prev(a, t0 + (step + 1) / S::PASSES);
+      }
       __syncthreads();  // the previous segment's LDS reads are done
 #pragma unroll
       for (int i = 0; i < S::LOADS; i++) {
@@ -507,10 +523,11 @@ __global__ __launch_bounds__(kThreads) void k_leaf(Launch<LeafArgs> L) {
           d = m3_final(h1, h2, CB);
         }
         if (c >= full) d = (c < a.n) ? hash_bytes<HK>(a.data + c * CB, a.len - c * CB) : Digest{0, 0};
-        leaf_epilogue(a, t, d, region, run, s_wcnt);
+        leaf_epilogue(a, t, d, pv, region, run, s_wcnt);
       }
 #pragma unroll
       for (int i = 0; i < S::LOADS; i++) v[i] = nv[i];
+      pv = npv;
     }
   }
   if (threadIdx.x == 0) a.blk_cnt[wh.blk] = run;
