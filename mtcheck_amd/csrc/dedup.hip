@@ -596,6 +596,7 @@ struct DenseArgs {
   uint32_t R0;          // levels of the first sweep stage
   uint8_t* tile_first;  // per stage-0 tile: holds a first-occurrence leaf
   uint8_t* tile_busy;   // per stage-0 tile: holds a leaf that is not a fixed duplicate
+  int baseline_hint;    // the table was reset for this checkpoint: every digest is new
   __device__ DenseArgs global() const {
     DenseArgs c = *this;
     MTC_GLOBAL(dense); MTC_GLOBAL(total); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(labels); MTC_GLOBAL(tab); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(tile_first); MTC_GLOBAL(tile_busy);
@@ -611,27 +612,15 @@ __device__ __forceinline__ uint64_t chunk_tile(const Geo& g, uint32_t R0, uint64
   return (c + g.half) >> (R0 - 1);
 }
 
-// Read-only pass before any insert of this checkpoint: a digest already in the
-// table belongs to an earlier checkpoint, so its leaf is a shifted duplicate and
-// needs no insert (include/reference_impl.hpp:51,66-67).  Plain loads, so popular
-// digests (zero chunks) are served from cache.
-__global__ __launch_bounds__(kThreads) void k_lookup(Launch<DenseArgs> L) {
-  const Where w = where(L);
-  const DenseArgs a = L.args[w.m].global();
-  const uint64_t total = *a.total;
-  for (uint64_t i = (uint64_t)w.blk * kThreads + threadIdx.x; i < total; i += (uint64_t)w.nblk * kThreads) {
-    const uint32_t c = a.dense[i];
-    const uint32_t s = table_find(a.tab, a.mask, a.leaf_dg[c]);
-    if (s == ~0u) continue;
-    a.leaf_slot[c] = s;
-    if (a.labels[c] == L_PENDING) a.labels[c] = L_SHIFT;
-  }
-}
-
 // Table inserts (src/tree_approach.cpp:320-349; include/reference_impl.hpp:51): the
 // packed value (cur << 32 | leaf) with atomicMin leaves the lowest leaf of a new
-// digest in the table.  Consecutive entries with the same digest (zero-filled
-// runs) share the first one's table access.
+// digest in the table, and leaves an earlier checkpoint's entry as it is (k_resolve
+// then labels the leaf a shifted duplicate).  Each probe reads the slot's sector
+// before any claim: a digest already in the table costs loads only, which matters
+// for digests that many changed chunks share (a copied zero chunk), where claim-
+// first atomics would serialise on one address.  Right after a reset every digest
+// is new and the home slot is claimed directly.  Consecutive entries with the same
+// digest (zero-filled runs) share the first one's table access.
 __global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
   const Where w = where(L);
   const DenseArgs a = L.args[w.m].global();
@@ -646,8 +635,7 @@ __global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
     Digest d{0, 0};
     if (v) {
       c = a.dense[i];
-      v = a.labels[c] != L_SHIFT;  // found by k_lookup: nothing to insert
-      if (v) d = a.leaf_dg[c];
+      d = a.leaf_dg[c];
     }
     const uint64_t plo = __shfl_up(d.lo, 1), phi = __shfl_up(d.hi, 1);
     const int pv = __shfl_up((int)v, 1);
@@ -656,7 +644,7 @@ __global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
     uint32_t slot = ~0u;
     bool claimed = false;
     if (head) slot = table_insert(a.tab, a.mask, d, ((uint64_t)a.cur << 32) | (uint32_t)(a.n - 1 + c), true,
-                                  &a.ctl->err, &claimed, true);
+                                  &a.ctl->err, &claimed, a.baseline_hint);
     const uint64_t mine = __ballot(head) & ((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
     const uint32_t hslot = __shfl(slot, mine ? 63 - __clzll((long long)mine) : (int)lane);
     if (follower) slot = hslot;
@@ -2069,20 +2057,18 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
     std::vector<CompactArgs> ca;
     std::vector<uint32_t> cb;
     std::vector<DenseArgs> da;
-    std::vector<uint32_t> db, dlook;
+    std::vector<uint32_t> db;
     for (auto& M : ms) {
       mtc_dedup* h = M.h;
       ca.push_back(CompactArgs{h->list.p, h->blk_cnt.p, h->blk_off.p, M.tpb, h->dense.p, M.nblk_leaf});
       cb.push_back((M.nblk_leaf + kCompactPer - 1) / kCompactPer);
       da.push_back(DenseArgs{h->dense.p, h->blk_off.p + M.nblk_leaf, M.n, h->leaf_dg.p, h->leaf_slot.p, h->labels.p,
                              h->table.p, h->table_cap - 1, M.cur, h->ctl_d, h->ctr.p, M.g, M.R0, h->tile_flags.p,
-                             h->tile_flags.p + M.ntile0});
+                             h->tile_flags.p + M.ntile0, M.baseline ? 1 : 0});
       db.push_back(grid_for(M.n, 8 * kThreads, 16384));
-      dlook.push_back(M.baseline ? 0u : db.back());  // a reset table holds nothing to find
     }
     ops.add<CompactArgs>(20, img.add(ca, cb), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
     const LaunchRec rd = img.add(da, db);
-    ops.add<DenseArgs>(21, img.add(da, dlook), [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_lookup<<<g, kThreads, 0, st>>>(L); });
     ops.add<DenseArgs>(22, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_insert<<<g, kThreads, 0, st>>>(L); });
     ops.add<DenseArgs>(23, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_resolve<<<g, kThreads, 0, st>>>(L); });
   }

@@ -342,7 +342,7 @@ __device__ __forceinline__ void table_update_val(Slot* s, uint64_t v, bool use_m
 // Insert d with value v.  use_min: val = min(val, v) (leaf inserts); otherwise
 // val is set only if still empty (insert-if-absent, first-occurrence nodes).
 // cas_first: the digest is expected to be new, so the home slot is claimed without
-// reading it first.  Returns the slot index, or ~0u on error (bit set in *err).
+// reading it first (a digest already present then costs a failed claim, an atomic).  Returns the slot index, or ~0u on error (bit set in *err).
 // *claimed = 1 when this call created the entry.
 __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& d, uint64_t v, bool use_min,
                                         uint32_t* err, bool* claimed, bool cas_first = false) {
