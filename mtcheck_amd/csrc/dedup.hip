@@ -596,7 +596,6 @@ struct DenseArgs {
   uint32_t R0;          // levels of the first sweep stage
   uint8_t* tile_first;  // per stage-0 tile: holds a first-occurrence leaf
   uint8_t* tile_busy;   // per stage-0 tile: holds a leaf that is not a fixed duplicate
-  int baseline_hint;    // the table was reset for this checkpoint: every digest is new
   __device__ DenseArgs global() const {
     DenseArgs c = *this;
     MTC_GLOBAL(dense); MTC_GLOBAL(total); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(labels); MTC_GLOBAL(tab); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(tile_first); MTC_GLOBAL(tile_busy);
@@ -618,9 +617,9 @@ __device__ __forceinline__ uint64_t chunk_tile(const Geo& g, uint32_t R0, uint64
 // then labels the leaf a shifted duplicate).  Each probe reads the slot's sector
 // before any claim: a digest already in the table costs loads only, which matters
 // for digests that many changed chunks share (a copied zero chunk), where claim-
-// first atomics would serialise on one address.  Right after a reset every digest
-// is new and the home slot is claimed directly.  Consecutive entries with the same
-// digest (zero-filled runs) share the first one's table access.
+// first atomics would serialise on one address; after a reset, the heads of all
+// zero-filled runs meet there.  Consecutive entries with the same digest (one
+// zero-filled run) share the first one's table access.
 __global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
   const Where w = where(L);
   const DenseArgs a = L.args[w.m].global();
@@ -644,7 +643,7 @@ __global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
     uint32_t slot = ~0u;
     bool claimed = false;
     if (head) slot = table_insert(a.tab, a.mask, d, ((uint64_t)a.cur << 32) | (uint32_t)(a.n - 1 + c), true,
-                                  &a.ctl->err, &claimed, a.baseline_hint);
+                                  &a.ctl->err, &claimed);
     const uint64_t mine = __ballot(head) & ((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
     const uint32_t hslot = __shfl(slot, mine ? 63 - __clzll((long long)mine) : (int)lane);
     if (follower) slot = hslot;
@@ -2064,7 +2063,7 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
       cb.push_back((M.nblk_leaf + kCompactPer - 1) / kCompactPer);
       da.push_back(DenseArgs{h->dense.p, h->blk_off.p + M.nblk_leaf, M.n, h->leaf_dg.p, h->leaf_slot.p, h->labels.p,
                              h->table.p, h->table_cap - 1, M.cur, h->ctl_d, h->ctr.p, M.g, M.R0, h->tile_flags.p,
-                             h->tile_flags.p + M.ntile0, M.baseline ? 1 : 0});
+                             h->tile_flags.p + M.ntile0});
       db.push_back(grid_for(M.n, 8 * kThreads, 16384));
     }
     ops.add<CompactArgs>(20, img.add(ca, cb), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
