@@ -642,12 +642,19 @@ __global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
     const bool head = v && !follower;
     uint32_t slot = ~0u;
     bool claimed = false;
+    uint64_t seen = TEMPTY;
     if (head) slot = table_insert(a.tab, a.mask, d, ((uint64_t)a.cur << 32) | (uint32_t)(a.n - 1 + c), true,
-                                  &a.ctl->err, &claimed);
+                                  &a.ctl->err, &claimed, false, &seen);
     const uint64_t mine = __ballot(head) & ((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
     const uint32_t hslot = __shfl(slot, mine ? 63 - __clzll((long long)mine) : (int)lane);
     if (follower) slot = hslot;
     if (v) a.leaf_slot[c] = slot;
+    // Decided here, so k_resolve skips them: a follower's leaf is above its head's,
+    // so it never holds the minimum, and an entry of an earlier checkpoint keeps
+    // its value.  A zero digest is left to k_resolve: after a reset it may belong
+    // to a fixed leaf, whose label stays.
+    const bool older = head && !claimed && seen != TEMPTY && (uint32_t)(seen >> 32) < a.cur;
+    if ((follower || older) && (d.lo | d.hi) != 0) a.labels[c] = L_SHIFT;
     claims += claimed;
   }
   const uint32_t wc = wave_sum(claims);

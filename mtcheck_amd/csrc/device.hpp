@@ -330,23 +330,29 @@ constexpr uint32_t kLineSlots = 2;
 // The value of a matched entry only ever decreases (use_min) or is set once (insert-
 // if-absent).  Skipping the atomic when it cannot change keeps lanes that repeat a
 // popular digest (zero chunks) from serialising on one address.
-__device__ __forceinline__ void table_update_val(Slot* s, uint64_t v, bool use_min) {
+// Returns the value it read.
+__device__ __forceinline__ uint64_t table_update_val(Slot* s, uint64_t v, bool use_min) {
   const uint64_t now = ld_relaxed(&s->val);
   if (use_min) {
     if (now > v) min_relaxed(&s->val, v);
   } else if (now == TEMPTY) {
     cas_relaxed(&s->val, TEMPTY, v);
   }
+  return now;
 }
 
 // Insert d with value v.  use_min: val = min(val, v) (leaf inserts); otherwise
 // val is set only if still empty (insert-if-absent, first-occurrence nodes).
 // cas_first: the digest is expected to be new, so the home slot is claimed without
 // reading it first (a digest already present then costs a failed claim, an atomic).  Returns the slot index, or ~0u on error (bit set in *err).
-// *claimed = 1 when this call created the entry.
+// *claimed = 1 when this call created the entry.  *seen (if given) receives the
+// value a matched entry held when it was read (TEMPTY when this call claimed it, or
+// when its claimer had not stored the value yet).
 __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& d, uint64_t v, bool use_min,
-                                        uint32_t* err, bool* claimed, bool cas_first = false) {
+                                        uint32_t* err, bool* claimed, bool cas_first = false,
+                                        uint64_t* seen = nullptr) {
   *claimed = false;
+  if (seen) *seen = TEMPTY;
   if (d.lo == TEMPTY) {  // overflow slot, keyed by k1 alone
     Slot* s = tab + mask + 1;
     uint64_t old = cas_relaxed(&s->k1, TEMPTY, d.hi);
@@ -402,7 +408,8 @@ __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& 
     if (k0 == d.lo) {
       const uint64_t k1 = ld_relaxed(&s->k1);
       if (k1 == d.hi) {
-        table_update_val(s, v, use_min);
+        const uint64_t now = table_update_val(s, v, use_min);
+        if (seen) *seen = now;
         return (uint32_t)i;
       }
       if (k1 == TEMPTY) {  // claimer has not published k1 yet: look again
