@@ -34,6 +34,10 @@ constexpr uint32_t kSortBlocks = 1024;         // most blocks per member of the 
 #define MTC_TPW_A 8
 #endif
 constexpr uint32_t kTilesPerWaveA = MTC_TPW_A;  // stage-0 tiles per wavefront, sweep A (<= 64)
+#ifndef MTC_TPW_B
+#define MTC_TPW_B 2
+#endif
+constexpr uint32_t kTilesPerWaveB = MTC_TPW_B;  // stage-0 tiles per wavefront, sweep B (1 or 2)
 
 // ---------------------------------------------------------------------------
 // Batched launches
@@ -1170,16 +1174,25 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
   }
 }
 
-// One wavefront per tile.
+// A wavefront per tiles_per_wave (1 or 2) consecutive tiles; the second tile's
+// labels are loaded with the first's, so their latency hides behind its work.
 template <int HK>
 __global__ __launch_bounds__(kThreads) void k_sweep_b(Launch<SweepArgs> L) {
   const Where wh = where(L);
   const SweepArgs a = L.args[wh.m].global();
   const uint32_t lane = lane_id();
-  const uint64_t tile = (uint64_t)wh.blk * (kThreads / 64) + (threadIdx.x >> 6);
-  if (tile >= (1ull << a.d_out)) return;  // whole wavefront
+  const uint64_t ntile = 1ull << a.d_out;
+  const uint64_t t0 = ((uint64_t)wh.blk * (kThreads / 64) + (threadIdx.x >> 6)) * a.tiles_per_wave;
+  if (t0 >= ntile) return;  // whole wavefront
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
-  sweep_b_tile<HK>(a, tile, load_lab4(a, tile << R, T, lane), lane);
+  const Lab4 lb0 = load_lab4(a, t0 << R, T, lane);
+  if (a.tiles_per_wave > 1 && t0 + 1 < ntile) {
+    const Lab4 lb1 = load_lab4(a, (t0 + 1) << R, T, lane);
+    sweep_b_tile<HK>(a, t0, lb0, lane);
+    sweep_b_tile<HK>(a, t0 + 1, lb1, lane);
+  } else {
+    sweep_b_tile<HK>(a, t0, lb0, lane);
+  }
 }
 
 
@@ -2137,7 +2150,7 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
             a.out_slot = h->fr_slot.p + out_off;
             // stage 0 of sweep A: several tiles per wavefront (their flags are read at
             // once); four wavefronts per block
-            a.tiles_per_wave = (a.stage0 && pass == 0) ? kTilesPerWaveA : 1u;
+            a.tiles_per_wave = a.stage0 ? (pass == 0 ? kTilesPerWaveA : kTilesPerWaveB) : 1u;
             const uint64_t waves = ((1ull << a.d_out) + a.tiles_per_wave - 1) / a.tiles_per_wave;
             blocks = (uint32_t)((waves + 3) / 4);
           }
