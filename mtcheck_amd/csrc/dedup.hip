@@ -28,16 +28,16 @@ namespace mtc {
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kFuse = 8;                  // most tree levels fused per sweep stage (one wavefront tile)
 constexpr uint64_t kFirstTag = 0xFFFFFFFFull;  // staging tag of a first-occurrence root
+// Staging tag of a shifted root that is a leaf: its table slot is leaf_slot of its
+// chunk (leaf n - 1 + c holds chunk c), read where the entry is consumed.  Table
+// slots are at most 2^31, so neither tag is a slot.
+constexpr uint64_t kLeafTag = 0xFFFFFFFEull;
 constexpr uint32_t kScanTile = 8 * kThreads;   // elements per block-tile of the batched scan
 constexpr uint32_t kSortBlocks = 1024;         // most blocks per member of the stable sort
 #ifndef MTC_TPW_A
 #define MTC_TPW_A 8
 #endif
 constexpr uint32_t kTilesPerWaveA = MTC_TPW_A;  // stage-0 tiles per wavefront, sweep A (<= 64)
-#ifndef MTC_TPW_B
-#define MTC_TPW_B 2
-#endif
-constexpr uint32_t kTilesPerWaveB = MTC_TPW_B;  // stage-0 tiles per wavefront, sweep B (1 or 2)
 
 // ---------------------------------------------------------------------------
 // Batched launches
@@ -1073,7 +1073,7 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
       lf[k] = a.labels[c];
       if (lf[k] == L_SHIFT) {
         gf[k] = a.leaf_dg[c];
-        sf[k] = a.leaf_slot[c];
+        sf[k] = (uint32_t)kLeafTag;
       }
       busy |= (lf[k] != L_FIXED);
     }
@@ -1093,7 +1093,7 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
       SweepB<HK> s{a};
 #pragma unroll
       for (int i = 0; i < 4; i++)
-        if (is_region(l0[i])) s.emit(l0[i], l0[i] == L_SHIFT ? a.leaf_slot[c0 + i] : 0u, heap_id(a.d_in, base + lp0 + i));
+        if (is_region(l0[i])) s.emit(l0[i], (uint32_t)kLeafTag, heap_id(a.d_in, base + lp0 + i));
       s.flush(a.d_in, tile);
       if (lane == 0) {
         a.out_lab[tile] = L_DONE;
@@ -1105,12 +1105,13 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
       return;
     }
   }
-  // Digests and table slots of the shifted positions.
+  // Digests and table slots of the shifted positions (a leaf's slot is read when
+  // its root entry is consumed).
 #pragma unroll
   for (int i = 0; i < 4; i++)
     if ((uint32_t)i < nval && l0[i] == L_SHIFT) {
       g0[i] = a.stage0 ? a.leaf_dg[c0 + i] : a.in_dg[c0 + i];
-      s0[i] = a.stage0 ? a.leaf_slot[c0 + i] : a.in_slot[c0 + i];
+      s0[i] = a.stage0 ? (uint32_t)kLeafTag : a.in_slot[c0 + i];
     }
   SweepB<HK> s{a};
   uint8_t l1[2] = {L_NONE, L_NONE};
@@ -1174,25 +1175,16 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
   }
 }
 
-// A wavefront per tiles_per_wave (1 or 2) consecutive tiles; the second tile's
-// labels are loaded with the first's, so their latency hides behind its work.
+// One wavefront per tile.
 template <int HK>
 __global__ __launch_bounds__(kThreads) void k_sweep_b(Launch<SweepArgs> L) {
   const Where wh = where(L);
   const SweepArgs a = L.args[wh.m].global();
   const uint32_t lane = lane_id();
-  const uint64_t ntile = 1ull << a.d_out;
-  const uint64_t t0 = ((uint64_t)wh.blk * (kThreads / 64) + (threadIdx.x >> 6)) * a.tiles_per_wave;
-  if (t0 >= ntile) return;  // whole wavefront
+  const uint64_t tile = (uint64_t)wh.blk * (kThreads / 64) + (threadIdx.x >> 6);
+  if (tile >= (1ull << a.d_out)) return;  // whole wavefront
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
-  const Lab4 lb0 = load_lab4(a, t0 << R, T, lane);
-  if (a.tiles_per_wave > 1 && t0 + 1 < ntile) {
-    const Lab4 lb1 = load_lab4(a, (t0 + 1) << R, T, lane);
-    sweep_b_tile<HK>(a, t0, lb0, lane);
-    sweep_b_tile<HK>(a, t0 + 1, lb1, lane);
-  } else {
-    sweep_b_tile<HK>(a, t0, lb0, lane);
-  }
+  sweep_b_tile<HK>(a, tile, load_lab4(a, tile << R, T, lane), lane);
 }
 
 
@@ -1253,6 +1245,8 @@ constexpr uint32_t kLdsBins = 4096;  // source-checkpoint bins counted in LDS
 struct ShiftArgs {
   const uint64_t* shift_tmp;
   const uint32_t* S;
+  const uint32_t* leaf_slot;
+  uint32_t n;
   const Slot* tab;
   uint64_t mask;
   uint32_t ref, cur;
@@ -1263,7 +1257,7 @@ struct ShiftArgs {
   Ctl* ctl;
   __device__ ShiftArgs global() const {
     ShiftArgs c = *this;
-    MTC_GLOBAL(shift_tmp); MTC_GLOBAL(S); MTC_GLOBAL(tab); MTC_GLOBAL(key); MTC_GLOBAL(val); MTC_GLOBAL(src_hist); MTC_GLOBAL(ctl);
+    MTC_GLOBAL(shift_tmp); MTC_GLOBAL(S); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(tab); MTC_GLOBAL(key); MTC_GLOBAL(val); MTC_GLOBAL(src_hist); MTC_GLOBAL(ctl);
     return c;
   }
 };
@@ -1279,7 +1273,9 @@ __global__ __launch_bounds__(kThreads) void k_shift_src(Launch<ShiftArgs> L) {
   const uint32_t S = *a.S;
   for (uint32_t i = w.blk * kThreads + threadIdx.x; i < S; i += w.nblk * kThreads) {
     const uint64_t e = a.shift_tmp[i];
-    const uint32_t node = (uint32_t)e, slot = (uint32_t)(e >> 32);
+    const uint32_t node = (uint32_t)e;
+    uint32_t slot = (uint32_t)(e >> 32);
+    if (slot == (uint32_t)kLeafTag) slot = (node >= a.n - 1 && node < 2 * a.n - 1) ? a.leaf_slot[node - (a.n - 1)] : ~0u;
     uint32_t k = 0, prev = 0;
     if (slot > a.mask + 1) {
       atomicOr(&a.ctl->err, ERR_CORRUPT);
@@ -2150,7 +2146,7 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
             a.out_slot = h->fr_slot.p + out_off;
             // stage 0 of sweep A: several tiles per wavefront (their flags are read at
             // once); four wavefronts per block
-            a.tiles_per_wave = a.stage0 ? (pass == 0 ? kTilesPerWaveA : kTilesPerWaveB) : 1u;
+            a.tiles_per_wave = (a.stage0 && pass == 0) ? kTilesPerWaveA : 1u;
             const uint64_t waves = ((1ull << a.d_out) + a.tiles_per_wave - 1) / a.tiles_per_wave;
             blocks = (uint32_t)((waves + 3) / 4);
           }
@@ -2203,7 +2199,7 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
     for (size_t i = 0; i < B; i++) {
       Member& M = ms[i];
       mtc_dedup* h = M.h;
-      sa.push_back(ShiftArgs{h->shift_tmp.p, &h->ctl_d->S, h->table.p, h->table_cap - 1, h->baseline_id, M.cur,
+      sa.push_back(ShiftArgs{h->shift_tmp.p, &h->ctl_d->S, h->leaf_slot.p, M.n, h->table.p, h->table_cap - 1, h->baseline_id, M.cur,
                              h->sort_key[0].p, h->sort_val[0].p, h->src_hist.p, M.nbins, h->ctl_d});
       sb.push_back(M.Gs);
       max_passes = std::max(max_passes, M.passes);

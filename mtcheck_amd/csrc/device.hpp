@@ -394,8 +394,10 @@ __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& 
       const uint64_t old = cas_relaxed(&s->k0, TEMPTY, d.lo);
       if (old == TEMPTY) {
         st_relaxed(&s->k1, d.hi);
+        // Insert-if-absent: the claimer's value stands.  A matcher only claims an
+        // empty value, so its CAS either fails or is overwritten by this store.
         if (use_min) min_relaxed(&s->val, v);
-        else cas_relaxed(&s->val, TEMPTY, v);
+        else st_relaxed(&s->val, v);
         mine = true;
       } else {
         k0 = old;
