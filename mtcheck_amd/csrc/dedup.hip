@@ -648,7 +648,7 @@ __global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
     bool claimed = false;
     uint64_t seen = TEMPTY;
     if (head) slot = table_insert(a.tab, a.mask, d, ((uint64_t)a.cur << 32) | (uint32_t)(a.n - 1 + c), true,
-                                  &a.ctl->err, &claimed, false, &seen);
+                                  &a.ctl->err, &claimed, false, &seen, true);
     const uint64_t mine = __ballot(head) & ((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
     const uint32_t hslot = __shfl(slot, mine ? 63 - __clzll((long long)mine) : (int)lane);
     if (follower) slot = hslot;
@@ -659,10 +659,30 @@ __global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
     // to a fixed leaf, whose label stays.
     const bool older = head && !claimed && seen != TEMPTY && (uint32_t)(seen >> 32) < a.cur;
     if ((follower || older) && (d.lo | d.hi) != 0) a.labels[c] = L_SHIFT;
+    // A head that matched an entry of this checkpoint: k_fixmin applies its min.
+    if (head && !claimed && !older && slot <= a.mask + 1) {
+      const uint8_t l = a.labels[c];
+      a.labels[c] = l == L_FIXED ? L_MIN_FIXED : L_MIN_PENDING;
+    }
     claims += claimed;
   }
   const uint32_t wc = wave_sum(claims);
   if (lane == 0 && wc) atomicAdd(stripe(a.ctr, CTR_CLAIMS), (unsigned long long)wc);
+}
+
+// Once every claimer has stored its value: the mins of the leaves whose insert
+// matched an entry of this checkpoint (digests that several changed chunks share).
+__global__ __launch_bounds__(kThreads) void k_fixmin(Launch<DenseArgs> L) {
+  const Where w = where(L);
+  const DenseArgs a = L.args[w.m].global();
+  const uint64_t total = *a.total;
+  for (uint64_t i = (uint64_t)w.blk * kThreads + threadIdx.x; i < total; i += (uint64_t)w.nblk * kThreads) {
+    const uint32_t c = a.dense[i];
+    const uint8_t l = a.labels[c];
+    if (l != L_MIN_PENDING && l != L_MIN_FIXED) continue;
+    min_relaxed(&a.tab[a.leaf_slot[c]].val, ((uint64_t)a.cur << 32) | (uint32_t)(a.n - 1 + c));
+    a.labels[c] = l == L_MIN_FIXED ? L_FIXED : L_PENDING;
+  }
 }
 
 // Once every insert has landed: a pending leaf is FIRST when its digest's entry is
@@ -2085,6 +2105,7 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
     ops.add<CompactArgs>(20, img.add(ca, cb), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
     const LaunchRec rd = img.add(da, db);
     ops.add<DenseArgs>(22, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_insert<<<g, kThreads, 0, st>>>(L); });
+    ops.add<DenseArgs>(24, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_fixmin<<<g, kThreads, 0, st>>>(L); });
     ops.add<DenseArgs>(23, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_resolve<<<g, kThreads, 0, st>>>(L); });
   }
   // 4. sweeps: every stage of sweep A (and its inserts) before any stage of sweep B

@@ -7,8 +7,8 @@
 
 namespace mtc {
 
-// Node labels: include/utils.hpp:51-57 (FIRST_OCUR..DONE), plus two
-// kernel-internal states.  PENDING = non-fixed leaf whose FIRST/SHIFT status is
+// Node labels: include/utils.hpp:51-57 (FIRST_OCUR..DONE), plus kernel-internal
+// states.  PENDING = non-fixed leaf whose FIRST/SHIFT status is
 // decided once every leaf of the checkpoint has been inserted.
 enum : uint8_t {
   L_FIRST = 0,
@@ -18,6 +18,8 @@ enum : uint8_t {
   L_DONE = 4,
   L_PENDING = 5,
   L_NONE = 6,
+  L_MIN_PENDING = 7,  // leaf insert matched an entry of this checkpoint; its min is still to apply
+  L_MIN_FIXED = 8,    // the same for a fixed leaf right after a reset (it stays fixed)
 };
 
 enum : int { HASH_MURMUR3 = 0, HASH_MD5 = 1 };
@@ -347,10 +349,12 @@ __device__ __forceinline__ uint64_t table_update_val(Slot* s, uint64_t v, bool u
 // reading it first (a digest already present then costs a failed claim, an atomic).  Returns the slot index, or ~0u on error (bit set in *err).
 // *claimed = 1 when this call created the entry.  *seen (if given) receives the
 // value a matched entry held when it was read (TEMPTY when this call claimed it, or
-// when its claimer had not stored the value yet).
+// when its claimer had not stored the value yet).  defer (with use_min): a claimer
+// stores v and a matcher leaves the value alone; the caller applies the matchers'
+// mins after the whole insert pass, when every claimer's store has landed.
 __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& d, uint64_t v, bool use_min,
                                         uint32_t* err, bool* claimed, bool cas_first = false,
-                                        uint64_t* seen = nullptr) {
+                                        uint64_t* seen = nullptr, bool defer = false) {
   *claimed = false;
   if (seen) *seen = TEMPTY;
   if (d.lo == TEMPTY) {  // overflow slot, keyed by k1 alone
@@ -396,7 +400,7 @@ __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& 
         st_relaxed(&s->k1, d.hi);
         // Insert-if-absent: the claimer's value stands.  A matcher only claims an
         // empty value, so its CAS either fails or is overwritten by this store.
-        if (use_min) min_relaxed(&s->val, v);
+        if (use_min && !defer) min_relaxed(&s->val, v);
         else st_relaxed(&s->val, v);
         mine = true;
       } else {
@@ -410,7 +414,7 @@ __device__ inline uint32_t table_insert(Slot* tab, uint64_t mask, const Digest& 
     if (k0 == d.lo) {
       const uint64_t k1 = ld_relaxed(&s->k1);
       if (k1 == d.hi) {
-        const uint64_t now = table_update_val(s, v, use_min);
+        const uint64_t now = defer ? ld_relaxed(&s->val) : table_update_val(s, v, use_min);
         if (seen) *seen = now;
         return (uint32_t)i;
       }
