@@ -659,6 +659,12 @@ __global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
     // to a fixed leaf, whose label stays.
     const bool older = head && !claimed && seen != TEMPTY && (uint32_t)(seen >> 32) < a.cur;
     if ((follower || older) && (d.lo | d.hi) != 0) a.labels[c] = L_SHIFT;
+    // A claimer is the first occurrence unless a marked match below it takes the
+    // entry in k_fixmin, which then relabels it.
+    if (claimed && (d.lo | d.hi) != 0) {
+      a.labels[c] = L_FIRST;
+      if (a.R0) a.tile_first[chunk_tile(a.g, a.R0, c)] = 1;
+    }
     // A head that matched an entry of this checkpoint: k_fixmin applies its min.
     if (head && !claimed && !older && slot <= a.mask + 1) {
       const uint8_t l = a.labels[c];
@@ -672,6 +678,10 @@ __global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
 
 // Once every claimer has stored its value: the mins of the leaves whose insert
 // matched an entry of this checkpoint (digests that several changed chunks share).
+// The first match below the entry's holder sees the holder's value and relabels it
+// a shifted duplicate (a claimer is labelled first by k_insert); later, lower
+// matches do the same to the match they displace, whose own label k_resolve then
+// decides from the final value either way.  Fixed leaves keep their label.
 __global__ __launch_bounds__(kThreads) void k_fixmin(Launch<DenseArgs> L) {
   const Where w = where(L);
   const DenseArgs a = L.args[w.m].global();
@@ -680,7 +690,16 @@ __global__ __launch_bounds__(kThreads) void k_fixmin(Launch<DenseArgs> L) {
     const uint32_t c = a.dense[i];
     const uint8_t l = a.labels[c];
     if (l != L_MIN_PENDING && l != L_MIN_FIXED) continue;
-    min_relaxed(&a.tab[a.leaf_slot[c]].val, ((uint64_t)a.cur << 32) | (uint32_t)(a.n - 1 + c));
+    const uint64_t v = ((uint64_t)a.cur << 32) | (uint32_t)(a.n - 1 + c);
+    const uint64_t old = fetch_min_relaxed(&a.tab[a.leaf_slot[c]].val, v);
+    if (v < old && (uint32_t)(old >> 32) == a.cur) {  // displaced a leaf of this checkpoint
+      const uint32_t node = (uint32_t)old;
+      if (node >= a.n - 1 && node < 2 * a.n - 1) {
+        const uint32_t cd = node - (a.n - 1);
+        const uint8_t ld = a.labels[cd];
+        if (ld != L_FIXED && ld != L_MIN_FIXED) a.labels[cd] = L_SHIFT;
+      }
+    }
     a.labels[c] = l == L_MIN_FIXED ? L_FIXED : L_PENDING;
   }
 }

@@ -309,6 +309,9 @@ __device__ __forceinline__ uint64_t cas_relaxed(uint64_t* p, uint64_t expect, ui
                                        __HIP_MEMORY_SCOPE_AGENT);
   return expect;
 }
+__device__ __forceinline__ uint64_t fetch_min_relaxed(uint64_t* p, uint64_t v) {
+  return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ void min_relaxed(uint64_t* p, uint64_t v) {
   __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
