@@ -428,7 +428,7 @@ def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, wa
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "diff_mib_mean": round(float(np.mean(dsize)) / (1 << 20), 2),
-        "roofline": {"kernel": "k_leaf (hash + fixed test + table insert)", "bound": "hbm",
+        "roofline": {"kernel": "k_leaf (hash + fixed test + label + list of chunks to insert)", "bound": "hbm",
                      "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": leaf_traffic(nbytes, C, hash_name, change),
                      "avg_ms": round(leaf_avg_ms, 4)},
