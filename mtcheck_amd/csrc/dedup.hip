@@ -111,38 +111,26 @@ __device__ __forceinline__ unsigned long long* stripe(unsigned long long* ctr, u
   return ctr + kind * kStripes + (blockIdx.x & (kStripes - 1));
 }
 
-struct CtrArgs {
-  const unsigned long long* ctr;
-  const uint32_t* changed;
-  Ctl* ctl;
-  Ctl* out;  // the member's entry of the step's result array
-  __device__ CtrArgs global() const {
-    CtrArgs c = *this;
-    MTC_GLOBAL(ctr); MTC_GLOBAL(changed); MTC_GLOBAL(ctl); MTC_GLOBAL(out);
-    return c;
-  }
-};
-
 // Fold the striped counters, then copy the member's control block into the step's
-// contiguous result array (one device-to-host copy for the whole batch).
-__global__ __launch_bounds__(kStripes) void k_finish(Launch<CtrArgs> L) {
-  const Where w = where(L);
-  const CtrArgs a = L.args[w.m].global();
+// contiguous result array (one device-to-host copy for the whole batch).  Run by
+// one whole block of kStripes threads, after every counter and control field of
+// the checkpoint is final.
+__device__ void finish_member(const unsigned long long* ctr, const uint32_t* changed, Ctl* ctl, Ctl* out) {
   __shared__ unsigned long long s[CTR_KINDS];
   if (threadIdx.x < CTR_KINDS) s[threadIdx.x] = 0;
   __syncthreads();
   for (uint32_t k = 0; k < CTR_KINDS; k++) {
-    const unsigned long long v = a.ctr[k * kStripes + threadIdx.x];
+    const unsigned long long v = ctr[k * kStripes + threadIdx.x];
     if (v) atomicAdd(&s[k], v);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    a.ctl->claims = s[CTR_CLAIMS];
-    a.ctl->changed = *a.changed;
+    ctl->claims = s[CTR_CLAIMS];
+    ctl->changed = *changed;
   }
   __syncthreads();
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(a.ctl);
-  uint32_t* dst = reinterpret_cast<uint32_t*>(a.out);
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(ctl);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(out);
   if (threadIdx.x < sizeof(Ctl) / 4) dst[threadIdx.x] = src[threadIdx.x];
 }
 
@@ -1442,10 +1430,10 @@ struct PtableArgs {
   }
 };
 
-__global__ __launch_bounds__(kThreads) void k_ptable(Launch<PtableArgs> L) {
+// The (chkpt, count) table (src/tree_approach.cpp:657-665): the source checkpoints
+// with at least one shifted entry, ascending.  One whole block.
+__device__ void build_ptable(const PtableArgs& a) {
   __shared__ uint32_t s_ws[kThreads / 64];
-  const Where w = where(L);
-  const PtableArgs a = L.args[w.m].global();
   uint32_t base = 0;
   for (uint32_t b0 = 0; b0 < a.nbins; b0 += kThreads) {
     const uint32_t b = b0 + threadIdx.x;
@@ -1466,9 +1454,11 @@ struct RegionArgs {
   const uint32_t* F;
   Geo g;
   uint32_t* sizes;
+  PtableArgs pt;  // the (chkpt, count) table, built by block 0
   __device__ RegionArgs global() const {
     RegionArgs c = *this;
     MTC_GLOBAL(first_roots); MTC_GLOBAL(F); MTC_GLOBAL(sizes);
+    c.pt = pt.global();
     return c;
   }
 };
@@ -1476,6 +1466,7 @@ struct RegionArgs {
 __global__ __launch_bounds__(kThreads) void k_region_sizes(Launch<RegionArgs> L) {
   const Where w = where(L);
   const RegionArgs a = L.args[w.m].global();
+  if (w.blk == 0) build_ptable(a.pt);
   const uint32_t F = *a.F;
   for (uint32_t i = w.blk * kThreads + threadIdx.x; i < F; i += w.nblk * kThreads)
     a.sizes[i] = leaf_count(a.g, a.first_roots[i]);
@@ -1499,16 +1490,18 @@ struct DiffArgs {
   Geo g;
   Ctl* ctl;
   int aligned;
+  const unsigned long long* ctr;  // striped counters
+  const uint32_t* changed;        // chunks inserted by the leaf pass
+  Ctl* out;                       // the member's entry of the step's result array
   __device__ DiffArgs global() const {
     DiffArgs c = *this;
     MTC_GLOBAL(diff); MTC_GLOBAL(first_roots); MTC_GLOBAL(ptable); MTC_GLOBAL(shift_sorted); MTC_GLOBAL(region_off); MTC_GLOBAL(total_chunks); MTC_GLOBAL(data); MTC_GLOBAL(ctl);
+    MTC_GLOBAL(ctr); MTC_GLOBAL(changed); MTC_GLOBAL(out);
     return c;
   }
 };
 
-__global__ __launch_bounds__(kThreads) void k_diff_meta(Launch<DiffArgs> L) {
-  const Where w = where(L);
-  const DiffArgs a = L.args[w.m].global();
+__device__ void diff_meta(const DiffArgs& a, const Where& w) {
   const uint32_t F = a.ctl->F, P = a.ctl->P, S = a.ctl->S;
   uint32_t* out = reinterpret_cast<uint32_t*>(a.diff);  // every metadata field is 4-byte aligned
   const uint64_t oF = 8, oP = oF + F, oS = oP + 2ull * P;
@@ -1539,12 +1532,21 @@ __global__ __launch_bounds__(kThreads) void k_diff_meta(Launch<DiffArgs> L) {
 }
 
 
-// Gather the chunks of every first-occurrence region, regions in root order and
-// leaves in ascending chunk order; the short last chunk is zero padded (:636-650).
-__global__ __launch_bounds__(kThreads) void k_gather(Launch<DiffArgs> L) {
+// The diff in one launch: metadata (every block, grid-stride), then block 0 folds
+// the counters and copies the control block out (its diff size is final once
+// block 0's own metadata thread has written it), then the chunks of every
+// first-occurrence region are gathered, regions in root order and leaves in
+// ascending chunk order; the short last chunk is zero padded (:636-650).
+__global__ __launch_bounds__(kThreads) void k_assemble(Launch<DiffArgs> L) {
+  static_assert(kThreads == kStripes, "one block folds the striped counters");
   __shared__ uint32_t s_src[kThreads];
   const Where wh = where(L);
   const DiffArgs a = L.args[wh.m].global();
+  diff_meta(a, wh);
+  if (wh.blk == 0) {
+    __syncthreads();  // the header thread's control fields
+    finish_member(a.ctr, a.changed, a.ctl, a.out);
+  }
   const uint32_t F = a.ctl->F;
   if (F == 0) return;
   const uint64_t total = *a.total_chunks;
@@ -1961,6 +1963,7 @@ struct Member {
   const uint8_t* data;
   uint64_t len;
   bool baseline;
+  Ctl* ctl_out = nullptr;  // this member's entry of the step's result array
   // derived
   uint32_t n = 0, cur = 0, nblk_leaf = 0, nseg = 0, nbins = 0, passes = 0, Gs = 0, R0 = 0;
   uint64_t tpb = 1, ntile0 = 1;
@@ -2052,7 +2055,6 @@ static void add_scan(Ops& ops, ArenaImage& img, const std::vector<ScanSeg>& segs
 // scan of the leaf blocks' counts), phase 2 (the rest), or both (phase 0).
 static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int phase) {
   const size_t B = ms.size();
-  std::vector<uint32_t> ones(B, 1);
   if (phase != 2) {
   // 1. per-step zero fills (counters, tile flags, root counts, histograms, debug views)
   {
@@ -2267,9 +2269,6 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
       add_scan(ops, img, hs);
       ops.add<SortArgs>(62, r, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_move<<<g, kThreads, 0, st>>>(L); });
     }
-    std::vector<PtableArgs> pa;
-    for (auto& M : ms) pa.push_back(PtableArgs{M.h->src_hist.p, M.nbins, M.h->baseline_id, M.h->ptable.p, M.h->ctl_d});
-    ops.add<PtableArgs>(63, img.add(pa, ones), [](Launch<PtableArgs> L, uint32_t g, hipStream_t st) { k_ptable<<<g, kThreads, 0, st>>>(L); });
   }
   // 7. region sizes and offsets, metadata, gathered data, counters
   {
@@ -2278,14 +2277,15 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
     std::vector<ScanSeg> ss;
     for (auto& M : ms) {
       mtc_dedup* h = M.h;
-      ra.push_back(RegionArgs{h->first_roots.p, &h->ctl_d->F, M.g, h->region_off.p});
+      ra.push_back(RegionArgs{h->first_roots.p, &h->ctl_d->F, M.g, h->region_off.p,
+                              PtableArgs{h->src_hist.p, M.nbins, h->baseline_id, h->ptable.p, h->ctl_d}});
       rb.push_back(grid_for(M.n, 16 * kThreads, 256));
       ss.push_back(ScanSeg{h->region_off.p, h->region_off.p, &h->ctl_d->F, M.n, h->region_off.p + M.n, M.ws_reg});
     }
     ops.add<RegionArgs>(70, img.add(ra, rb), [](Launch<RegionArgs> L, uint32_t g, hipStream_t st) { k_region_sizes<<<g, kThreads, 0, st>>>(L); });
     add_scan(ops, img, ss);
     std::vector<DiffArgs> da;
-    std::vector<uint32_t> mb, gb;
+    std::vector<uint32_t> gb;
     for (size_t i = 0; i < B; i++) {
       Member& M = ms[i];
       mtc_dedup* h = M.h;
@@ -2305,12 +2305,13 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
       d.g = M.g;
       d.ctl = h->ctl_d;
       d.aligned = ((h->chunk_size % 16) == 0 && (uintptr_t)M.data % 16 == 0) ? 1 : 0;
+      d.ctr = h->ctr.p;
+      d.changed = h->blk_off.p + M.nblk_leaf;
+      d.out = M.ctl_out;
       da.push_back(d);
-      mb.push_back(grid_for(M.n, 16 * kThreads, 2048));
       gb.push_back(grid_for(M.n, 8 * kThreads, 8192));
     }
-    ops.add<DiffArgs>(71, img.add(da, mb), [](Launch<DiffArgs> L, uint32_t g, hipStream_t st) { k_diff_meta<<<g, kThreads, 0, st>>>(L); });
-    ops.add<DiffArgs>(72, img.add(da, gb), [](Launch<DiffArgs> L, uint32_t g, hipStream_t st) { k_gather<<<g, kThreads, 0, st>>>(L); });
+    ops.add<DiffArgs>(72, img.add(da, gb), [](Launch<DiffArgs> L, uint32_t g, hipStream_t st) { k_assemble<<<g, kThreads, 0, st>>>(L); });
   }
 }
 
@@ -2375,13 +2376,7 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
   const bool debug = std::any_of(ms.begin(), ms.end(), [](const Member& M) { return M.h->debug != 0; });
   const bool split = std::any_of(ms.begin(), ms.end(), [](const Member& M) { return M.split; });
   ctx.ctl_all.ensure(B);
-  auto add_finish = [&](Ops& ops, ArenaImage& img) {
-    std::vector<CtrArgs> fa;
-    for (size_t i = 0; i < B; i++)
-      fa.push_back(CtrArgs{ms[i].h->ctr.p, ms[i].h->blk_off.p + ms[i].nblk_leaf, ms[i].h->ctl_d, ctx.ctl_all.p + i});
-    ops.add<CtrArgs>(90, img.add(fa, std::vector<uint32_t>(B, 1)),
-                     [](Launch<CtrArgs> L, uint32_t g, hipStream_t st) { k_finish<<<g, kStripes, 0, st>>>(L); });
-  };
+  for (size_t i = 0; i < B; i++) ms[i].ctl_out = ctx.ctl_all.p + i;
   // Uploads the arena image and runs the launches, replayed from a graph or eagerly
   // (eager runs time the leaf launches on their own: ev[1] .. ev[4]).
   auto run_ops = [&](Ops& ops, ArenaImage& img, bool graph) {
@@ -2427,7 +2422,6 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
     Ops ops;
     ArenaImage img;
     build_ops(ms, ops, img, 0);
-    add_finish(ops, img);
     ht.mark("build");
     run_ops(ops, img, use_graph && !debug);
   } else {
@@ -2452,7 +2446,6 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
     Ops ops2;
     ArenaImage img2;
     build_ops(ms, ops2, img2, 2);
-    add_finish(ops2, img2);
     run_ops(ops2, img2, false);
   }
   if (debug)
