@@ -20,6 +20,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <tuple>
 
 #include "internal.hpp"
 
@@ -527,6 +528,85 @@ __global__ __launch_bounds__(kThreads) void k_leaf(Launch<LeafArgs> L) {
   if (threadIdx.x == 0) a.blk_cnt[wh.blk] = run;
 }
 
+// One 256-chunk tile per block, 128-byte chunks: the kernel used when a block
+// takes a single tile.  LDS holds exactly the tile (32 KiB: five blocks per CU),
+// its 16-byte columns swizzled by (chunk & 7) instead of padded, so both the
+// row-wise stores and the per-lane reads stay free of bank conflicts.  Each
+// wavefront reserves its run of the block's region with one atomic add (the
+// block's count is zeroed by the step's first fill), so no block-wide count is
+// needed; within a region the runs are in arrival order.
+template <int HK>
+__global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
+  constexpr int CB = 128;
+  __shared__ __align__(16) uint8_t lds[kThreads * CB];
+  const Where wh = where(L);
+  const LeafArgs a = L.args[wh.m].global();
+  const uint64_t t = wh.blk;
+  const uint64_t full = a.len / CB;
+  const uint32_t tid = threadIdx.x;
+  uint4 v[CB / 16];
+#pragma unroll
+  for (int i = 0; i < CB / 16; i++) {
+    const uint32_t u = i * (kThreads * 16) + 16 * tid;
+    const uint32_t j = u / CB, w = u % CB;
+    const uint64_t chunk = t * kThreads + j;
+    v[i] = uint4{0, 0, 0, 0};
+    if (chunk < full) v[i] = ld_stream16(a.data + chunk * CB + w);
+  }
+  const Digest pv = leaf_prev(a, t);
+#pragma unroll
+  for (int i = 0; i < CB / 16; i++) {
+    const uint32_t u = i * (kThreads * 16) + 16 * tid;
+    const uint32_t j = u / CB, col = (u % CB) / 16;
+    *reinterpret_cast<uint4*>(lds + j * CB + 16 * (col ^ (j & 7))) = v[i];
+  }
+  __syncthreads();
+  const uint8_t* mine = lds + tid * CB;
+  Digest d;
+  if constexpr (HK == HASH_MD5) {
+    uint32_t st[4], X[16];
+    md5_init(st);
+#pragma unroll
+    for (int q = 0; q < CB / 16; q++) {
+      const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * (q ^ (tid & 7)));
+      const int wq = q & 3;
+      X[4 * wq] = b.x;
+      X[4 * wq + 1] = b.y;
+      X[4 * wq + 2] = b.z;
+      X[4 * wq + 3] = b.w;
+      if (wq == 3) md5_compress(st, X);
+    }
+    uint32_t T[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) T[k] = 0u;
+    md5_finish_words(st, T, 0u, (uint64_t)CB);
+    d = md5_digest(st);
+  } else {
+    uint64_t h1 = 0, h2 = 0;
+#pragma unroll
+    for (int q = 0; q < CB / 16; q++) {
+      const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * (q ^ (tid & 7)));
+      m3_block(h1, h2, (uint64_t)b.x | ((uint64_t)b.y << 32), (uint64_t)b.z | ((uint64_t)b.w << 32));
+    }
+    d = m3_final(h1, h2, CB);
+  }
+  const uint64_t c = t * kThreads + tid;
+  if (c >= full) d = (c < a.n) ? hash_bytes<HK>(a.data + c * CB, a.len - c * CB) : Digest{0, 0};
+  const bool valid = c < a.n;
+  const bool fixed = valid && d == pv;
+  const bool ins = valid && (a.baseline || !fixed);  // as in leaf_epilogue
+  if (valid) {
+    a.labels[c] = fixed ? L_FIXED : L_PENDING;
+    if (ins) a.leaf_dg[c] = d;
+  }
+  const uint32_t lane = lane_id();
+  const uint64_t m = __ballot(ins);
+  uint32_t base = 0;
+  if (lane == 0 && m) base = atomicAdd(a.blk_cnt + wh.blk, (uint32_t)__popcll(m));
+  base = __shfl(base, 0);
+  if (ins) a.list[t * kThreads + base + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)c;
+}
+
 // Concatenate the per-block regions into the dense list of chunks to insert.
 struct CompactArgs {
   const uint32_t* list;
@@ -575,7 +655,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(Launch<CompactArgs> L) {
 }
 
 struct DenseArgs {
-  const uint32_t* dense;  // chunks to insert, ascending
+  const uint32_t* dense;  // chunks to insert, ascending within each wavefront's run
   const uint32_t* total;  // their number
   uint32_t n;
   const Digest* leaf_dg;
@@ -632,7 +712,10 @@ __global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
     }
     const uint64_t plo = __shfl_up(d.lo, 1), phi = __shfl_up(d.hi, 1);
     const int pv = __shfl_up((int)v, 1);
-    const bool follower = v && lane > 0 && pv && plo == d.lo && phi == d.hi;
+    const uint32_t pc = __shfl_up(c, 1);
+    // Same digest as the entry before, and a higher chunk: a block's runs of the
+    // dense list may come in any order.
+    const bool follower = v && lane > 0 && pv && plo == d.lo && phi == d.hi && pc < c;
     const bool head = v && !follower;
     uint32_t slot = ~0u;
     bool claimed = false;
@@ -2072,6 +2155,7 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
       fill(h->seg_cnt_f.p, M.nseg, 0);
       fill(h->seg_cnt_s.p, M.nseg, 0);
       fill(h->src_hist.p, M.nbins, 0);
+      fill(h->blk_cnt.p, M.nblk_leaf, 0);  // counts that k_leaf1 accumulates
       if (h->debug) {
         fill(h->dbg_tree.p, 4 * (2ull * M.n - 1), 0);
         fill(h->dbg_labels.p, (2ull * M.n - 1 + 3) / 4, 0);
@@ -2081,8 +2165,10 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
   }
   // 2. leaf kernel, one launch per (hash, staged chunk size) group
   {
-    std::map<std::pair<int, int>, std::vector<size_t>> groups;
-    for (size_t i = 0; i < B; i++) groups[{ms[i].h->hash_kind, ms[i].cb}].push_back(i);
+    // (hash, staged chunk size, one tile per block with 128-byte chunks)
+    std::map<std::tuple<int, int, bool>, std::vector<size_t>> groups;
+    for (size_t i = 0; i < B; i++)
+      groups[{ms[i].h->hash_kind, ms[i].cb, ms[i].cb == 128 && ms[i].tpb == 1}].push_back(i);
     for (auto& kv : groups) {
       std::vector<LeafArgs> la;
       std::vector<uint32_t> lb;
@@ -2093,11 +2179,18 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
                               h->list.p, h->blk_cnt.p, M.tpb});
         lb.push_back(M.nblk_leaf);
       }
-      const int hk = kv.first.first, cb = kv.first.second;
-      const uint32_t tag = 100 + (uint32_t)hk * 16 + (uint32_t)(cb ? __builtin_ctz((unsigned)cb) : 0);
-      ops.add<LeafArgs>(tag, img.add(la, lb), [hk, cb](Launch<LeafArgs> L, uint32_t g, hipStream_t st) {
-        if (hk == HASH_MD5) launch_leaf<HASH_MD5>(cb, L, g, st);
-        else launch_leaf<HASH_MURMUR3>(cb, L, g, st);
+      const int hk = std::get<0>(kv.first), cb = std::get<1>(kv.first);
+      const bool lean = std::get<2>(kv.first);
+      const uint32_t tag = 100 + (uint32_t)hk * 32 + (lean ? 16u : 0u) + (uint32_t)(cb ? __builtin_ctz((unsigned)cb) : 0);
+      ops.add<LeafArgs>(tag, img.add(la, lb), [hk, cb, lean](Launch<LeafArgs> L, uint32_t g, hipStream_t st) {
+        if (lean) {
+          if (hk == HASH_MD5) k_leaf1<HASH_MD5><<<g, kThreads, 0, st>>>(L);
+          else k_leaf1<HASH_MURMUR3><<<g, kThreads, 0, st>>>(L);
+        } else if (hk == HASH_MD5) {
+          launch_leaf<HASH_MD5>(cb, L, g, st);
+        } else {
+          launch_leaf<HASH_MURMUR3>(cb, L, g, st);
+        }
       });
     }
   }
