@@ -528,16 +528,23 @@ __global__ __launch_bounds__(kThreads) void k_leaf(Launch<LeafArgs> L) {
   if (threadIdx.x == 0) a.blk_cnt[wh.blk] = run;
 }
 
-// One 256-chunk tile per block, 128-byte chunks: the kernel used when a block
-// takes a single tile.  LDS holds exactly the tile (32 KiB: five blocks per CU),
-// its 16-byte columns swizzled by (chunk & 7) instead of padded, so both the
-// row-wise stores and the per-lane reads stay free of bank conflicts.  Each
+// One 256-chunk tile per block, chunks of 16..128 bytes staged whole: the kernel
+// used when a block takes a single tile.  LDS holds exactly the tile (32 KiB at
+// 128-byte chunks: five blocks per CU), its 16-byte columns swizzled instead of
+// padded: column ^ ((row / (128 / CB)) & (CB / 16 - 1)) gives the eight rows that
+// share a 128-byte bank window distinct columns, so both the row-wise stores and
+// the per-lane reads stay free of bank conflicts.  Each
 // wavefront reserves its run of the block's region with one atomic add (the
 // block's count is zeroed by the step's first fill), so no block-wide count is
 // needed; within a region the runs are in arrival order.
-template <int HK>
+template <int HK, int CB>
+__device__ __forceinline__ uint32_t lean_col(uint32_t row, uint32_t col) {
+  return col ^ ((row / (128 / CB)) & (CB / 16 - 1));
+}
+
+template <int HK, int CB>
 __global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
-  constexpr int CB = 128;
+  static_assert(CB >= 16 && CB <= 128 && (CB & (CB - 1)) == 0, "one staged segment per chunk");
   __shared__ __align__(16) uint8_t lds[kThreads * CB];
   const Where wh = where(L);
   const LeafArgs a = L.args[wh.m].global();
@@ -558,7 +565,7 @@ __global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
   for (int i = 0; i < CB / 16; i++) {
     const uint32_t u = i * (kThreads * 16) + 16 * tid;
     const uint32_t j = u / CB, col = (u % CB) / 16;
-    *reinterpret_cast<uint4*>(lds + j * CB + 16 * (col ^ (j & 7))) = v[i];
+    *reinterpret_cast<uint4*>(lds + j * CB + 16 * lean_col<HK, CB>(j, col)) = v[i];
   }
   __syncthreads();
   const uint8_t* mine = lds + tid * CB;
@@ -568,7 +575,7 @@ __global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
     md5_init(st);
 #pragma unroll
     for (int q = 0; q < CB / 16; q++) {
-      const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * (q ^ (tid & 7)));
+      const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * lean_col<HK, CB>(tid, q));
       const int wq = q & 3;
       X[4 * wq] = b.x;
       X[4 * wq + 1] = b.y;
@@ -576,16 +583,17 @@ __global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
       X[4 * wq + 3] = b.w;
       if (wq == 3) md5_compress(st, X);
     }
+    constexpr int RW = (CB % 64) / 16;  // words of a final partial 64-byte block already in X
     uint32_t T[16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) T[k] = 0u;
-    md5_finish_words(st, T, 0u, (uint64_t)CB);
+    for (int k = 0; k < 16; k++) T[k] = (k < 4 * RW) ? X[k] : 0u;
+    md5_finish_words(st, T, (uint32_t)(CB % 64), (uint64_t)CB);
     d = md5_digest(st);
   } else {
     uint64_t h1 = 0, h2 = 0;
 #pragma unroll
     for (int q = 0; q < CB / 16; q++) {
-      const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * (q ^ (tid & 7)));
+      const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * lean_col<HK, CB>(tid, q));
       m3_block(h1, h2, (uint64_t)b.x | ((uint64_t)b.y << 32), (uint64_t)b.z | ((uint64_t)b.w << 32));
     }
     d = m3_final(h1, h2, CB);
@@ -1831,6 +1839,16 @@ static void launch_leaf(int cb, Launch<LeafArgs> L, uint32_t grid, hipStream_t s
   }
 }
 
+template <int HK>
+static void launch_leaf1(int cb, Launch<LeafArgs> L, uint32_t grid, hipStream_t st) {
+  switch (cb) {
+    case 16: k_leaf1<HK, 16><<<grid, kThreads, 0, st>>>(L); break;
+    case 32: k_leaf1<HK, 32><<<grid, kThreads, 0, st>>>(L); break;
+    case 64: k_leaf1<HK, 64><<<grid, kThreads, 0, st>>>(L); break;
+    default: k_leaf1<HK, 128><<<grid, kThreads, 0, st>>>(L); break;
+  }
+}
+
 // Plan sweep stages and root segments for n chunks (fuse: levels per stage).
 static void plan_tree(mtc_dedup* h) {
   const Geo g = make_geo(h->n);
@@ -2168,7 +2186,7 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
     // (hash, staged chunk size, one tile per block with 128-byte chunks)
     std::map<std::tuple<int, int, bool>, std::vector<size_t>> groups;
     for (size_t i = 0; i < B; i++)
-      groups[{ms[i].h->hash_kind, ms[i].cb, ms[i].cb == 128 && ms[i].tpb == 1}].push_back(i);
+      groups[{ms[i].h->hash_kind, ms[i].cb, ms[i].cb >= 16 && ms[i].cb <= 128 && ms[i].tpb == 1}].push_back(i);
     for (auto& kv : groups) {
       std::vector<LeafArgs> la;
       std::vector<uint32_t> lb;
@@ -2184,8 +2202,8 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
       const uint32_t tag = 100 + (uint32_t)hk * 32 + (lean ? 16u : 0u) + (uint32_t)(cb ? __builtin_ctz((unsigned)cb) : 0);
       ops.add<LeafArgs>(tag, img.add(la, lb), [hk, cb, lean](Launch<LeafArgs> L, uint32_t g, hipStream_t st) {
         if (lean) {
-          if (hk == HASH_MD5) k_leaf1<HASH_MD5><<<g, kThreads, 0, st>>>(L);
-          else k_leaf1<HASH_MURMUR3><<<g, kThreads, 0, st>>>(L);
+          if (hk == HASH_MD5) launch_leaf1<HASH_MD5>(cb, L, g, st);
+          else launch_leaf1<HASH_MURMUR3>(cb, L, g, st);
         } else if (hk == HASH_MD5) {
           launch_leaf<HASH_MD5>(cb, L, g, st);
         } else {
