@@ -537,52 +537,66 @@ __global__ __launch_bounds__(kThreads) void k_leaf(Launch<LeafArgs> L) {
 // wavefront reserves its run of the block's region with one atomic add (the
 // block's count is zeroed by the step's first fill), so no block-wide count is
 // needed; within a region the runs are in arrival order.
-template <int HK, int CB>
+template <int SEG>
 __device__ __forceinline__ uint32_t lean_col(uint32_t row, uint32_t col) {
-  return col ^ ((row / (128 / CB)) & (CB / 16 - 1));
+  return col ^ ((row / (128 / SEG)) & (SEG / 16 - 1));
 }
 
 template <int HK, int CB>
 __global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
-  static_assert(CB >= 16 && CB <= 128 && (CB & (CB - 1)) == 0, "one staged segment per chunk");
-  __shared__ __align__(16) uint8_t lds[kThreads * CB];
+  static_assert(CB >= 16 && (CB & (CB - 1)) == 0, "power-of-two chunk size");
+  constexpr int SEG = CB < 128 ? CB : 128;  // bytes of each chunk per pass
+  constexpr int PASSES = CB / SEG, LOADS = SEG / 16;
+  __shared__ __align__(16) uint8_t lds[kThreads * SEG];
   const Where wh = where(L);
   const LeafArgs a = L.args[wh.m].global();
   const uint64_t t = wh.blk;
   const uint64_t full = a.len / CB;
   const uint32_t tid = threadIdx.x;
-  uint4 v[CB / 16];
-#pragma unroll
-  for (int i = 0; i < CB / 16; i++) {
-    const uint32_t u = i * (kThreads * 16) + 16 * tid;
-    const uint32_t j = u / CB, w = u % CB;
-    const uint64_t chunk = t * kThreads + j;
-    v[i] = uint4{0, 0, 0, 0};
-    if (chunk < full) v[i] = ld_stream16(a.data + chunk * CB + w);
-  }
   const Digest pv = leaf_prev(a, t);
+  const uint8_t* mine = lds + tid * SEG;
+  uint64_t h1 = 0, h2 = 0;
+  uint32_t st[4], X[16];
+  md5_init(st);
+#pragma unroll 1
+  for (int pass = 0; pass < PASSES; pass++) {
+    uint4 v[LOADS];
 #pragma unroll
-  for (int i = 0; i < CB / 16; i++) {
-    const uint32_t u = i * (kThreads * 16) + 16 * tid;
-    const uint32_t j = u / CB, col = (u % CB) / 16;
-    *reinterpret_cast<uint4*>(lds + j * CB + 16 * lean_col<HK, CB>(j, col)) = v[i];
+    for (int i = 0; i < LOADS; i++) {
+      const uint32_t u = i * (kThreads * 16) + 16 * tid;
+      const uint64_t chunk = t * kThreads + u / SEG;
+      v[i] = uint4{0, 0, 0, 0};
+      if (chunk < full) v[i] = ld_stream16(a.data + chunk * CB + (uint64_t)pass * SEG + u % SEG);
+    }
+    if (PASSES > 1) __syncthreads();  // the previous pass's reads are done
+#pragma unroll
+    for (int i = 0; i < LOADS; i++) {
+      const uint32_t u = i * (kThreads * 16) + 16 * tid;
+      const uint32_t j = u / SEG, col = (u % SEG) / 16;
+      *reinterpret_cast<uint4*>(lds + j * SEG + 16 * lean_col<SEG>(j, col)) = v[i];
+    }
+    __syncthreads();
+    if constexpr (HK == HASH_MD5) {
+#pragma unroll
+      for (int q = 0; q < LOADS; q++) {
+        const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * lean_col<SEG>(tid, q));
+        const int wq = q & 3;  // LOADS is a multiple of 4 whenever PASSES > 1
+        X[4 * wq] = b.x;
+        X[4 * wq + 1] = b.y;
+        X[4 * wq + 2] = b.z;
+        X[4 * wq + 3] = b.w;
+        if (wq == 3) md5_compress(st, X);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < LOADS; q++) {
+        const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * lean_col<SEG>(tid, q));
+        m3_block(h1, h2, (uint64_t)b.x | ((uint64_t)b.y << 32), (uint64_t)b.z | ((uint64_t)b.w << 32));
+      }
+    }
   }
-  __syncthreads();
-  const uint8_t* mine = lds + tid * CB;
   Digest d;
   if constexpr (HK == HASH_MD5) {
-    uint32_t st[4], X[16];
-    md5_init(st);
-#pragma unroll
-    for (int q = 0; q < CB / 16; q++) {
-      const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * lean_col<HK, CB>(tid, q));
-      const int wq = q & 3;
-      X[4 * wq] = b.x;
-      X[4 * wq + 1] = b.y;
-      X[4 * wq + 2] = b.z;
-      X[4 * wq + 3] = b.w;
-      if (wq == 3) md5_compress(st, X);
-    }
     constexpr int RW = (CB % 64) / 16;  // words of a final partial 64-byte block already in X
     uint32_t T[16];
 #pragma unroll
@@ -590,12 +604,6 @@ __global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
     md5_finish_words(st, T, (uint32_t)(CB % 64), (uint64_t)CB);
     d = md5_digest(st);
   } else {
-    uint64_t h1 = 0, h2 = 0;
-#pragma unroll
-    for (int q = 0; q < CB / 16; q++) {
-      const uint4 b = *reinterpret_cast<const uint4*>(mine + 16 * lean_col<HK, CB>(tid, q));
-      m3_block(h1, h2, (uint64_t)b.x | ((uint64_t)b.y << 32), (uint64_t)b.z | ((uint64_t)b.w << 32));
-    }
     d = m3_final(h1, h2, CB);
   }
   const uint64_t c = t * kThreads + tid;
@@ -1845,7 +1853,12 @@ static void launch_leaf1(int cb, Launch<LeafArgs> L, uint32_t grid, hipStream_t 
     case 16: k_leaf1<HK, 16><<<grid, kThreads, 0, st>>>(L); break;
     case 32: k_leaf1<HK, 32><<<grid, kThreads, 0, st>>>(L); break;
     case 64: k_leaf1<HK, 64><<<grid, kThreads, 0, st>>>(L); break;
-    default: k_leaf1<HK, 128><<<grid, kThreads, 0, st>>>(L); break;
+    case 128: k_leaf1<HK, 128><<<grid, kThreads, 0, st>>>(L); break;
+    case 256: k_leaf1<HK, 256><<<grid, kThreads, 0, st>>>(L); break;
+    case 512: k_leaf1<HK, 512><<<grid, kThreads, 0, st>>>(L); break;
+    case 1024: k_leaf1<HK, 1024><<<grid, kThreads, 0, st>>>(L); break;
+    case 2048: k_leaf1<HK, 2048><<<grid, kThreads, 0, st>>>(L); break;
+    default: k_leaf1<HK, 4096><<<grid, kThreads, 0, st>>>(L); break;
   }
 }
 
@@ -2186,7 +2199,7 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
     // (hash, staged chunk size, one tile per block with 128-byte chunks)
     std::map<std::tuple<int, int, bool>, std::vector<size_t>> groups;
     for (size_t i = 0; i < B; i++)
-      groups[{ms[i].h->hash_kind, ms[i].cb, ms[i].cb >= 16 && ms[i].cb <= 128 && ms[i].tpb == 1}].push_back(i);
+      groups[{ms[i].h->hash_kind, ms[i].cb, ms[i].cb >= 16 && ms[i].tpb == 1}].push_back(i);
     for (auto& kv : groups) {
       std::vector<LeafArgs> la;
       std::vector<uint32_t> lb;
