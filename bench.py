@@ -37,6 +37,8 @@ def parse():
     p.add_argument("--cpu-par-sample-mib", type=int, default=2048)
     p.add_argument("--cpu-par-steps", type=int, default=6)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-host-path", action="store_true",
+                   help="skip the PCIe-inclusive runs (diffs copied to pinned host memory)")
     p.add_argument("--load-factor", type=float, default=0.125, help="digest-table entries per slot at a baseline")
     p.add_argument("--tiles-per-block", type=int, default=0, help="leaf-kernel tiles per block (0: the library default)")
     p.add_argument("--sweep", action="store_true",
@@ -369,8 +371,47 @@ def served_bench(args):
             "mean_batch": round(st["served"] / max(1, st["steps"]), 2)}
 
 
+def host_path_runs(versions, C: int, hk: int, warmup: int, steps: int, diff_max: int, tune) -> dict:
+    """The same checkpoints with each diff handed to pinned host memory: serially
+    (checkpoint, then a synchronous copy) and overlapped (the copy of checkpoint k
+    runs during checkpoint k+1; two pinned buffers, a buffer is reused once its copy
+    has finished).  PCIe-inclusive rates; the device-resident rate is `value`."""
+    import torch
+
+    import mtcheck_amd as M
+    nbytes = versions[0].numel()
+    cap = int(diff_max * 1.5) + (1 << 20)
+    pinned = [torch.empty(cap, dtype=torch.uint8, pin_memory=True) for _ in range(2)]
+    out = {}
+    for mode in ("serial", "overlapped"):
+        d = M.TreeDeduplicator(C, hash_kind=hk)
+        if tune:
+            d.set_tuning(*tune)
+        for k in range(warmup):
+            d.checkpoint_device(versions[k], make_baseline=(k == 0))
+        torch.cuda.synchronize()
+        copy_ms = []
+        t0 = time.perf_counter()
+        for j, k in enumerate(range(warmup, warmup + steps)):
+            d.checkpoint_device(versions[k])
+            if mode == "serial":
+                d.copy_last_diff_device(pinned[0])
+                copy_ms.append(d.stats().ms_copy)
+            else:
+                if j >= 2:
+                    d.wait_copies()  # pinned[j % 2] is free again
+                d.copy_last_diff_async(pinned[j % 2])
+        d.wait_copies()
+        elapsed = time.perf_counter() - t0
+        out[f"{mode}_gbs"] = round(nbytes * steps / elapsed / 1e9, 2)
+        if copy_ms:
+            out["d2h_ms_mean"] = round(sum(copy_ms) / len(copy_ms), 3)
+        d.close()
+    return out
+
+
 def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, warmup: int,
-              load_factor: float = 0.125, tiles_per_block: int = 0) -> dict:
+              load_factor: float = 0.125, tiles_per_block: int = 0, host_path: bool = False) -> dict:
     """Incremental checkpoints of one buffer resident in HBM; every version is
     generated before the timed region (version k = version k-1 + step k)."""
     import numpy as np
@@ -416,10 +457,16 @@ def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, wa
     leaf_bytes = [nbytes + 17 * n + 20 * c for c in changed]
     leaf_avg_ms = float(np.mean(leaf_ms))
     achieved = float(np.mean(leaf_bytes)) / (leaf_avg_ms / 1e3)
-    del versions
     d.close()
+    extra = {}
+    if host_path:
+        tune = (tiles_per_block or 1, load_factor, 8) if (tiles_per_block or load_factor != 0.125) else None
+        hp = host_path_runs(versions, C, hk, warmup, steps, max(dsize), tune)
+        hp["d2h_gbs"] = round(float(np.mean(dsize)) / (hp["d2h_ms_mean"] / 1e3) / 1e9, 2) if hp.get("d2h_ms_mean") else None
+        extra["host_path"] = {"workload": "the same checkpoints, each diff copied to pinned host memory", **hp}
+    del versions
     torch.cuda.empty_cache()
-    return {
+    return {**extra,
         "value": round(nbytes * steps / elapsed / 1e9, 3), "ms_per_step": round(elapsed / steps * 1e3, 4),
         "config": {"workload": f"plain single-buffer dedup, {nbytes / 2**30:g} GiB uint8, {C} B chunks, "
                                f"{hash_name}, {change * 100:g}% change per checkpoint (2:1 fresh runs:copies)",
@@ -471,7 +518,7 @@ def main():
         sweep(args)
         return
     r = plain_run(int(args.size_gib * (1 << 30)), args.chunk, args.hash, args.change, args.steps, args.warmup,
-                  args.load_factor, args.tiles_per_block)
+                  args.load_factor, args.tiles_per_block, host_path=not args.no_host_path)
     out = {
         "metric": METRIC, "value": r["value"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -479,6 +526,8 @@ def main():
         "hbm_frac_checkpoint": r["hbm_frac_checkpoint"], "latency_ms": r["latency_ms"],
         "diff_mib_mean": r["diff_mib_mean"], "roofline": r["roofline"],
     }
+    if "host_path" in r:
+        out["host_path"] = r["host_path"]
     hk = M.MD5 if args.hash == "md5" else M.MURMUR3
     if not args.no_batched:
         out["batched"] = batched_bench(args)

@@ -108,6 +108,18 @@ int mtc_diff_device(mtc_dedup* h, const void** dev_diff, uint64_t* size);
  * `capacity` must be at least the diff size. */
 int mtc_copy_diff(mtc_dedup* h, void* host_dst, uint64_t capacity, void* stream);
 
+/* The same copy, queued on the handle's own copy stream; returns at once
+ * (src/tree_approach.cpp:1678-1684 copies synchronously; the overlap is this
+ * library's).  From then on the handle alternates two device diff arenas, so the
+ * next checkpoint runs while this copy is in flight; the checkpoint after it waits,
+ * on the device, until the copy out of its arena has finished.  host_dst should
+ * be pinned (hipHostMalloc): a pageable destination makes the copy synchronous.
+ * host_dst must stay valid, and is complete, once mtc_copy_wait returns. */
+int mtc_copy_diff_async(mtc_dedup* h, void* host_dst, uint64_t capacity);
+
+/* Waits for every copy queued by mtc_copy_diff_async on this handle. */
+int mtc_copy_wait(mtc_dedup* h);
+
 /* num_first_ocur() / num_shift_dupl() (include/tree_approach.hpp:53-59) and timers. */
 int mtc_get_stats(const mtc_dedup* h, mtc_stats* out);
 

@@ -22,7 +22,8 @@ STATUS_NAMES = {
 
 # Every symbol the public headers declare (checked by the CPU test suite).
 EXPORTS = [
-    "mtc_create", "mtc_destroy", "mtc_checkpoint", "mtc_diff_device", "mtc_copy_diff", "mtc_get_stats",
+    "mtc_create", "mtc_destroy", "mtc_checkpoint", "mtc_diff_device", "mtc_copy_diff", "mtc_copy_diff_async",
+    "mtc_copy_wait", "mtc_get_stats",
     "mtc_restart", "mtc_digest_host", "mtc_hash_chunks", "mtc_set_debug", "mtc_copy_tree", "mtc_copy_labels",
     "mtc_last_error", "mtc_status_string", "mtc_set_tuning", "mtc_batch_create", "mtc_batch_destroy",
     "mtc_checkpoint_batch", "mtc_batch_last_error", "mtc_server_create", "mtc_server_destroy", "mtc_server_add_client",
@@ -69,6 +70,8 @@ def lib():
         L.mtc_checkpoint.argtypes = [vp, vp, u64, i32, i32, vp, ctypes.POINTER(Header), ctypes.POINTER(u64)]
         L.mtc_diff_device.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(u64)]
         L.mtc_copy_diff.argtypes = [vp, vp, u64, vp]
+        L.mtc_copy_diff_async.argtypes = [vp, vp, u64]
+        L.mtc_copy_wait.argtypes = [vp]
         L.mtc_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.mtc_restart.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(u64), u32, u32, i32, vp, u64, i32, vp]
         L.mtc_digest_host.argtypes = [vp, u64, i32, vp]

@@ -106,6 +106,12 @@ int mtc_create(mtc_dedup** out, uint32_t chunk_size, int approach, int hash_kind
 int mtc_destroy(mtc_dedup* h) {
   if (!h) return MTC_OK;
   (void)hipSetDevice(h->device);
+  if (h->copy_st) {
+    (void)hipStreamSynchronize(h->copy_st);
+    (void)hipStreamDestroy(h->copy_st);
+    for (hipEvent_t e : h->copy_done)
+      if (e) (void)hipEventDestroy(e);
+  }
   if (h->ctl_d) (void)hipFree(h->ctl_d);
   if (h->ctl_h) (void)hipHostFree(h->ctl_h);
   delete h;  // device buffers are released by their destructors
@@ -129,7 +135,7 @@ int mtc_checkpoint(mtc_dedup* h, const void* dev_data, uint64_t len, int make_ba
 int mtc_diff_device(mtc_dedup* h, const void** dev_diff, uint64_t* size) {
   if (!h || !dev_diff || !size) return MTC_ERR_INVALID_ARG;
   if (!h->has_state || !h->last_diff_size) return MTC_ERR_STATE;
-  *dev_diff = h->diff.p;
+  *dev_diff = h->diff_ptr();
   *size = h->last_diff_size;
   return MTC_OK;
 }
@@ -146,12 +152,37 @@ int mtc_copy_diff(mtc_dedup* h, void* host_dst, uint64_t capacity, void* stream)
     MTC_HIP(hipEventCreate(&b));
     MTC_HIP(hipEventRecord(a, st));
     // host or device destination (unified addressing decides the direction)
-    MTC_HIP(hipMemcpyAsync(host_dst, h->diff.p, h->last_diff_size, hipMemcpyDefault, st));
+    MTC_HIP(hipMemcpyAsync(host_dst, h->diff_ptr(), h->last_diff_size, hipMemcpyDefault, st));
     MTC_HIP(hipEventRecord(b, st));
     MTC_HIP(hipEventSynchronize(b));
     (void)hipEventElapsedTime(&h->last_ms[3], a, b);
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+  });
+}
+
+int mtc_copy_diff_async(mtc_dedup* h, void* host_dst, uint64_t capacity) {
+  if (!h || !host_dst) return MTC_ERR_INVALID_ARG;
+  return guarded(h, [&] {
+    if (!h->last_diff_size) throw Status{MTC_ERR_STATE, "no checkpoint taken"};
+    if (capacity < h->last_diff_size) throw Status{MTC_ERR_BUFFER_TOO_SMALL, "host buffer smaller than the diff"};
+    MTC_HIP(hipSetDevice(h->device));
+    if (!h->copy_st) {
+      MTC_HIP(hipStreamCreateWithFlags(&h->copy_st, hipStreamNonBlocking));
+      for (int i = 0; i < 2; i++) MTC_HIP(hipEventCreateWithFlags(&h->copy_done[i], hipEventDisableTiming));
+    }
+    h->async_copies = true;  // from the next checkpoint on, checkpoints alternate diff arenas
+    // The diff is complete: mtc_checkpoint returns after its last kernel.
+    MTC_HIP(hipMemcpyAsync(host_dst, h->diff_ptr(), h->last_diff_size, hipMemcpyDefault, h->copy_st));
+    MTC_HIP(hipEventRecord(h->copy_done[h->diff_cur], h->copy_st));
+    h->copy_pending[h->diff_cur] = true;
+  });
+}
+
+int mtc_copy_wait(mtc_dedup* h) {
+  if (!h) return MTC_ERR_INVALID_ARG;
+  return guarded(h, [&] {
+    if (h->copy_st) MTC_HIP(hipStreamSynchronize(h->copy_st));
   });
 }
 

@@ -1932,6 +1932,7 @@ static void ensure_buffers(mtc_dedup* h, hipStream_t st) {
   h->src_hist.ensure(pcap);
   h->ptable.ensure(2 * pcap);
   h->diff.ensure(32 + 12 * n + 8 * pcap + n * (uint64_t)h->chunk_size);
+  if (h->async_copies) h->diff_alt.ensure(32 + 12 * n + 8 * pcap + n * (uint64_t)h->chunk_size);
   h->ctr.ensure(CTR_KINDS * kStripes);
   (void)g;
 }
@@ -2078,6 +2079,7 @@ struct Member {
   uint64_t len;
   bool baseline;
   Ctl* ctl_out = nullptr;  // this member's entry of the step's result array
+  int arena = 0;           // diff arena this step writes
   // derived
   uint32_t n = 0, cur = 0, nblk_leaf = 0, nseg = 0, nbins = 0, passes = 0, Gs = 0, R0 = 0;
   uint64_t tpb = 1, ntile0 = 1;
@@ -2128,6 +2130,13 @@ static void prepare_member(Member& M, hipStream_t st) {
   M.cur = h->current_id;
   M.g = make_geo(M.n);
   M.cb = staged_cb(h->chunk_size, (uintptr_t)M.data % 16 == 0);
+  // With asynchronous copies, checkpoints alternate diff arenas; this step waits
+  // (on the device) for the copy still reading the arena it is about to write.
+  M.arena = h->async_copies ? 1 - h->diff_cur : 0;
+  if (h->copy_pending[M.arena]) {
+    MTC_HIP(hipStreamWaitEvent(st, h->copy_done[M.arena], 0));
+    h->copy_pending[M.arena] = false;
+  }
   const uint64_t ntiles = (M.n + kThreads - 1) / kThreads;
   M.tpb = std::max<uint64_t>(1, h->tune.leaf_tiles_per_block);
   M.nblk_leaf = (uint32_t)((ntiles + M.tpb - 1) / M.tpb);
@@ -2414,7 +2423,7 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
       Member& M = ms[i];
       mtc_dedup* h = M.h;
       DiffArgs d{};
-      d.diff = h->diff.p;
+      d.diff = M.arena ? h->diff_alt.p : h->diff.p;
       d.first_roots = h->first_roots.p;
       d.ptable = h->ptable.p;
       d.shift_sorted = sorted[i];
@@ -2610,6 +2619,7 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
     *h->ctl_h = c;
     h->table_size += c.claims;
     h->last_diff_size = c.diff_size;
+    h->diff_cur = ms[i].arena;
     h->last_header = mtc_header{h->baseline_id, ms[i].cur, ms[i].len, h->chunk_size, c.F, c.P, c.S};
     h->last_ms[0] = ms_setup;
     h->last_ms[1] = ms_total;

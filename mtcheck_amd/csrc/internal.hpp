@@ -157,7 +157,14 @@ struct mtc_dedup {
   mtc::DevBuf<uint32_t> src_hist;     // per source checkpoint
   mtc::DevBuf<uint32_t> ptable;       // P (chkpt, count) pairs
   mtc::DevBuf<uint32_t> region_off;   // F region offsets (chunks), then total
-  mtc::DevBuf<uint8_t> diff;          // device diff of the last checkpoint
+  mtc::DevBuf<uint8_t> diff;          // device diff arena 0
+  mtc::DevBuf<uint8_t> diff_alt;      // arena 1, allocated once asynchronous copies are used
+  int diff_cur = 0;                   // arena holding the last checkpoint's diff
+  bool async_copies = false;          // checkpoints alternate arenas
+  hipStream_t copy_st = nullptr;      // asynchronous diff copies
+  hipEvent_t copy_done[2] = {nullptr, nullptr};  // after the last copy out of each arena
+  bool copy_pending[2] = {false, false};
+  const uint8_t* diff_ptr() const { return diff_cur ? diff_alt.p : diff.p; }
   mtc::DevBuf<mtc::Digest> dbg_tree;  // debug: full tree, as the serial reference leaves it
   mtc::DevBuf<uint8_t> dbg_labels;    // debug: all node labels
   mtc::DevBuf<unsigned long long> ctr;  // striped per-checkpoint counters

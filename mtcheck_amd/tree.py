@@ -142,6 +142,20 @@ class TreeDeduplicator:
         check(lib().mtc_copy_diff(self._h, buf, size.value, None), self._h)
         return buf.raw
 
+    def copy_last_diff_async(self, out) -> int:
+        """Queues a copy of the last diff into `out` (a pinned uint8 CPU tensor of at
+        least the diff's size) and returns the diff's size; `out` is complete once
+        wait_copies() returns.  The next checkpoint may run meanwhile."""
+        size = self.stats().diff_size
+        if out.numel() < size:
+            raise ValueError("buffer smaller than the diff")
+        check(lib().mtc_copy_diff_async(self._h, out.data_ptr(), out.numel()), self._h)
+        return size
+
+    def wait_copies(self) -> None:
+        """Waits for every copy queued by copy_last_diff_async."""
+        check(lib().mtc_copy_wait(self._h), self._h)
+
     def checkpoint_header(self, data, make_baseline: bool = False, stream=None):
         """checkpoint(header_t&, ptr, len, diff_h, bool): does not advance the id."""
         hdr, diff = self._run(data, make_baseline, advance=False, stream=stream)
