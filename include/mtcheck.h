@@ -83,7 +83,10 @@ typedef struct mtc_stats {
 } mtc_stats;
 
 /* TreeDeduplicator(uint32_t bytes_per_chunk), src/tree_approach.cpp:5-9, plus the
- * approach selection of src/utils.cpp:15-36.  `device` is a HIP device ordinal. */
+ * approach selection of src/utils.cpp:15-36.  `device` is a HIP device ordinal.
+ * MTC_APPROACH_BASIC (BasicDeduplicator, src/basic_approach.cpp:468-549: the diff
+ * lists every chunk whose digest changed since the previous checkpoint) is switched
+ * off in this build and returns MTC_ERR_UNSUPPORTED_APPROACH. */
 int mtc_create(mtc_dedup** out, uint32_t chunk_size, int approach, int hash_kind, int device);
 
 /* ~TreeDeduplicator, src/tree_approach.cpp (destructor). */
@@ -130,6 +133,13 @@ int mtc_get_stats(const mtc_dedup* h, mtc_stats* out);
  * The restored buffer (header.datalen bytes) is written to device pointer dev_out. */
 int mtc_restart(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t chkpt_id,
                 int diffs_on_device, void* dev_out, uint64_t out_len, int device, void* stream);
+
+/* BasicDeduplicator::restart(View, vector<HostMirror>&, logname, chkpt_id),
+ * src/basic_approach.cpp:593-604 (-> restart_chkpt :155-322), and the raw-pointer
+ * overload :606-614: the same call as mtc_restart for basic diffs (header with
+ * P = S = 0, the changed chunks' indices, then their bytes, C each). */
+int mtc_restart_basic(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t chkpt_id,
+                      int diffs_on_device, void* dev_out, uint64_t out_len, int device, void* stream);
 
 /* calculate_digest_host, include/hash_functions.hpp:46-82: digest of a whole host
  * buffer with the selected hash (restart verification). */

@@ -7,7 +7,7 @@
 namespace mtc {
 void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_baseline, hipStream_t st);
 void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t k, bool on_device,
-                  uint8_t* out, uint64_t out_len, hipStream_t st);
+                  uint8_t* out, uint64_t out_len, hipStream_t st, bool basic);
 void hash_chunks(const uint8_t* data, uint64_t len, uint32_t C, int hk, Digest* out, hipStream_t st);
 void checkpoint_batch_impl(mtc_dedup* const* hs, const uint8_t* const* datas, const uint64_t* lens,
                            const int* baselines, uint32_t count, StepCtx& ctx, hipStream_t st, bool use_graph);
@@ -79,10 +79,13 @@ int mtc_create(mtc_dedup** out, uint32_t chunk_size, int approach, int hash_kind
       case MTC_APPROACH_TREE_LOW_ROOT_REF:
       case MTC_APPROACH_TREE_LOW_ROOT:
         break;
-      case MTC_APPROACH_FULL:
       case MTC_APPROACH_BASIC:
+        // Switched off: the basic checkpoint's device path has not passed its parity
+        // test yet (mtc_restart_basic, which reads basic diffs, is available).
+        throw Status{MTC_ERR_UNSUPPORTED_APPROACH, "basic-approach checkpoints are switched off in this build"};
+      case MTC_APPROACH_FULL:
       case MTC_APPROACH_LIST:
-        throw Status{MTC_ERR_UNSUPPORTED_APPROACH, "only the tree approach is implemented"};
+        throw Status{MTC_ERR_UNSUPPORTED_APPROACH, "the tree approach is implemented"};
       default:
         throw Status{MTC_ERR_INVALID_ARG, "unknown approach"};
     }
@@ -215,7 +218,17 @@ int mtc_restart(const void* const* diffs, const uint64_t* sizes, uint32_t count,
     if (!dev_out) throw Status{MTC_ERR_INVALID_ARG, "no output buffer"};
     use_device(device);
     mtc::restart_impl(diffs, sizes, count, chkpt_id, diffs_on_device != 0, static_cast<uint8_t*>(dev_out), out_len,
-                      static_cast<hipStream_t>(stream));
+                      static_cast<hipStream_t>(stream), false);
+  });
+}
+
+int mtc_restart_basic(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t chkpt_id,
+                int diffs_on_device, void* dev_out, uint64_t out_len, int device, void* stream) {
+  return guarded(nullptr, [&] {
+    if (!dev_out) throw Status{MTC_ERR_INVALID_ARG, "no output buffer"};
+    use_device(device);
+    mtc::restart_impl(diffs, sizes, count, chkpt_id, diffs_on_device != 0, static_cast<uint8_t*>(dev_out), out_len,
+                      static_cast<hipStream_t>(stream), true);
   });
 }
 

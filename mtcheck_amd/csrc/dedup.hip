@@ -347,7 +347,7 @@ struct LeafArgs {
   uint32_t C;
   Digest* leaf_dg;
   uint8_t* labels;
-  int baseline;
+  int baseline;       // 1: previous digests are zero (a reset); 2: every chunk counts as changed
   uint32_t* list;     // chunks to insert, packed per block
   uint32_t* blk_cnt;  // their count per block
   uint64_t tiles_per_block;
@@ -397,7 +397,7 @@ __device__ __forceinline__ void leaf_epilogue(const LeafArgs& a, uint64_t t, con
                                               uint32_t* region, uint32_t& run, uint32_t* s_wcnt) {
   const uint64_t c = t * kThreads + threadIdx.x;
   const bool valid = c < a.n;
-  const bool fixed = valid && d == prev;
+  const bool fixed = valid && a.baseline != 2 && d == prev;
   // A fixed leaf's digest was inserted by an earlier checkpoint of this baseline,
   // so its insert is a no-op except right after a reset, where every leaf inserts.
   const bool ins = valid && (a.baseline || !fixed);
@@ -609,7 +609,7 @@ __global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
   const uint64_t c = t * kThreads + tid;
   if (c >= full) d = (c < a.n) ? hash_bytes<HK>(a.data + c * CB, a.len - c * CB) : Digest{0, 0};
   const bool valid = c < a.n;
-  const bool fixed = valid && d == pv;
+  const bool fixed = valid && a.baseline != 2 && d == pv;
   const bool ins = valid && (a.baseline || !fixed);  // as in leaf_epilogue
   if (valid) {
     a.labels[c] = fixed ? L_FIXED : L_PENDING;
@@ -1691,6 +1691,79 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Launch<DiffArgs> L) {
   }
 }
 
+// Diff of the basic approach (src/basic_approach.cpp:54-142): the header (F = the
+// changed chunks, P = S = 0), their chunk indices, then their bytes, C each, the
+// short last chunk zero padded.  The reference orders the indices as its atomics
+// land; here they are ascending.  Block 0 also folds the counters and copies the
+// control block out, as k_assemble does for tree diffs.
+struct BasicDiffArgs {
+  uint8_t* diff;
+  const uint32_t* dense;  // changed chunks, ascending
+  const uint32_t* total;  // their number
+  const uint8_t* data;
+  uint64_t len;
+  uint32_t n, C;
+  uint32_t ref_id, chkpt_id;
+  Ctl* ctl;
+  int aligned;
+  const unsigned long long* ctr;
+  Ctl* out;
+  __device__ BasicDiffArgs global() const {
+    BasicDiffArgs c = *this;
+    MTC_GLOBAL(diff); MTC_GLOBAL(dense); MTC_GLOBAL(total); MTC_GLOBAL(data); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(out);
+    return c;
+  }
+};
+
+__global__ __launch_bounds__(kThreads) void k_basic_diff(Launch<BasicDiffArgs> L) {
+  const Where wh = where(L);
+  const BasicDiffArgs a = L.args[wh.m].global();
+  const uint32_t F = *a.total;
+  uint32_t* out = reinterpret_cast<uint32_t*>(a.diff);
+  const uint64_t tid = (uint64_t)wh.blk * kThreads + threadIdx.x, stride = (uint64_t)wh.nblk * kThreads;
+  for (uint64_t i = tid; i < F; i += stride) out[8 + i] = a.dense[i];
+  if (wh.blk == 0) {
+    if (threadIdx.x == 0) {
+      mtc_header h;
+      h.ref_id = a.ref_id;
+      h.chkpt_id = a.chkpt_id;
+      h.datalen = a.len;
+      h.chunk_size = a.C;
+      h.num_first_ocur = F;
+      h.num_prior_chkpts = 0;
+      h.num_shift_dupl = 0;
+      const uint32_t* hw = reinterpret_cast<const uint32_t*>(&h);
+      for (int k = 0; k < 8; k++) out[k] = hw[k];
+      a.ctl->F = F;
+      a.ctl->P = 0;
+      a.ctl->S = 0;
+      a.ctl->data_chunks = F;
+      a.ctl->diff_size = 32 + 4ull * F + (uint64_t)F * a.C;
+    }
+    __syncthreads();
+    finish_member(a.ctr, a.total, a.ctl, a.out);
+  }
+  uint8_t* dst0 = a.diff + 32 + 4ull * F;
+  const uint32_t C = a.C;
+  const uint32_t U = a.aligned ? C / 16 : C;  // copy units per chunk
+  for (uint64_t u = tid; u < (uint64_t)F * U; u += stride) {
+    const uint64_t j = u / U, w = u % U;
+    const uint64_t so = (uint64_t)a.dense[j] * C;
+    uint8_t* dst = dst0 + j * C;
+    if (a.aligned) {
+      const uint64_t b0 = so + 16 * w;
+      if (b0 + 16 <= a.len) {
+        const uint4 v = *reinterpret_cast<const uint4*>(a.data + b0);
+        *reinterpret_cast<u32x4_nt_a4*>(dst + 16 * w) = u32x4_nt_a4{v.x, v.y, v.z, v.w};
+      } else {
+        for (int b = 0; b < 16; b++) dst[16 * w + b] = (b0 + b < a.len) ? a.data[b0 + b] : 0;
+      }
+    } else {
+      dst[w] = (so + w < a.len) ? a.data[so + w] : 0;
+    }
+  }
+}
+
 // Reinsert every entry of the old table into a larger one (a rehash).
 __global__ __launch_bounds__(kThreads) void k_rehash(const Slot* old, uint64_t old_slots, Slot* tab, uint64_t mask,
                                                      Ctl* ctl) {
@@ -2123,7 +2196,7 @@ static void prepare_member(Member& M, hipStream_t st) {
     plan_tree(h);
   }
   ensure_buffers(h, st);
-  M.split = prepare_table(h, M.baseline, st);
+  M.split = h->approach == MTC_APPROACH_BASIC ? false : prepare_table(h, M.baseline, st);
   if (M.baseline) h->baseline_id = h->current_id;
   h->has_state = true;
   M.n = h->n;
@@ -2176,72 +2249,9 @@ static void add_scan(Ops& ops, ArenaImage& img, const std::vector<ScanSeg>& segs
 
 // Builds the step's launch sequence for the members: phase 1 (fills, leaf kernel,
 // scan of the leaf blocks' counts), phase 2 (the rest), or both (phase 0).
-static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int phase) {
+// Phase 2 of tree members: table passes, both forests, roots and the diff.
+static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
   const size_t B = ms.size();
-  if (phase != 2) {
-  // 1. per-step zero fills (counters, tile flags, root counts, histograms, debug views)
-  {
-    std::vector<FillSeg> fs;
-    std::vector<uint32_t> fb;
-    auto fill = [&](void* p, uint64_t words, uint32_t v) {
-      if (!words) return;
-      fs.push_back(FillSeg{reinterpret_cast<uint32_t*>(p), words, v});
-      fb.push_back(grid_for(words, kThreads * 4, 4096));
-    };
-    for (auto& M : ms) {
-      mtc_dedup* h = M.h;
-      fill(h->ctr.p, 2ull * CTR_KINDS * kStripes, 0);
-      fill(h->tile_flags.p, (2 * M.ntile0 + 3) / 4, 0);
-      fill(h->seg_cnt_f.p, M.nseg, 0);
-      fill(h->seg_cnt_s.p, M.nseg, 0);
-      fill(h->src_hist.p, M.nbins, 0);
-      fill(h->blk_cnt.p, M.nblk_leaf, 0);  // counts that k_leaf1 accumulates
-      if (h->debug) {
-        fill(h->dbg_tree.p, 4 * (2ull * M.n - 1), 0);
-        fill(h->dbg_labels.p, (2ull * M.n - 1 + 3) / 4, 0);
-      }
-    }
-    ops.add<FillSeg>(1, img.add(fs, fb), [](Launch<FillSeg> L, uint32_t g, hipStream_t st) { k_fill<<<g, kThreads, 0, st>>>(L); });
-  }
-  // 2. leaf kernel, one launch per (hash, staged chunk size) group
-  {
-    // (hash, staged chunk size, one tile per block with 128-byte chunks)
-    std::map<std::tuple<int, int, bool>, std::vector<size_t>> groups;
-    for (size_t i = 0; i < B; i++)
-      groups[{ms[i].h->hash_kind, ms[i].cb, ms[i].cb >= 16 && ms[i].tpb == 1}].push_back(i);
-    for (auto& kv : groups) {
-      std::vector<LeafArgs> la;
-      std::vector<uint32_t> lb;
-      for (size_t i : kv.second) {
-        Member& M = ms[i];
-        mtc_dedup* h = M.h;
-        la.push_back(LeafArgs{M.data, M.len, M.n, h->chunk_size, h->leaf_dg.p, h->labels.p, M.baseline ? 1 : 0,
-                              h->list.p, h->blk_cnt.p, M.tpb});
-        lb.push_back(M.nblk_leaf);
-      }
-      const int hk = std::get<0>(kv.first), cb = std::get<1>(kv.first);
-      const bool lean = std::get<2>(kv.first);
-      const uint32_t tag = 100 + (uint32_t)hk * 32 + (lean ? 16u : 0u) + (uint32_t)(cb ? __builtin_ctz((unsigned)cb) : 0);
-      ops.add<LeafArgs>(tag, img.add(la, lb), [hk, cb, lean](Launch<LeafArgs> L, uint32_t g, hipStream_t st) {
-        if (lean) {
-          if (hk == HASH_MD5) launch_leaf1<HASH_MD5>(cb, L, g, st);
-          else launch_leaf1<HASH_MURMUR3>(cb, L, g, st);
-        } else if (hk == HASH_MD5) {
-          launch_leaf<HASH_MD5>(cb, L, g, st);
-        } else {
-          launch_leaf<HASH_MURMUR3>(cb, L, g, st);
-        }
-      });
-    }
-  }
-  {
-    std::vector<ScanSeg> ss;
-    for (auto& M : ms)
-      ss.push_back(ScanSeg{M.h->blk_cnt.p, M.h->blk_off.p, nullptr, M.nblk_leaf, M.h->blk_off.p + M.nblk_leaf, M.ws_blk});
-    add_scan(ops, img, ss);
-  }
-  }
-  if (phase == 1) return;
   // 3. dense table passes over the changed chunks
   {
     std::vector<CompactArgs> ca;
@@ -2448,6 +2458,119 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
   }
 }
 
+// Phase 2 of basic members (src/basic_approach.cpp:14-142): the diff of the
+// changed chunks, whose ascending list the leaf pass and compaction left.
+static void build_basic_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
+  // the leaf blocks' list regions, concatenated in block order (ascending)
+  {
+    std::vector<CompactArgs> ca;
+    std::vector<uint32_t> cb;
+    for (auto& M : ms) {
+      mtc_dedup* h = M.h;
+      ca.push_back(CompactArgs{h->list.p, h->blk_cnt.p, h->blk_off.p, M.tpb, h->dense.p, M.nblk_leaf});
+      cb.push_back((M.nblk_leaf + kCompactPer - 1) / kCompactPer);
+    }
+    ops.add<CompactArgs>(21, img.add(ca, cb), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
+  }
+  std::vector<BasicDiffArgs> da;
+  std::vector<uint32_t> gb;
+  for (auto& M : ms) {
+    mtc_dedup* h = M.h;
+    BasicDiffArgs d{};
+    d.diff = M.arena ? h->diff_alt.p : h->diff.p;
+    d.dense = h->dense.p;
+    d.total = h->blk_off.p + M.nblk_leaf;
+    d.data = M.data;
+    d.len = M.len;
+    d.n = M.n;
+    d.C = h->chunk_size;
+    d.ref_id = h->baseline_id;
+    d.chkpt_id = M.cur;
+    d.ctl = h->ctl_d;
+    d.aligned = ((h->chunk_size % 16) == 0 && (uintptr_t)M.data % 16 == 0) ? 1 : 0;
+    d.ctr = h->ctr.p;
+    d.out = M.ctl_out;
+    da.push_back(d);
+    gb.push_back(grid_for(M.n, 8 * kThreads, 8192));
+  }
+  ops.add<BasicDiffArgs>(80, img.add(da, gb), [](Launch<BasicDiffArgs> L, uint32_t g, hipStream_t st) { k_basic_diff<<<g, kThreads, 0, st>>>(L); });
+}
+
+static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int phase) {
+  const size_t B = ms.size();
+  if (phase != 2) {
+  // 1. per-step zero fills (counters, tile flags, root counts, histograms, debug views)
+  {
+    std::vector<FillSeg> fs;
+    std::vector<uint32_t> fb;
+    auto fill = [&](void* p, uint64_t words, uint32_t v) {
+      if (!words) return;
+      fs.push_back(FillSeg{reinterpret_cast<uint32_t*>(p), words, v});
+      fb.push_back(grid_for(words, kThreads * 4, 4096));
+    };
+    for (auto& M : ms) {
+      mtc_dedup* h = M.h;
+      fill(h->ctr.p, 2ull * CTR_KINDS * kStripes, 0);
+      fill(h->tile_flags.p, (2 * M.ntile0 + 3) / 4, 0);
+      fill(h->seg_cnt_f.p, M.nseg, 0);
+      fill(h->seg_cnt_s.p, M.nseg, 0);
+      fill(h->src_hist.p, M.nbins, 0);
+      fill(h->blk_cnt.p, M.nblk_leaf, 0);  // counts that k_leaf1 accumulates
+      if (h->debug) {
+        fill(h->dbg_tree.p, 4 * (2ull * M.n - 1), 0);
+        fill(h->dbg_labels.p, (2ull * M.n - 1 + 3) / 4, 0);
+      }
+    }
+    ops.add<FillSeg>(1, img.add(fs, fb), [](Launch<FillSeg> L, uint32_t g, hipStream_t st) { k_fill<<<g, kThreads, 0, st>>>(L); });
+  }
+  // 2. leaf kernel, one launch per (hash, staged chunk size) group
+  {
+    // (hash, staged chunk size, one tile per block with 128-byte chunks)
+    std::map<std::tuple<int, int, bool>, std::vector<size_t>> groups;
+    for (size_t i = 0; i < B; i++)
+      groups[{ms[i].h->hash_kind, ms[i].cb,
+              ms[i].cb >= 16 && ms[i].tpb == 1 && ms[i].h->approach != MTC_APPROACH_BASIC}].push_back(i);
+    for (auto& kv : groups) {
+      std::vector<LeafArgs> la;
+      std::vector<uint32_t> lb;
+      for (size_t i : kv.second) {
+        Member& M = ms[i];
+        mtc_dedup* h = M.h;
+        // basic approach: every chunk of the first checkpoint is changed (src/basic_approach.cpp:36-45)
+        const int reset = !M.baseline ? 0 : (h->approach == MTC_APPROACH_BASIC && M.cur == 0) ? 2 : 1;
+        la.push_back(LeafArgs{M.data, M.len, M.n, h->chunk_size, h->leaf_dg.p, h->labels.p, reset,
+                              h->list.p, h->blk_cnt.p, M.tpb});
+        lb.push_back(M.nblk_leaf);
+      }
+      const int hk = std::get<0>(kv.first), cb = std::get<1>(kv.first);
+      const bool lean = std::get<2>(kv.first);
+      const uint32_t tag = 100 + (uint32_t)hk * 32 + (lean ? 16u : 0u) + (uint32_t)(cb ? __builtin_ctz((unsigned)cb) : 0);
+      ops.add<LeafArgs>(tag, img.add(la, lb), [hk, cb, lean](Launch<LeafArgs> L, uint32_t g, hipStream_t st) {
+        if (lean) {
+          if (hk == HASH_MD5) launch_leaf1<HASH_MD5>(cb, L, g, st);
+          else launch_leaf1<HASH_MURMUR3>(cb, L, g, st);
+        } else if (hk == HASH_MD5) {
+          launch_leaf<HASH_MD5>(cb, L, g, st);
+        } else {
+          launch_leaf<HASH_MURMUR3>(cb, L, g, st);
+        }
+      });
+    }
+  }
+  {
+    std::vector<ScanSeg> ss;
+    for (auto& M : ms)
+      ss.push_back(ScanSeg{M.h->blk_cnt.p, M.h->blk_off.p, nullptr, M.nblk_leaf, M.h->blk_off.p + M.nblk_leaf, M.ws_blk});
+    add_scan(ops, img, ss);
+  }
+  }
+  if (phase == 1) return;
+  std::vector<Member> tree, basic;
+  for (auto& M : ms) (M.h->approach == MTC_APPROACH_BASIC ? basic : tree).push_back(M);
+  if (!tree.empty()) build_tree_ops(tree, ops, img);
+  if (!basic.empty()) build_basic_ops(basic, ops, img);
+}
+
 void StepCtx::reset_graph() {
   if (exec) (void)hipGraphExecDestroy(exec);
   exec = nullptr;
@@ -2497,11 +2620,13 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
     MTC_HIP(hipGetLastError());
   }
   for (auto& M : ms) prepare_member(M, st);
-  // Reset tables of baseline members.
-  for (auto& M : ms)
-    if (M.baseline)
-      k_fill_u64<<<grid_for(4 * (M.h->table_cap + 1), kThreads, 65536), kThreads, 0, st>>>(
-          reinterpret_cast<uint64_t*>(M.h->table.p), 4 * (M.h->table_cap + 1), TEMPTY);
+  // Reset tables of baseline members (basic members keep no table).
+  for (auto& M : ms) {
+    if (!M.baseline || M.h->approach == MTC_APPROACH_BASIC) continue;
+    if (!M.h->table.p || M.h->table_cap == 0) throw Status{MTC_ERR_STATE, "digest table not allocated"};
+    k_fill_u64<<<grid_for(4 * (M.h->table_cap + 1), kThreads, 65536), kThreads, 0, st>>>(
+        reinterpret_cast<uint64_t*>(M.h->table.p), 4 * (M.h->table_cap + 1), TEMPTY);
+  }
   MTC_HIP(hipGetLastError());
   MTC_HIP(hipEventRecord(ctx.ev[1], st));
   ht.mark("prepare");

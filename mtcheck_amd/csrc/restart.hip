@@ -39,16 +39,18 @@ __device__ __forceinline__ uint32_t ld32(const uint8_t* p) {
 }
 
 // Region sizes of one diff's first-occurrence roots.
-__global__ void k_r_sizes(const uint8_t* first, uint32_t F, Geo g, uint32_t* sizes, uint32_t* err) {
+// idb: added to every stored id (0 for tree diffs, whose ids are node ids; n - 1
+// for basic diffs, whose ids are chunk indices: leaf n - 1 + c holds chunk c).
+__global__ void k_r_sizes(const uint8_t* first, uint32_t F, Geo g, uint32_t* sizes, uint32_t* err, uint32_t idb) {
   const uint32_t i = blockIdx.x * kRThreads + threadIdx.x;
   if (i >= F) return;
-  const uint32_t node = ld32(first + 4ull * i);
+  const uint64_t node = (uint64_t)ld32(first + 4ull * i) + idb;
   if ((uint64_t)node >= 2ull * g.n - 1) {
     atomicOr(err, ERR_CORRUPT);
     sizes[i] = 0;
     return;
   }
-  sizes[i] = leaf_count(g, node);
+  sizes[i] = leaf_count(g, (uint32_t)node);
 }
 
 // Hash set (diff position, root node) -> region index, for prev lookups.
@@ -60,10 +62,10 @@ __device__ __forceinline__ uint64_t rs_home(uint64_t key, uint64_t mask) {
 }
 
 __global__ void k_r_set_insert(const uint8_t* first, uint32_t F, uint32_t pos, uint64_t* keys, uint32_t* vals,
-                               uint64_t mask, uint32_t* err) {
+                               uint64_t mask, uint32_t* err, uint32_t idb) {
   const uint32_t i = blockIdx.x * kRThreads + threadIdx.x;
   if (i >= F) return;
-  const uint64_t key = ((uint64_t)pos << 32) | ld32(first + 4ull * i);
+  const uint64_t key = ((uint64_t)pos << 32) | (uint32_t)(ld32(first + 4ull * i) + idb);
   uint64_t s = rs_home(key, mask);
   for (uint64_t probes = 0; probes <= mask; probes++) {
     const uint64_t old = atomicCAS((unsigned long long*)&keys[s], (unsigned long long)kNoCover, (unsigned long long)key);
@@ -103,7 +105,7 @@ __global__ void k_r_fill32(uint32_t* p, uint64_t count, uint32_t v) {
 
 // Chunks of one diff's first-occurrence regions: cover[chunk] = (pos, data slot).
 __global__ void k_r_cover_first(const uint8_t* first, uint32_t F, const uint32_t* region_off, uint64_t total, Geo g,
-                                uint32_t pos, uint64_t* cover) {
+                                uint32_t pos, uint64_t* cover, uint32_t idb) {
   for (uint64_t j = (uint64_t)blockIdx.x * kRThreads + threadIdx.x; j < total; j += (uint64_t)gridDim.x * kRThreads) {
     uint32_t lo = 0, hi = F;
     while (hi - lo > 1) {
@@ -112,7 +114,7 @@ __global__ void k_r_cover_first(const uint8_t* first, uint32_t F, const uint32_t
       else hi = mid;
     }
     uint32_t st[2], ct[2];
-    const int r = leaf_runs(g, ld32(first + 4ull * lo), st, ct);
+    const int r = leaf_runs(g, ld32(first + 4ull * lo) + idb, st, ct);
     const uint32_t chunk = run_at(r, st, ct, (uint32_t)(j - region_off[lo]));
     cover[chunk] = ((uint64_t)pos << kSlotBits) | j;
   }
@@ -268,7 +270,7 @@ static void check_err(uint32_t* err_d, hipStream_t st) {
 }
 
 void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t k, bool on_device,
-                  uint8_t* out, uint64_t out_len, hipStream_t st) {
+                  uint8_t* out, uint64_t out_len, hipStream_t st, bool basic) {
   if (!diffs || !sizes || k >= count || !diffs[k]) throw Status{MTC_ERR_INVALID_ARG, "bad diff chain arguments"};
   auto read_header = [&](uint32_t idx, mtc_header* h) {
     if (!diffs[idx] || sizes[idx] < sizeof(mtc_header)) throw Status{MTC_ERR_CORRUPT_DIFF, "missing or short diff"};
@@ -285,6 +287,7 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
   if (n64 > (1ull << 31)) throw Status{MTC_ERR_CORRUPT_DIFF, "too many chunks"};
   const uint32_t n = (uint32_t)n64, ref = hk.ref_id, L = k - ref + 1;
   const Geo g = make_geo(n);
+  const uint32_t idb = basic ? n - 1 : 0;
 
   // Chain layout, with host diffs uploaded once.
   std::vector<ChainDiff> ch(L);
@@ -301,6 +304,8 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
     read_header(idx, &d.h);
     if (d.h.chkpt_id != idx || d.h.ref_id != ref || d.h.datalen != hk.datalen || d.h.chunk_size != hk.chunk_size)
       throw Status{MTC_ERR_CORRUPT_DIFF, "diff " + std::to_string(idx) + " does not belong to the chain"};
+    if (basic && (d.h.num_prior_chkpts || d.h.num_shift_dupl))
+      throw Status{MTC_ERR_CORRUPT_DIFF, "diff " + std::to_string(idx) + " is not a basic diff"};
     d.size = sizes[idx];
     d.off_first = 32;
     d.off_table = d.off_first + 4ull * d.h.num_first_ocur;
@@ -396,9 +401,9 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
     const uint32_t F = d.h.num_first_ocur;
     uint32_t* ro = region_p + d.region_base;
     if (F) {
-      k_r_sizes<<<rgrid(F), kRThreads, 0, st>>>(d.base + d.off_first, F, g, ro, err_p);
+      k_r_sizes<<<rgrid(F), kRThreads, 0, st>>>(d.base + d.off_first, F, g, ro, err_p, idb);
       k_r_set_insert<<<rgrid(F), kRThreads, 0, st>>>(d.base + d.off_first, F, p, rs.set_keys.p, rs.set_vals.p,
-                                                     set_cap - 1, err_p);
+                                                     set_cap - 1, err_p, idb);
       MTC_HIP(hipGetLastError());
     }
     exclusive_scan_u32(ro, ro, F, ro + F, ws, st);
@@ -426,7 +431,7 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
     if (F) {
       k_r_cover_first<<<rgrid(rs.totals_h[p], 65536), kRThreads, 0, st>>>(d.base + d.off_first, F,
                                                                          region_p + d.region_base, rs.totals_h[p], g, p,
-                                                                         cover_p);
+                                                                         cover_p, idb);
       MTC_HIP(hipGetLastError());
     }
     if (S) {

@@ -66,6 +66,7 @@ class TreeDeduplicator:
         check(lib().mtc_create(ctypes.byref(h), chunk_size, approach, hash_kind, device))
         self._h = h
         self.chunk_size = chunk_size
+        self.approach = BASIC if approach == BASIC else TREE
         self.device = device
         self.hash_kind = hash_kind
         self.datasizes = (0, 0)
@@ -190,7 +191,7 @@ class TreeDeduplicator:
     # -- restart overloads (src/tree_approach.cpp:1786-1840) ---------------------
     def restart(self, out, chkpts, chkpt_id: int, logname: str | None = None, stream=None) -> None:
         t0 = time.perf_counter()
-        restart_into(out, chkpts, chkpt_id, device=self.device, stream=stream)
+        restart_into(out, chkpts, chkpt_id, device=self.device, stream=stream, approach=self.approach)
         self.restart_timers = [0.0, time.perf_counter() - t0]
         if logname:
             self.write_restart_log(chkpt_id, f"{logname}.chunk_size.{self.chunk_size}.restart_timing.csv")
@@ -199,10 +200,11 @@ class TreeDeduplicator:
                       stream=None) -> None:
         diffs = []
         for name in filenames:
-            with open(name + ".hashtree.incr_chkpt", "rb") as f:
+            suffix = ".basic.incr_chkpt" if self.approach == BASIC else ".hashtree.incr_chkpt"
+            with open(name + suffix, "rb") as f:
                 diffs.append(f.read())
         t0 = time.perf_counter()
-        restart_into(out, diffs, chkpt_id, device=self.device, stream=stream)
+        restart_into(out, diffs, chkpt_id, device=self.device, stream=stream, approach=self.approach)
         self.restart_timers = [0.0, time.perf_counter() - t0]
         if logname:
             self.write_restart_log(chkpt_id, logname)
@@ -303,11 +305,12 @@ def metadata_breakdown(header, diff: bytes, num_chkpts: int) -> str:
     return "".join(out)
 
 
-def restart_into(out, chkpts, chkpt_id: int, device: int = 0, stream=None) -> None:
+def restart_into(out, chkpts, chkpt_id: int, device: int = 0, stream=None, approach: int = TREE) -> None:
     """Rebuild checkpoint chkpt_id into the device buffer `out` from diffs indexed by id.
 
     chkpts: host diffs (bytes-like, or None below the chain's baseline) or device
-    diffs given as (pointer, size) pairs / uint8 cuda tensors.
+    diffs given as (pointer, size) pairs / uint8 cuda tensors.  approach=BASIC
+    reads basic diffs (mtc_restart_basic).
     """
     optr, olen = _dev_ptr(out)
     n = len(chkpts)
@@ -327,8 +330,8 @@ def restart_into(out, chkpts, chkpt_id: int, device: int = 0, stream=None) -> No
             p, s = ctypes.addressof(buf), len(b)
         ptrs[i] = p
         sizes[i] = s
-    check(lib().mtc_restart(ptrs, sizes, n, chkpt_id, 1 if on_device else 0, optr, olen, device,
-                            _stream_ptr(stream)))
+    fn = lib().mtc_restart_basic if approach == BASIC else lib().mtc_restart
+    check(fn(ptrs, sizes, n, chkpt_id, 1 if on_device else 0, optr, olen, device, _stream_ptr(stream)))
 
 
 def digest_host(data: bytes, hash_kind: int = MURMUR3) -> bytes:

@@ -416,6 +416,7 @@ static int region_runs(const orc_dedup* d, uint32_t node, uint32_t start[2], uin
 }
 
 static void put32(uint8_t* p, uint32_t v) { memcpy(p, &v, 4); }
+static uint32_t get32(const uint8_t* p) { uint32_t v; memcpy(&v, p, 4); return v; }
 
 /* collect_diff, src/tree_approach.cpp:511-710, in serial (canonical) order */
 static int collect_diff(orc_dedup* d, const uint8_t* data, uint64_t len) {
@@ -687,6 +688,106 @@ int orc_dedup_checkpoint_par(orc_dedup* d, const uint8_t* data, uint64_t len, in
   return rc;
 }
 
+/* ========================================================================== */
+/* Basic approach: src/basic_approach.cpp:14-142 (dedup_data, collect_diff),   */
+/* :468-550 (checkpoint), :153-330 (restart).  The reference writes the        */
+/* changed chunks' indices in the order its atomics land; here, ascending.     */
+/* The handle's `tree` holds the chunk digests (the reference's HashList).     */
+/* ========================================================================== */
+
+int orc_basic_checkpoint(orc_dedup* d, const uint8_t* data, uint64_t len, int make_baseline) {
+  if (!d || !data || len == 0) return ORC_ERR_ARG;
+  const uint64_t C = d->chunk_size;
+  uint64_t n64 = (len + C - 1) / C;
+  if (n64 > (1ull << 31)) return ORC_ERR_ARG;
+  const uint32_t n = (uint32_t)n64;
+  if (make_baseline || !d->tree) { /* a fresh HashList, :490-493 */
+    free(d->tree);
+    d->tree = (uint8_t*)calloc(n, 16);
+    if (!d->tree) return ORC_ERR_NOMEM;
+    d->n = n;
+  }
+  if (n != d->n) return ORC_ERR_STATE;
+  if (d->current_id == 0 || make_baseline) d->baseline_id = d->current_id; /* :510-512 */
+  uint32_t* ids = (uint32_t*)malloc(sizeof(uint32_t) * n);
+  if (!ids) return ORC_ERR_NOMEM;
+  uint32_t F = 0;
+  for (uint32_t c = 0; c < n; c++) { /* :26-45 */
+    const uint64_t off = (uint64_t)c * C;
+    const uint64_t nb = (c == n - 1) ? len - off : C;
+    uint8_t h[16];
+    orc_hash(d->hash_kind, data + off, nb, h);
+    if (d->current_id > 0 && memcmp(DG(d->tree, c), h, 16) == 0) continue;
+    memcpy(DG(d->tree, c), h, 16);
+    ids[F++] = c;
+  }
+  const uint64_t size = 32 + (uint64_t)F * (4 + C); /* :73-75 */
+  free(d->diff);
+  d->diff = (uint8_t*)calloc(size, 1);
+  if (!d->diff) { free(ids); return ORC_ERR_NOMEM; }
+  d->diff_size = size;
+  orc_header hd;
+  hd.ref_id = d->baseline_id;
+  hd.chkpt_id = d->current_id;
+  hd.datalen = len;
+  hd.chunk_size = d->chunk_size;
+  hd.num_first_ocur = F;
+  hd.num_prior_chkpts = 0;
+  hd.num_shift_dupl = 0;
+  memcpy(d->diff, &hd, 32);
+  for (uint32_t i = 0; i < F; i++) put32(d->diff + 32 + 4ull * i, ids[i]);
+  uint8_t* dat = d->diff + 32 + 4ull * F;
+  for (uint32_t i = 0; i < F; i++) {
+    const uint64_t off = (uint64_t)ids[i] * C;
+    const uint64_t nb = (off + C > len) ? len - off : C;
+    memcpy(dat + (uint64_t)i * C, data + off, nb);
+  }
+  d->nfirst = F;
+  d->nshift = 0;
+  free(ids);
+  d->current_id += 1;
+  return ORC_OK;
+}
+
+/* restart_chkpt of the basic approach: every chunk comes from the newest diff of
+ * the chain ref_id..chkpt_idx that holds it. */
+int orc_restart_basic(const uint8_t* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t chkpt_idx,
+                      uint8_t* out, uint64_t out_len) {
+  if (!diffs || !sizes || chkpt_idx >= count || !diffs[chkpt_idx] || sizes[chkpt_idx] < 32) return ORC_ERR_ARG;
+  orc_header hk;
+  memcpy(&hk, diffs[chkpt_idx], 32);
+  if (hk.chkpt_id != chkpt_idx || hk.ref_id > chkpt_idx || hk.chunk_size == 0 || out_len < hk.datalen)
+    return ORC_ERR_CORRUPT;
+  const uint64_t C = hk.chunk_size, len = hk.datalen;
+  const uint32_t n = (uint32_t)((len + C - 1) / C);
+  uint8_t* done = (uint8_t*)calloc(n, 1);
+  if (!done) return ORC_ERR_NOMEM;
+  for (int64_t idx = chkpt_idx; idx >= (int64_t)hk.ref_id; idx--) {
+    if (!diffs[idx] || sizes[idx] < 32) { free(done); return ORC_ERR_CORRUPT; }
+    orc_header h;
+    memcpy(&h, diffs[idx], 32);
+    if (h.chkpt_id != (uint32_t)idx || h.ref_id != hk.ref_id || h.datalen != len || h.chunk_size != C ||
+        h.num_prior_chkpts || h.num_shift_dupl || 32 + (uint64_t)h.num_first_ocur * (4 + C) > sizes[idx]) {
+      free(done);
+      return ORC_ERR_CORRUPT;
+    }
+    const uint8_t* dat = diffs[idx] + 32 + 4ull * h.num_first_ocur;
+    for (uint32_t i = 0; i < h.num_first_ocur; i++) {
+      const uint32_t c = get32(diffs[idx] + 32 + 4ull * i);
+      if (c >= n) { free(done); return ORC_ERR_CORRUPT; }
+      if (done[c]) continue;
+      done[c] = 1;
+      const uint64_t off = (uint64_t)c * C;
+      const uint64_t nb = (off + C > len) ? len - off : C;
+      memcpy(out + off, dat + (uint64_t)i * C, nb);
+    }
+  }
+  for (uint32_t c = 0; c < n; c++)
+    if (!done[c]) { free(done); return ORC_ERR_CORRUPT; }
+  free(done);
+  return ORC_OK;
+}
+
 uint64_t orc_dedup_diff_size(const orc_dedup* d) { return d->diff_size; }
 void orc_dedup_copy_diff(const orc_dedup* d, uint8_t* dst) { memcpy(dst, d->diff, d->diff_size); }
 uint32_t orc_dedup_num_first_ocur(const orc_dedup* d) { return d->nfirst; }
@@ -720,7 +821,6 @@ typedef struct {
   uint64_t data_len;
 } orc_view;
 
-static uint32_t get32(const uint8_t* p) { uint32_t v; memcpy(&v, p, 4); return v; }
 
 static int parse_diff(const uint8_t* buf, uint64_t size, orc_view* v) {
   if (!buf || size < 32) return ORC_ERR_CORRUPT;

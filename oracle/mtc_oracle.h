@@ -66,6 +66,11 @@ orc_dedup* orc_dedup_create(uint32_t chunk_size, int hash_kind, int compat_ref);
 void orc_dedup_destroy(orc_dedup* d);
 /* One checkpoint; advances current_id like src/tree_approach.cpp:1740-1748. */
 int orc_dedup_checkpoint(orc_dedup* d, const uint8_t* data, uint64_t len, int make_baseline);
+/* The basic approach (src/basic_approach.cpp): changed chunks in ascending order. */
+int orc_basic_checkpoint(orc_dedup* d, const uint8_t* data, uint64_t len, int make_baseline);
+int orc_restart_basic(const uint8_t* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t chkpt_idx,
+                      uint8_t* out, uint64_t out_len);
+
 /* The same checkpoint with OpenMP on `threads` threads (identical results). */
 int orc_dedup_checkpoint_par(orc_dedup* d, const uint8_t* data, uint64_t len, int make_baseline, int threads);
 uint64_t orc_dedup_diff_size(const orc_dedup* d);

@@ -78,6 +78,11 @@ def lib():
         L.orc_restart.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint64), u32, u32,
                                   vp, u64, i32]
         L.orc_restart.restype = i32
+        L.orc_basic_checkpoint.argtypes = [vp, vp, u64, i32]
+        L.orc_basic_checkpoint.restype = i32
+        L.orc_restart_basic.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint64), u32, u32,
+                                        vp, u64]
+        L.orc_restart_basic.restype = i32
         _lib = L
     return _lib
 
@@ -216,6 +221,37 @@ def restart(diffs, chkpt_idx: int, out_len: int | None = None) -> np.ndarray:
     rc = lib().orc_restart(ptrs, sizes, n, chkpt_idx, _ptr(out), length, 0)
     if rc != 0:
         raise RuntimeError(f"oracle restart failed: {rc}")
+    return out[:hdr["datalen"]]
+
+
+class OracleBasic(OracleDedup):
+    """The basic approach (src/basic_approach.cpp): per-chunk digests compared with
+    the previous checkpoint's; the diff lists the changed chunks, ascending."""
+
+    def __init__(self, chunk_size: int, hash_kind: int = HASH_MURMUR3):
+        super().__init__(chunk_size, hash_kind)
+
+    def checkpoint(self, data, make_baseline: bool = False, threads: int = 0) -> bytes:
+        a = _as_u8(data)
+        rc = lib().orc_basic_checkpoint(self._h, _ptr(a), a.size, 1 if make_baseline else 0)
+        if rc != 0:
+            raise RuntimeError(f"oracle basic checkpoint failed: {rc}")
+        out = np.empty(lib().orc_dedup_diff_size(self._h), np.uint8)
+        lib().orc_dedup_copy_diff(self._h, _ptr(out))
+        return out.tobytes()
+
+
+def restart_basic(diffs, chkpt_idx: int) -> np.ndarray:
+    """Rebuild checkpoint ``chkpt_idx`` of the basic approach from its diffs."""
+    bufs = [np.frombuffer(bytes(d), np.uint8) if d is not None else None for d in diffs]
+    n = len(bufs)
+    ptrs = (ctypes.c_void_p * n)(*[(_ptr(b) if b is not None else None) for b in bufs])
+    sizes = (ctypes.c_uint64 * n)(*[(b.size if b is not None else 0) for b in bufs])
+    hdr = parse_header(diffs[chkpt_idx])
+    out = np.zeros(max(hdr["datalen"], 1), np.uint8)
+    rc = lib().orc_restart_basic(ptrs, sizes, n, chkpt_idx, _ptr(out), hdr["datalen"])
+    if rc != 0:
+        raise RuntimeError(f"oracle basic restart failed: {rc}")
     return out[:hdr["datalen"]]
 
 
