@@ -244,6 +244,66 @@ class TreeDeduplicator {
   int device_;
 };
 
+// BasicDeduplicator's restart overloads (src/basic_approach.cpp:593-630) and its
+// restart log (:681-690) over mtc_restart_basic: restores chains of basic diffs,
+// such as the .basic.incr_chkpt files the reference writes.  It holds no
+// deduplicator handle; basic checkpoints themselves are not offered by this
+// library (mtc_create refuses MTC_APPROACH_BASIC).
+class BasicRestarter {
+ public:
+  explicit BasicRestarter(uint32_t chunk_size, int device = 0) : chunk_size(chunk_size), device_(device) {}
+
+  // restart(View, vector<HostMirror>&, logname, id), :593-614.
+  void restart(uint8_t* data_ptr, size_t len, const std::vector<std::vector<uint8_t>>& chkpts,
+               const std::string& logname, uint32_t chkpt_id, void* stream = nullptr) {
+    run(chkpts, data_ptr, len, chkpt_id, stream);
+    if (!logname.empty()) write_restart_log(chkpt_id, logname + ".chunk_size." + std::to_string(chunk_size) +
+                                                          ".restart_timing.csv");
+  }
+  // restart(View, vector<string>& names, logname, id), :616-630: reads
+  // <name>.basic.incr_chkpt for every name.
+  void restart(uint8_t* data_ptr, size_t len, const std::vector<std::string>& chkpt_filenames,
+               const std::string& logname, uint32_t chkpt_id, void* stream = nullptr) {
+    std::vector<std::vector<uint8_t>> diffs;
+    for (const auto& name : chkpt_filenames) {
+      std::ifstream f(name + ".basic.incr_chkpt", std::ios::binary | std::ios::ate);
+      if (!f) throw Error(MTC_ERR_INVALID_ARG, "cannot open " + name + ".basic.incr_chkpt");
+      std::vector<uint8_t> d((size_t)f.tellg());
+      f.seekg(0);
+      f.read(reinterpret_cast<char*>(d.data()), (std::streamsize)d.size());
+      diffs.push_back(std::move(d));
+    }
+    run(diffs, data_ptr, len, chkpt_id, stream);
+    if (!logname.empty()) write_restart_log(chkpt_id, logname);
+  }
+  // write_restart_log, :681-690.
+  void write_restart_log(uint32_t select_chkpt, const std::string& logname) {
+    std::ofstream f(logname, std::ios::app);
+    f << "Basic," << select_chkpt << "," << chunk_size << "," << restart_timers[0] << "," << restart_timers[1] << "\n";
+  }
+
+  const uint32_t chunk_size;
+  double restart_timers[2] = {0, 0};
+
+ private:
+  void run(const std::vector<std::vector<uint8_t>>& chkpts, uint8_t* out, size_t len, uint32_t chkpt_id,
+           void* stream) {
+    std::vector<const void*> ptrs;
+    std::vector<uint64_t> sizes;
+    for (const auto& c : chkpts) {
+      ptrs.push_back(c.empty() ? nullptr : c.data());
+      sizes.push_back(c.size());
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    check(mtc_restart_basic(ptrs.data(), sizes.data(), (uint32_t)ptrs.size(), chkpt_id, 0, out, len, device_,
+                            stream));
+    restart_timers[0] = 0.0;
+    restart_timers[1] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  int device_;
+};
+
 }  // namespace mtcheck
 
 #endif

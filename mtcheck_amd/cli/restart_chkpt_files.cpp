@@ -1,8 +1,9 @@
-// restart_chkpt_files -- rebuild a checkpoint from its .hashtree.incr_chkpt chain.
+// restart_chkpt_files -- rebuild a checkpoint from its chain of diffs.
 //
 // Usage (src/restart_chkpt_files.cpp:21-46):
 //   restart_chkpt_files restart_id num_chkpts num_tests chunk_size <mode> file0 file1 ...
-// The original files give the restored size; the diffs are <file>.hashtree.incr_chkpt.
+// The original files give the restored size; the diffs are <file>.hashtree.incr_chkpt
+// for the tree modes and <file>.basic.incr_chkpt for --run-basic-chkpt (:109-136).
 // Prints the digest of the restored buffer, as the reference's VERIFY_OUTPUT build
 // does (:192-194), and the digest of the original file beside it.
 #include <hip/hip_runtime_api.h>
@@ -44,8 +45,8 @@ int main(int argc, char** argv) {
   const uint32_t num_tests = (uint32_t)std::atoi(argv[3]);
   const uint32_t chunk_size = (uint32_t)std::atoi(argv[4]);
   const mtc_approach mode = mtcheck::get_mode(argc, argv);
-  if (mode == MTC_APPROACH_FULL || mode == MTC_APPROACH_BASIC || mode == MTC_APPROACH_LIST) {
-    std::fprintf(stderr, "ERROR: only the tree approaches are provided\n");
+  if (mode == MTC_APPROACH_FULL || mode == MTC_APPROACH_LIST) {
+    std::fprintf(stderr, "ERROR: only the tree and basic approaches are provided\n");
     return 2;
   }
   if ((uint32_t)argc < 6 + num_chkpts || restart_id >= num_chkpts) {
@@ -60,17 +61,23 @@ int main(int argc, char** argv) {
     }
     const std::vector<uint8_t> original = read_file(files[restart_id]);
     std::vector<std::vector<uint8_t>> chkpts;
-    for (const auto& f : files) chkpts.push_back(read_file(f + ".hashtree.incr_chkpt"));
+    const bool basic = mode == MTC_APPROACH_BASIC;
+    for (const auto& f : files) chkpts.push_back(read_file(f + (basic ? ".basic.incr_chkpt" : ".hashtree.incr_chkpt")));
     uint8_t* dev = nullptr;
     HIP_OK(hipMalloc(&dev, original.size() ? original.size() : 1));
     std::vector<uint8_t> restored(original.size());
     for (uint32_t t = 0; t < num_tests; t++) {
       HIP_OK(hipMemset(dev, 0, original.size()));
-      mtcheck::TreeDeduplicator dedup(chunk_size);
-      dedup.restart(dev, original.size(), chkpts, trimmed[restart_id], restart_id);
+      if (basic) {
+        mtcheck::BasicRestarter dedup(chunk_size);
+        dedup.restart(dev, original.size(), chkpts, trimmed[restart_id], restart_id);
+      } else {
+        mtcheck::TreeDeduplicator dedup(chunk_size);
+        dedup.restart(dev, original.size(), chkpts, trimmed[restart_id], restart_id);
+      }
       HIP_OK(hipMemcpy(restored.data(), dev, original.size(), hipMemcpyDeviceToHost));
-      std::cout << "Hashtree digest:       " << mtcheck::calculate_digest_host(restored.data(), restored.size())
-                << std::endl;
+      std::cout << (basic ? "Basic chkpt digest:    " : "Hashtree digest:       ")
+                << mtcheck::calculate_digest_host(restored.data(), restored.size()) << std::endl;
     }
     std::cout << "Original digest:       " << mtcheck::calculate_digest_host(original.data(), original.size())
               << std::endl;

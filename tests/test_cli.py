@@ -14,7 +14,7 @@ BIN = os.path.join(ROOT, "mtcheck_amd", "bin")
 def test_cpp_wrapper_compiles_with_gcc(tmp_path):
     src = tmp_path / "use.cpp"
     src.write_text('#include "mtcheck.hpp"\nint main(){ mtcheck::TreeDeduplicator* p = nullptr; (void)p; '
-                   'return mtcheck::get_mode(0, nullptr); }\n')
+                   'mtcheck::BasicRestarter* q = nullptr; (void)q; return mtcheck::get_mode(0, nullptr); }\n')
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
@@ -62,3 +62,31 @@ def test_file_commands_round_trip(tmp_path):
         lines = [l for l in r.stdout.splitlines() if "digest" in l]
         assert lines[0].split(":")[1].strip() == lines[1].split(":")[1].strip()
         assert (tmp_path / f"chk{rid}.dat.chunk_size.{C}.restart_timing.csv").exists()
+
+
+@pytest.mark.gpu
+def test_restart_command_basic_mode(tmp_path):
+    """restart_chkpt_files --run-basic-chkpt rebuilds checkpoints from <file>.basic.incr_chkpt
+    chains (written here by the oracle's basic approach) and logs 'Basic' restart rows."""
+    from mtcheck_amd.workload import CheckpointSeries
+    C, nbytes, k = 64, (1 << 19) + 5, 4
+    series = CheckpointSeries(nbytes, C, seed=8)
+    buf = np.zeros(nbytes, np.uint8)
+    orc = O.OracleBasic(C)
+    files = []
+    for i in range(k):
+        series.apply_host(buf, i)
+        path = tmp_path / f"chk{i}.dat"
+        path.write_bytes(buf.tobytes())
+        files.append(str(path))
+        (tmp_path / f"chk{i}.dat.basic.incr_chkpt").write_bytes(orc.checkpoint(buf, make_baseline=(i == 0)))
+    for rid in (0, k - 1):
+        print(f"restart_chkpt_files basic {rid}", flush=True)
+        r = subprocess.run([os.path.join(BIN, "restart_chkpt_files"), str(rid), str(k), "1", str(C),
+                            "--run-basic-chkpt", *files], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+        assert r.returncode == 0, r.stderr
+        lines = [l for l in r.stdout.splitlines() if "digest" in l]
+        assert lines[0].startswith("Basic chkpt digest")
+        assert lines[0].split(":")[1].strip() == lines[1].split(":")[1].strip()
+        log = (tmp_path / f"chk{rid}.dat.chunk_size.{C}.restart_timing.csv").read_text().splitlines()
+        assert log[-1].startswith(f"Basic,{rid},{C},")
