@@ -15,6 +15,18 @@ DEFAULT = (1, 0.125, 8)
 # checkpoint, and a sparser one costs memory for little speed.
 CANDIDATES = list(itertools.product((1, 2, 4, 8), (0.125,), (6, 7, 8)))
 MARGIN = 0.03  # a schedule replaces the default only when it is this much faster
+# Chunk sizes the leaf kernel stages through LDS.  For these, one tile per block
+# selects the single-tile leaf kernel, which is faster at full size than the
+# multi-tile one (64-byte chunks in a 23 GiB batch: 6.0 against 5.6 TB/s); a
+# 256 MiB sample can time the multi-tile kernel a few percent ahead by noise, so
+# only the fuse depth is tuned for them.
+STAGED = {16, 32, 64, 128, 256, 512, 1024, 2048, 4096}
+
+
+def candidates_for(chunk_size: int):
+    if chunk_size in STAGED:
+        return [t for t in CANDIDATES if t[0] == 1]
+    return CANDIDATES
 
 
 def time_schedule(series: CheckpointSeries, bufs, hash_kind: int, tuning, reps: int = 2) -> float:
@@ -50,7 +62,7 @@ def autotune(chunk_sizes, hash_kind: int = MURMUR3, sample_bytes: int = 256 << 2
                 v.copy_(bufs[-1])
             series.apply_device(v, k)
             bufs.append(v)
-        timings = {t: time_schedule(series, bufs, hash_kind, t) for t in (candidates or CANDIDATES)}
+        timings = {t: time_schedule(series, bufs, hash_kind, t) for t in (candidates or candidates_for(C))}
         timings[DEFAULT] = min(timings.get(DEFAULT, float("inf")), time_schedule(series, bufs, hash_kind, DEFAULT))
         best = min(timings, key=timings.get)
         out[C] = best if timings[best] < (1 - MARGIN) * timings[DEFAULT] else DEFAULT

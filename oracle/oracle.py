@@ -141,15 +141,19 @@ class OracleDedup:
     """Serial tree deduplicator (the reference's lowest-offset rule)."""
 
     def __init__(self, chunk_size: int, hash_kind: int = HASH_MURMUR3, compat_ref: bool = False):
+        self._destroy = lib().orc_dedup_destroy
         self._h = lib().orc_dedup_create(chunk_size, hash_kind, 1 if compat_ref else 0)
         if not self._h:
             raise ValueError("bad chunk size")
         self.chunk_size = chunk_size
 
     def __del__(self):
+        # the destroy function is held by the instance: module globals may
+        # already be cleared when an instance dies at interpreter exit
         h = getattr(self, "_h", None)
-        if h:
-            lib().orc_dedup_destroy(h)
+        destroy = getattr(self, "_destroy", None)
+        if h and destroy:
+            destroy(h)
             self._h = None
 
     def checkpoint(self, data, make_baseline: bool = False, threads: int = 0) -> bytes:
