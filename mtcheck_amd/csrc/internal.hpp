@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include <string>
 #include <vector>
 
@@ -55,6 +57,12 @@ struct DevBuf {
     release();
     MTC_HIP(hipMalloc((void**)&p, (want ? want : 1) * sizeof(T)));
     count = want;
+  }
+  // As ensure, growing by at least half the current size: for small scratch whose
+  // need creeps up call after call (each reallocation synchronises the device).
+  void reserve(uint64_t want) {
+    if (want <= count) return;
+    ensure(std::max<uint64_t>(want, count + count / 2));
   }
   void release() {
     if (p) (void)hipFree(p);

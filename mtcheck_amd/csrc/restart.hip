@@ -241,12 +241,30 @@ struct RestartScratch {
   Workspace ws;
   DevBuf<uint32_t> err, region_off, set_vals;
   DevBuf<uint64_t> set_keys, cover, meta;
-  uint8_t* stage = nullptr;  // pinned
+  DevBuf<uint8_t> upload;    // host chains, uploaded once per restart
+  uint8_t* stage = nullptr;  // pinned: per-position tables for the kernels
   uint64_t stage_cap = 0;
-  uint32_t* totals = nullptr;  // pinned
-  uint32_t totals_cap = 0;
+  uint8_t* hstage = nullptr;  // pinned: headers and prior-checkpoint tables of device chains
+  uint64_t hstage_cap = 0;
+  uint8_t* totals = nullptr;  // pinned: every position's region total
+  uint64_t totals_cap = 0;
   std::vector<uint32_t> totals_h;
 };
+
+// Pinned staging that grows by at least half; the device is idle before the old
+// block is freed, since an earlier restart's copy may have used it.
+static void pinned_reserve(uint8_t*& p, uint64_t& cap, uint64_t want) {
+  if (want <= cap) return;
+  const uint64_t c = std::max<uint64_t>({want, cap + cap / 2, 4096});
+  if (p) {
+    MTC_HIP(hipDeviceSynchronize());
+    (void)hipHostFree(p);
+  }
+  p = nullptr;
+  cap = 0;
+  MTC_HIP(hipHostMalloc((void**)&p, c));
+  cap = c;
+}
 
 static std::mutex& scratch_mutex() {
   static std::mutex m;
@@ -272,13 +290,32 @@ static void check_err(uint32_t* err_d, hipStream_t st) {
 void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t k, bool on_device,
                   uint8_t* out, uint64_t out_len, hipStream_t st, bool basic) {
   if (!diffs || !sizes || k >= count || !diffs[k]) throw Status{MTC_ERR_INVALID_ARG, "bad diff chain arguments"};
-  auto read_header = [&](uint32_t idx, mtc_header* h) {
+  // Scratch buffers are kept per device between restarts (grow-only); one restart
+  // at a time uses them.
+  int dev = 0;
+  MTC_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> guard(scratch_mutex());
+  RestartScratch& rs = scratch_for(dev);
+  auto present = [&](uint32_t idx) {
     if (!diffs[idx] || sizes[idx] < sizeof(mtc_header)) throw Status{MTC_ERR_CORRUPT_DIFF, "missing or short diff"};
-    if (on_device) MTC_HIP(hipMemcpy(h, diffs[idx], sizeof(mtc_header), hipMemcpyDeviceToHost));
-    else std::memcpy(h, diffs[idx], sizeof(mtc_header));
+  };
+  // Headers of device chains come back through pinned staging, one
+  // synchronisation per batch of copies.
+  auto read_headers = [&](uint32_t first, uint32_t num, mtc_header* out) {
+    for (uint32_t i = 0; i < num; i++) present(first + i);
+    if (!on_device) {
+      for (uint32_t i = 0; i < num; i++) std::memcpy(out + i, diffs[first + i], sizeof(mtc_header));
+      return;
+    }
+    pinned_reserve(rs.hstage, rs.hstage_cap, (uint64_t)num * sizeof(mtc_header));
+    for (uint32_t i = 0; i < num; i++)
+      MTC_HIP(hipMemcpyAsync(rs.hstage + (uint64_t)i * sizeof(mtc_header), diffs[first + i], sizeof(mtc_header),
+                             hipMemcpyDeviceToHost, st));
+    MTC_HIP(hipStreamSynchronize(st));
+    std::memcpy(out, rs.hstage, (uint64_t)num * sizeof(mtc_header));
   };
   mtc_header hk;
-  read_header(k, &hk);
+  read_headers(k, 1, &hk);
   if (hk.chkpt_id != k || hk.ref_id > k || hk.chunk_size == 0 || hk.datalen == 0)
     throw Status{MTC_ERR_CORRUPT_DIFF, "bad header of the requested diff"};
   if (out_len < hk.datalen) throw Status{MTC_ERR_BUFFER_TOO_SMALL, "output shorter than datalen"};
@@ -291,17 +328,19 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
 
   // Chain layout, with host diffs uploaded once.
   std::vector<ChainDiff> ch(L);
-  DevBuf<uint8_t> upload;
+  {
+    std::vector<mtc_header> hs(L);
+    read_headers(ref, L, hs.data());
+    for (uint32_t p = 0; p < L; p++) ch[p].h = hs[p];
+  }
   uint64_t up_total = 0;
   if (!on_device)
     for (uint32_t p = 0; p < L; p++) up_total += (sizes[ref + p] + 255) & ~255ull;
-  if (!on_device) upload.ensure(up_total);
-  uint64_t up_off = 0, sumF = 0;
-  std::vector<std::vector<uint32_t>> ptables(L);
+  if (!on_device) rs.upload.reserve(up_total);
+  uint64_t up_off = 0, sumF = 0, pt_total = 0;
   for (uint32_t p = 0; p < L; p++) {
     const uint32_t idx = ref + p;
     ChainDiff& d = ch[p];
-    read_header(idx, &d.h);
     if (d.h.chkpt_id != idx || d.h.ref_id != ref || d.h.datalen != hk.datalen || d.h.chunk_size != hk.chunk_size)
       throw Status{MTC_ERR_CORRUPT_DIFF, "diff " + std::to_string(idx) + " does not belong to the chain"};
     if (basic && (d.h.num_prior_chkpts || d.h.num_shift_dupl))
@@ -316,32 +355,43 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
     if (on_device) {
       d.base = static_cast<const uint8_t*>(diffs[idx]);
     } else {
-      MTC_HIP(hipMemcpyAsync(upload.p + up_off, diffs[idx], d.size, hipMemcpyHostToDevice, st));
-      d.base = upload.p + up_off;
+      MTC_HIP(hipMemcpyAsync(rs.upload.p + up_off, diffs[idx], d.size, hipMemcpyHostToDevice, st));
+      d.base = rs.upload.p + up_off;
       up_off += (d.size + 255) & ~255ull;
-    }
-    ptables[p].resize(2ull * d.h.num_prior_chkpts);
-    if (d.h.num_prior_chkpts) {
-      if (on_device)
-        MTC_HIP(hipMemcpy(ptables[p].data(), d.base + d.off_table, 8ull * d.h.num_prior_chkpts, hipMemcpyDeviceToHost));
-      else
-        std::memcpy(ptables[p].data(), static_cast<const uint8_t*>(diffs[idx]) + d.off_table,
-                    8ull * d.h.num_prior_chkpts);
     }
     d.region_base = sumF + p;
     sumF += d.h.num_first_ocur;
+    pt_total += 8ull * d.h.num_prior_chkpts;
+  }
+  // The prior-checkpoint tables (in bounds: off_data <= size was checked above).
+  std::vector<std::vector<uint32_t>> ptables(L);
+  if (on_device && pt_total) pinned_reserve(rs.hstage, rs.hstage_cap, pt_total);
+  uint64_t pt_off = 0;
+  for (uint32_t p = 0; p < L; p++) {
+    const ChainDiff& d = ch[p];
+    const uint64_t bytes = 8ull * d.h.num_prior_chkpts;
+    ptables[p].resize(2ull * d.h.num_prior_chkpts);
+    if (!bytes) continue;
+    if (on_device)
+      MTC_HIP(hipMemcpyAsync(rs.hstage + pt_off, d.base + d.off_table, bytes, hipMemcpyDeviceToHost, st));
+    else
+      std::memcpy(ptables[p].data(), static_cast<const uint8_t*>(diffs[ref + p]) + d.off_table, bytes);
+    pt_off += bytes;
+  }
+  if (on_device && pt_total) {
+    MTC_HIP(hipStreamSynchronize(st));
+    pt_off = 0;
+    for (uint32_t p = 0; p < L; p++) {
+      const uint64_t bytes = 8ull * ch[p].h.num_prior_chkpts;
+      if (bytes) std::memcpy(ptables[p].data(), rs.hstage + pt_off, bytes);
+      pt_off += bytes;
+    }
   }
 
-  // Scratch buffers are kept per device between restarts (grow-only); one restart
-  // at a time uses them.
-  int dev = 0;
-  MTC_HIP(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> guard(scratch_mutex());
-  RestartScratch& rs = scratch_for(dev);
   Workspace& ws = rs.ws;
   rs.err.ensure(1);
   k_r_fill32<<<1, kRThreads, 0, st>>>(rs.err.p, 1, 0);
-  rs.region_off.ensure(sumF + L);
+  rs.region_off.reserve(sumF + L);
   uint64_t set_cap = 16;
   while (set_cap < 2 * sumF) set_cap <<= 1;
   rs.set_keys.ensure(set_cap);
@@ -381,13 +431,8 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
   }
   // meta and cum share one upload from pinned staging
   const uint64_t meta_bytes = 8 * meta_h.size(), cum_bytes = 4 * cum_h.size();
-  rs.meta.ensure(meta_h.size() + (cum_h.size() + 1) / 2);
-  if (rs.stage_cap < meta_bytes + cum_bytes) {
-    if (rs.stage) (void)hipHostFree(rs.stage);
-    rs.stage = nullptr;
-    MTC_HIP(hipHostMalloc((void**)&rs.stage, meta_bytes + cum_bytes));
-    rs.stage_cap = meta_bytes + cum_bytes;
-  }
+  rs.meta.reserve(meta_h.size() + (cum_h.size() + 1) / 2);
+  pinned_reserve(rs.stage, rs.stage_cap, meta_bytes + cum_bytes);
   MTC_HIP(hipStreamSynchronize(st));  // the staging may still feed the previous restart's copy
   std::memcpy(rs.stage, meta_h.data(), meta_bytes);
   std::memcpy(rs.stage + meta_bytes, cum_h.data(), cum_bytes);
@@ -409,18 +454,22 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
     exclusive_scan_u32(ro, ro, F, ro + F, ws, st);
   }
   rs.totals_h.resize(L);
-  if (rs.totals_cap < L) {
-    if (rs.totals) (void)hipHostFree(rs.totals);
-    rs.totals = nullptr;
-    MTC_HIP(hipHostMalloc((void**)&rs.totals, 4ull * L));
-    rs.totals_cap = L;
+  // Each position's total sits at region_base + F.  A short region array comes
+  // back in one copy; a long one total by total.
+  const uint64_t region_words = sumF + L;
+  const bool whole = region_words <= (1u << 20);
+  pinned_reserve(rs.totals, rs.totals_cap, 4ull * (whole ? region_words : L));
+  uint32_t* totals = reinterpret_cast<uint32_t*>(rs.totals);
+  if (whole) {
+    MTC_HIP(hipMemcpyAsync(totals, region_p, 4ull * region_words, hipMemcpyDeviceToHost, st));
+  } else {
+    for (uint32_t p = 0; p < L; p++)
+      MTC_HIP(hipMemcpyAsync(totals + p, region_p + ch[p].region_base + ch[p].h.num_first_ocur, sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, st));
   }
-  for (uint32_t p = 0; p < L; p++)
-    MTC_HIP(hipMemcpyAsync(rs.totals + p, region_p + ch[p].region_base + ch[p].h.num_first_ocur, sizeof(uint32_t),
-                           hipMemcpyDeviceToHost, st));
   check_err(err_p, st);
   for (uint32_t p = 0; p < L; p++) {
-    rs.totals_h[p] = rs.totals[p];
+    rs.totals_h[p] = whole ? totals[ch[p].region_base + ch[p].h.num_first_ocur] : totals[p];
     if (rs.totals_h[p] > ch[p].data_chunks) throw Status{MTC_ERR_CORRUPT_DIFF, "data section shorter than its regions"};
   }
 

@@ -73,7 +73,7 @@ static void scan_level(const uint32_t* in, uint32_t* out, uint64_t count, uint32
   }
   DevBuf<uint32_t>& sums = depth == 0 ? ws.u32a : (depth == 1 ? ws.u32b : ws.u32c);
   if (depth > 2) throw HipError{hipErrorInvalidValue, "scan depth"};
-  sums.ensure(tiles);
+  sums.reserve(tiles);
   k_scan_sums<<<(uint32_t)tiles, kScanThreads, 0, st>>>(in, count, sums.p);
   MTC_HIP(hipGetLastError());
   scan_level(sums.p, sums.p, tiles, total, ws, depth + 1, st);
