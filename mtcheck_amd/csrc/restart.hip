@@ -38,21 +38,6 @@ __device__ __forceinline__ uint32_t ld32(const uint8_t* p) {
   return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
 }
 
-// Region sizes of one diff's first-occurrence roots.
-// idb: added to every stored id (0 for tree diffs, whose ids are node ids; n - 1
-// for basic diffs, whose ids are chunk indices: leaf n - 1 + c holds chunk c).
-__global__ void k_r_sizes(const uint8_t* first, uint32_t F, Geo g, uint32_t* sizes, uint32_t* err, uint32_t idb) {
-  const uint32_t i = blockIdx.x * kRThreads + threadIdx.x;
-  if (i >= F) return;
-  const uint64_t node = (uint64_t)ld32(first + 4ull * i) + idb;
-  if ((uint64_t)node >= 2ull * g.n - 1) {
-    atomicOr(err, ERR_CORRUPT);
-    sizes[i] = 0;
-    return;
-  }
-  sizes[i] = leaf_count(g, (uint32_t)node);
-}
-
 // Hash set (diff position, root node) -> region index, for prev lookups.
 __device__ __forceinline__ uint64_t rs_home(uint64_t key, uint64_t mask) {
   key ^= key >> 33;
@@ -61,11 +46,22 @@ __device__ __forceinline__ uint64_t rs_home(uint64_t key, uint64_t mask) {
   return key & mask;
 }
 
-__global__ void k_r_set_insert(const uint8_t* first, uint32_t F, uint32_t pos, uint64_t* keys, uint32_t* vals,
-                               uint64_t mask, uint32_t* err, uint32_t idb) {
+// One diff's first-occurrence roots: the region size of each, and its entry in
+// the (diff position, root node) set that resolves shifted entries' sources.
+// idb: added to every stored id (0 for tree diffs, whose ids are node ids; n - 1
+// for basic diffs, whose ids are chunk indices: leaf n - 1 + c holds chunk c).
+__global__ void k_r_roots(const uint8_t* first, uint32_t F, Geo g, uint32_t* sizes, uint32_t pos, uint64_t* keys,
+                          uint32_t* vals, uint64_t mask, uint32_t* err, uint32_t idb) {
   const uint32_t i = blockIdx.x * kRThreads + threadIdx.x;
   if (i >= F) return;
-  const uint64_t key = ((uint64_t)pos << 32) | (uint32_t)(ld32(first + 4ull * i) + idb);
+  const uint64_t node = (uint64_t)ld32(first + 4ull * i) + idb;
+  if (node >= 2ull * g.n - 1) {
+    atomicOr(err, ERR_CORRUPT);
+    sizes[i] = 0;
+    return;
+  }
+  sizes[i] = leaf_count(g, (uint32_t)node);
+  const uint64_t key = ((uint64_t)pos << 32) | node;
   uint64_t s = rs_home(key, mask);
   for (uint64_t probes = 0; probes <= mask; probes++) {
     const uint64_t old = atomicCAS((unsigned long long*)&keys[s], (unsigned long long)kNoCover, (unsigned long long)key);
@@ -446,9 +442,8 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
     const uint32_t F = d.h.num_first_ocur;
     uint32_t* ro = region_p + d.region_base;
     if (F) {
-      k_r_sizes<<<rgrid(F), kRThreads, 0, st>>>(d.base + d.off_first, F, g, ro, err_p, idb);
-      k_r_set_insert<<<rgrid(F), kRThreads, 0, st>>>(d.base + d.off_first, F, p, rs.set_keys.p, rs.set_vals.p,
-                                                     set_cap - 1, err_p, idb);
+      k_r_roots<<<rgrid(F), kRThreads, 0, st>>>(d.base + d.off_first, F, g, ro, p, rs.set_keys.p, rs.set_vals.p,
+                                                set_cap - 1, err_p, idb);
       MTC_HIP(hipGetLastError());
     }
     exclusive_scan_u32(ro, ro, F, ro + F, ws, st);
