@@ -2580,6 +2580,7 @@ void StepCtx::reset_graph() {
 StepCtx::~StepCtx() {
   reset_graph();
   if (arena_h) (void)hipHostFree(arena_h);
+  if (lead_h) (void)hipHostFree(lead_h);
   if (pre_h) (void)hipHostFree(pre_h);
   if (ctl_h) (void)hipHostFree(ctl_h);
   if (cnt_h) (void)hipHostFree(cnt_h);
@@ -2636,20 +2637,21 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
   ctx.ctl_all.ensure(B);
   for (size_t i = 0; i < B; i++) ms[i].ctl_out = ctx.ctl_all.p + i;
   // Uploads the arena image and runs the launches, replayed from a graph or eagerly
-  // (eager runs time the leaf launches on their own: ev[1] .. ev[4]).
-  auto run_ops = [&](Ops& ops, ArenaImage& img, bool graph) {
+  // (the leaf launches always run eagerly, timed on their own: ev[1] .. ev[4]).
+  auto run_ops = [&](Ops& ops, ArenaImage& img, bool graph, DevBuf<uint8_t>& arena_d, uint8_t*& arena_h,
+                     uint64_t& arena_h_cap) {
     const uint64_t bytes = img.bytes.size();
-    const uint8_t* old_arena = ctx.arena_d.p;
-    ctx.arena_d.ensure(bytes);
-    if (ctx.arena_h_cap < bytes) {
-      if (ctx.arena_h) (void)hipHostFree(ctx.arena_h);
-      ctx.arena_h = nullptr;
-      MTC_HIP(hipHostMalloc((void**)&ctx.arena_h, bytes));
-      ctx.arena_h_cap = bytes;
+    const uint8_t* old_arena = arena_d.p;
+    arena_d.ensure(bytes);
+    if (arena_h_cap < bytes) {
+      if (arena_h) (void)hipHostFree(arena_h);
+      arena_h = nullptr;
+      MTC_HIP(hipHostMalloc((void**)&arena_h, bytes));
+      arena_h_cap = bytes;
     }
-    std::memcpy(ctx.arena_h, img.bytes.data(), bytes);
-    MTC_HIP(hipMemcpyAsync(ctx.arena_d.p, ctx.arena_h, bytes, hipMemcpyHostToDevice, st));
-    uint8_t* base = ctx.arena_d.p;
+    std::memcpy(arena_h, img.bytes.data(), bytes);
+    MTC_HIP(hipMemcpyAsync(arena_d.p, arena_h, bytes, hipMemcpyHostToDevice, st));
+    uint8_t* base = arena_d.p;
     if (graph) {
       ops.sig.push_back(reinterpret_cast<uint64_t>(base));
       if (!ctx.exec || ctx.sig != ops.sig || old_arena != base) {
@@ -2664,7 +2666,6 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
         ctx.sig = ops.sig;
       }
       MTC_HIP(hipGraphLaunch(ctx.exec, st));
-      MTC_HIP(hipEventRecord(ctx.ev[4], st));
       return;
     }
     for (size_t i = 0; i < ops.fn.size(); i++) {
@@ -2676,18 +2677,31 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
       if (i == 0 && (ops.sig[0] >> 32) == 1) MTC_HIP(hipEventRecord(ctx.ev[1], st));
     }
   };
-  if (!split) {
+  if (!split && use_graph && !debug) {
+    // The leading launches (fills, leaf kernels, scan of their counts) run at once,
+    // from their own arena; the rest is built while the leaf kernels stream and is
+    // replayed from a graph.
+    Ops ops1;
+    ArenaImage img1;
+    build_ops(ms, ops1, img1, 1);
+    run_ops(ops1, img1, false, ctx.lead_d, ctx.lead_h, ctx.lead_h_cap);
+    Ops ops2;
+    ArenaImage img2;
+    build_ops(ms, ops2, img2, 2);
+    ht.mark("build");
+    run_ops(ops2, img2, true, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
+  } else if (!split) {
     Ops ops;
     ArenaImage img;
     build_ops(ms, ops, img, 0);
     ht.mark("build");
-    run_ops(ops, img, use_graph && !debug);
+    run_ops(ops, img, false, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
   } else {
     // Phase 1, then the exact table bound of every member, then phase 2.
     Ops ops1;
     ArenaImage img1;
     build_ops(ms, ops1, img1, 1);
-    run_ops(ops1, img1, false);
+    run_ops(ops1, img1, false, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
     if (ctx.cnt_h_cap < B) {
       if (ctx.cnt_h) (void)hipHostFree(ctx.cnt_h);
       ctx.cnt_h = nullptr;
@@ -2704,7 +2718,7 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
     Ops ops2;
     ArenaImage img2;
     build_ops(ms, ops2, img2, 2);
-    run_ops(ops2, img2, false);
+    run_ops(ops2, img2, false, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
   }
   if (debug)
     for (auto& M : ms)
@@ -2750,7 +2764,7 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
     h->last_ms[1] = ms_total;
     h->last_ms[2] = 0;
     h->last_ms[3] = 0;
-    h->last_ms[4] = (use_graph && !debug) ? 0.f : ms_leaf;
+    h->last_ms[4] = ms_leaf;
   }
   ht.print(B);
 }

@@ -99,9 +99,11 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t count, uint3
 // uploaded argument arena, the step's result array and, when steps are replayed
 // from a graph, the captured graph and the launch shape it was captured for.
 struct StepCtx {
-  DevBuf<uint8_t> arena_d, pre_d;
+  DevBuf<uint8_t> arena_d, pre_d, lead_d;
   uint8_t* arena_h = nullptr;  // pinned staging of the arena
   uint64_t arena_h_cap = 0;
+  uint8_t* lead_h = nullptr;  // pinned staging of the leading launches' arena (graph steps)
+  uint64_t lead_h_cap = 0;
   uint8_t* pre_h = nullptr;
   uint64_t pre_h_cap = 0;
   DevBuf<Ctl> ctl_all;
