@@ -16,10 +16,10 @@ DEFAULT = (1, 0.125, 8)
 CANDIDATES = list(itertools.product((1, 2, 4, 8), (0.125,), (6, 7, 8)))
 MARGIN = 0.03  # a schedule replaces the default only when it is this much faster
 # Chunk sizes the leaf kernel stages through LDS.  For these, one tile per block
-# selects the single-tile leaf kernel, which is faster at full size than the
-# multi-tile one (64-byte chunks in a 23 GiB batch: 6.0 against 5.6 TB/s); a
-# 256 MiB sample can time the multi-tile kernel a few percent ahead by noise, so
-# only the fuse depth is tuned for them.
+# selects the single-tile leaf kernel; at full size one to eight tiles per block
+# run within 2% of each other (64-byte chunks, 8 GiB: 1.83-1.91 ms), below the
+# margin, so a 256 MiB sample would pick among them by noise.  Only the fuse
+# depth is tuned for them.
 STAGED = {16, 32, 64, 128, 256, 512, 1024, 2048, 4096}
 
 
