@@ -237,7 +237,6 @@ struct RestartScratch {
   Workspace ws;
   DevBuf<uint32_t> err, region_off, set_vals;
   DevBuf<uint64_t> set_keys, cover, meta;
-  DevBuf<uint8_t> upload;    // host chains, uploaded once per restart
   uint8_t* stage = nullptr;  // pinned: per-position tables for the kernels
   uint64_t stage_cap = 0;
   uint8_t* hstage = nullptr;  // pinned: headers and prior-checkpoint tables of device chains
@@ -332,7 +331,10 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
   uint64_t up_total = 0;
   if (!on_device)
     for (uint32_t p = 0; p < L; p++) up_total += (sizes[ref + p] + 255) & ~255ull;
-  if (!on_device) rs.upload.reserve(up_total);
+  // host chains: an exact-size upload, freed when the restart returns (a chain
+  // can be as large as the buffer, so it is not kept)
+  DevBuf<uint8_t> upload;
+  if (!on_device) upload.ensure(up_total);
   uint64_t up_off = 0, sumF = 0, pt_total = 0;
   for (uint32_t p = 0; p < L; p++) {
     const uint32_t idx = ref + p;
@@ -351,8 +353,8 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
     if (on_device) {
       d.base = static_cast<const uint8_t*>(diffs[idx]);
     } else {
-      MTC_HIP(hipMemcpyAsync(rs.upload.p + up_off, diffs[idx], d.size, hipMemcpyHostToDevice, st));
-      d.base = rs.upload.p + up_off;
+      MTC_HIP(hipMemcpyAsync(upload.p + up_off, diffs[idx], d.size, hipMemcpyHostToDevice, st));
+      d.base = upload.p + up_off;
       up_off += (d.size + 255) & ~255ull;
     }
     d.region_base = sumF + p;
