@@ -2061,8 +2061,14 @@ static void rehash_to(mtc_dedup* h, uint64_t cap, hipStream_t st) {
 // piece.  Otherwise the step pauses after the leaf kernel, when an exact bound is
 // known (twice the chunks to insert: every new internal node has two new leaves
 // below it), and grow_exact rehashes only if that bound does not fit.
-static bool prepare_table(mtc_dedup* h, bool reset, hipStream_t st) {
+// The next checkpoint runs in one piece in a table of `cap` slots: its worst case
+// (2n - 1 new entries) fits and the table is at most 80% full.
+static bool table_fits_step(const mtc_dedup* h, uint64_t cap) {
   const uint64_t nodes = 2ull * h->n - 1;
+  return h->table_size + nodes < cap && h->table_size <= cap / 10 * 8;
+}
+
+static bool prepare_table(mtc_dedup* h, bool reset, hipStream_t st) {
   const double lf = h->tune.load_factor > 0.05 ? h->tune.load_factor : 0.125;
   (void)st;
   if (reset) {
@@ -2071,7 +2077,24 @@ static bool prepare_table(mtc_dedup* h, bool reset, hipStream_t st) {
     h->table_cap = cap;
     h->table_size = 0;
   }
-  return !(h->table_size + nodes < h->table_cap && h->table_size <= h->table_cap / 10 * 8);
+  return !table_fits_step(h, h->table_cap);
+}
+
+// Grows, between two checkpoints of a handle, a table that the next checkpoint
+// could not run in without pausing after the leaf kernel (and perhaps rehashing
+// there): the serving loop calls it once a client's result is posted, so the
+// rehash falls outside its requests.  A rehash that reports a failed insert
+// leaves the handle to start a new baseline.
+void pregrow(mtc_dedup* h, hipStream_t st) {
+  if (h->approach == MTC_APPROACH_BASIC || !h->has_state || h->table_cap == 0) return;
+  if (table_fits_step(h, h->table_cap)) return;
+  uint64_t cap = h->table_cap;
+  while (!table_fits_step(h, cap)) cap *= 2;
+  rehash_to(h, cap, st);
+  uint32_t err = 0;
+  MTC_HIP(hipMemcpyAsync(&err, &h->ctl_d->err, sizeof(err), hipMemcpyDeviceToHost, st));
+  MTC_HIP(hipStreamSynchronize(st));
+  if (err) h->has_state = false;
 }
 
 static void grow_exact(mtc_dedup* h, uint64_t to_insert, hipStream_t st) {

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include <string>
 #include <vector>
@@ -141,7 +142,9 @@ struct mtc_dedup {
   bool has_state = false;
   uint64_t data_len = 0;
   uint32_t n = 0;
-  uint64_t table_cap = 0;   // power of two; slot index cap is the overflow slot
+  // power of two; slot index cap is the overflow slot.  Atomic: a serving lane may
+  // grow a client's table while the client reads its statistics.
+  std::atomic<uint64_t> table_cap{0};
   uint64_t table_size = 0;  // entries (host mirror, refreshed after every checkpoint)
   int debug = 0;
 

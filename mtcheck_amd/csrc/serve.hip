@@ -25,6 +25,7 @@ namespace mtc {
 void checkpoint_batch_impl(mtc_dedup* const* hs, const uint8_t* const* datas, const uint64_t* lens,
                            const int* baselines, uint32_t count, StepCtx& ctx, hipStream_t st, bool use_graph);
 uint64_t state_bytes(uint64_t n, uint32_t chunk_size, double load_factor);
+void pregrow(mtc_dedup* h, hipStream_t st);
 }  // namespace mtc
 
 namespace {
@@ -156,6 +157,7 @@ struct mtc_server {
         status = MTC_ERR_HIP;
         err = "unexpected exception";
       }
+      std::vector<uint32_t> served_clients;
       {
         std::lock_guard<std::mutex> lk(mu);
         const double t = now_s();
@@ -163,6 +165,7 @@ struct mtc_server {
         for (size_t i = 0; i < tickets.size(); i++) {
           Request& r = reqs[tickets[i]];
           Client& c = clients[r.client];
+          served_clients.push_back(r.client);
           r.t_done = t;
           r.done = true;
           r.status = status;
@@ -176,8 +179,23 @@ struct mtc_server {
           }
           if (r.ready) free_events.push_back(r.ready);
           r.ready = nullptr;
-          c.busy = false;
         }
+      }
+      cv_done.notify_all();
+      // Results are posted; the clients stay busy while a table that their next
+      // checkpoint would have to grow is grown here, outside their requests.  A
+      // failure leaves the growth to that checkpoint.
+      if (status == MTC_OK)
+        for (mtc_dedup* h : hs) {
+          try {
+            mtc::pregrow(h, lane->stream);
+          } catch (...) {
+          }
+        }
+      {
+        // (a posted request may already have been collected and erased)
+        std::lock_guard<std::mutex> lk(mu);
+        for (uint32_t c : served_clients) clients[c].busy = false;
       }
       cv_done.notify_all();
       cv_work.notify_all();

@@ -73,3 +73,24 @@ def test_queue_orders_each_clients_checkpoints():
     ids = [srv.result(t)[0].chkpt_id for t in tickets]
     assert ids == [0, 1, 2, 3]
     srv.close()
+
+
+def test_tables_grow_between_checkpoints():
+    """A client whose table fills is grown by its lane after its result is posted;
+    its diffs stay equal to a serial handle's, which grows inside its checkpoints."""
+    nb = 64 << 10  # 512 chunks: a 4096-slot table, full after about seven steps at 40% change
+    series = CheckpointSeries.with_change(nb, 128, 0.4, seed=77)
+    srv = Server(lanes=2, max_batch=4)
+    c = srv.add_client(128, nb)
+    serial = M.TreeDeduplicator(128)
+    buf = torch.empty(nb, dtype=torch.uint8, device="cuda:0")
+    for k in range(14):
+        say(f"serve growth step {k}")
+        series.apply_device(buf, k)
+        torch.cuda.synchronize()
+        srv.result(srv.submit(c, buf, make_baseline=(k == 0)))
+        got = srv.client(c).copy_last_diff()
+        assert got == serial.checkpoint(buf, make_baseline=(k == 0)), k
+    assert srv.client(c).stats().table_capacity > 4096
+    assert serial.stats().table_capacity > 4096
+    srv.close()
