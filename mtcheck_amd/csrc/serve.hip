@@ -25,6 +25,7 @@ namespace mtc {
 void checkpoint_batch_impl(mtc_dedup* const* hs, const uint8_t* const* datas, const uint64_t* lens,
                            const int* baselines, uint32_t count, StepCtx& ctx, hipStream_t st, bool use_graph);
 uint64_t state_bytes(uint64_t n, uint32_t chunk_size, double load_factor);
+uint64_t table_slots(uint64_t n, double load_factor);
 void pregrow(mtc_dedup* h, hipStream_t st);
 }  // namespace mtc
 
@@ -51,6 +52,8 @@ struct Request {
 struct Client {
   mtc_dedup* h = nullptr;
   uint64_t state_est = 0;
+  uint64_t table_slots = 0;  // table slots counted in state_est
+  uint64_t table_extra = 0;  // bytes of table grown beyond them (counted while resident)
   bool resident = false;
   bool busy = false;
   bool removed = false;
@@ -195,7 +198,15 @@ struct mtc_server {
       {
         // (a posted request may already have been collected and erased)
         std::lock_guard<std::mutex> lk(mu);
-        for (uint32_t c : served_clients) clients[c].busy = false;
+        for (uint32_t ci : served_clients) {
+          Client& c = clients[ci];
+          // grown tables count against the budget
+          const uint64_t slots = c.h->table.count;
+          const uint64_t extra = slots > c.table_slots ? 32 * (slots - c.table_slots) : 0;
+          if (c.resident) resident_bytes = resident_bytes - c.table_extra + extra;
+          c.table_extra = extra;
+          c.busy = false;
+        }
       }
       cv_done.notify_all();
       cv_work.notify_all();
@@ -207,10 +218,18 @@ namespace mtc {
 
 // Device bytes a deduplicator of n chunks holds (the buffers of ensure_buffers,
 // the table at the given load factor and the diff arena).
-uint64_t state_bytes(uint64_t n, uint32_t chunk_size, double load_factor) {
+// Slots of a baseline's digest table (without the overflow slot), as the
+// deduplicator sizes it.
+uint64_t table_slots(uint64_t n, double load_factor) {
+  const double lf = load_factor > 0.05 ? load_factor : 0.125;
+  const uint64_t want = std::max<uint64_t>((uint64_t)((double)n / lf), 1024);
   uint64_t cap = 1;
-  while ((double)cap < std::max<double>((double)n / load_factor, 1024.0)) cap <<= 1;
-  const uint64_t table = 32 * (cap + 1);
+  while (cap < want) cap <<= 1;
+  return cap;
+}
+
+uint64_t state_bytes(uint64_t n, uint32_t chunk_size, double load_factor) {
+  const uint64_t table = 32 * (table_slots(n, load_factor) + 1);
   const uint64_t per_chunk = 16 + 1 + 4 + 4 + 4 + 8 * 2 + 4 + 8 + 2 * (4 + 8) + 4 + 12 + 1;
   return table + per_chunk * n + (uint64_t)chunk_size * n + (8u << 20);
 }
@@ -312,6 +331,7 @@ int mtc_server_add_client(mtc_server* s, uint32_t chunk_size, int hash_kind, uin
   c.h = h;
   const uint64_t n = (max_len + chunk_size - 1) / chunk_size;
   c.state_est = mtc::state_bytes(n, chunk_size, h->tune.load_factor);
+  c.table_slots = mtc::table_slots(n, h->tune.load_factor) + 1;
   s->clients.push_back(c);
   *client = (uint32_t)(s->clients.size() - 1);
   return MTC_OK;
@@ -338,7 +358,7 @@ int mtc_server_remove_client(mtc_server* s, uint32_t client) {
       return true;
     });
     Client& c = s->clients[client];
-    if (c.resident) s->resident_bytes -= c.state_est;
+    if (c.resident) s->resident_bytes -= c.state_est + c.table_extra;
     c.resident = false;
     c.removed = true;
     h = c.h;

@@ -91,6 +91,10 @@ def test_tables_grow_between_checkpoints():
         srv.result(srv.submit(c, buf, make_baseline=(k == 0)))
         got = srv.client(c).copy_last_diff()
         assert got == serial.checkpoint(buf, make_baseline=(k == 0)), k
+        if k == 1:
+            resident0 = srv.stats()["resident_bytes"]
     assert srv.client(c).stats().table_capacity > 4096
     assert serial.stats().table_capacity > 4096
+    # the grown table (at least 4096 more 32-byte slots) counts against the budget
+    assert srv.stats()["resident_bytes"] >= resident0 + 32 * 4096
     srv.close()
