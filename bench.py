@@ -256,7 +256,8 @@ def served_bench(args):
     total offered load; the server batches whatever is queued.  A generator thread
     (its own stream) produces a client's next version as soon as its previous
     checkpoint completes; latency runs from the scheduled arrival to completion
-    (queueing included)."""
+    (queueing included), and is also reported from the submit call to the diff
+    being ready."""
     import collections
     import heapq
     import queue
@@ -349,10 +350,13 @@ def served_bench(args):
                 time.sleep(0.0001)
         lat_ms = np.array(lat) * 1e3
         pm = np.array(parts) * 1e3
+        sub_ms = pm[:, 1] + pm[:, 2]  # submit -> diff ready
         grown = sum(srv.client(c).stats().table_capacity != c0 for c, c0 in zip(cids, cap0))
         levels.append({"offered_gbs": load, "achieved_gbs": round(done_bytes / (last_done - start) / 1e9, 2),
                        "requests": n_events, "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
                        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+                       "p50_submit_to_ready_ms": round(float(np.percentile(sub_ms, 50)), 3),
+                       "p99_submit_to_ready_ms": round(float(np.percentile(sub_ms, 99)), 3),
                        "p99_parts_ms": {k: round(float(np.percentile(pm[:, j], 99)), 3)
                                         for j, k in enumerate(("to_submit", "queued", "service"))},
                        "max_service_ms": round(float(pm[:, 2].max()), 3), "tables_grown": int(grown)})
