@@ -252,6 +252,30 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(Launch<ScanSeg> L) {
   }
 }
 
+// A scan whose every segment fits one tile (count_max <= kScanTile): one block per
+// segment scans it and writes its total, in one launch instead of three.
+__global__ __launch_bounds__(kThreads) void k_scan_one(Launch<ScanSeg> L) {
+  __shared__ uint32_t ws[kThreads / 64];
+  const Where w = where(L);
+  const ScanSeg s = L.args[w.m].global();
+  const uint32_t cnt = scan_count(s);
+  const uint32_t base = 8u * threadIdx.x;
+  uint32_t v[8], run = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 8; j++) {
+    v[j] = (base + j < cnt) ? s.in[base + j] : 0u;
+    const uint32_t x = v[j];
+    v[j] = run;
+    run += x;
+  }
+  uint32_t tot;
+  const uint32_t off = block_scan_excl(run, &tot, ws);
+#pragma unroll
+  for (uint32_t j = 0; j < 8; j++)
+    if (base + j < cnt) s.out[base + j] = off + v[j];
+  if (threadIdx.x == 0 && s.total) *s.total = tot;
+}
+
 // ---------------------------------------------------------------------------
 // LDS-staged tile hashing.  A 256-thread block hashes 256 consecutive chunks, one
 // per thread.  The chunks are brought in as segments of SEG bytes per chunk with
@@ -2263,6 +2287,10 @@ static void prepare_member(Member& M, hipStream_t st) {
 static void add_scan(Ops& ops, ArenaImage& img, const std::vector<ScanSeg>& segs) {
   if (segs.empty()) return;
   std::vector<uint32_t> tiles, ones(segs.size(), 1);
+  if (std::all_of(segs.begin(), segs.end(), [](const ScanSeg& s) { return s.count_max <= kScanTile; })) {
+    ops.add<ScanSeg>(13, img.add(segs, ones), [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_one<<<g, kThreads, 0, st>>>(L); });
+    return;
+  }
   for (const auto& s : segs) tiles.push_back(std::max<uint32_t>(1, std::min<uint32_t>(512, (s.count_max + kScanTile - 1) / kScanTile)));
   const LaunchRec rt = img.add(segs, tiles), r1 = img.add(segs, ones);
   ops.add<ScanSeg>(10, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_reduce<<<g, kThreads, 0, st>>>(L); });
