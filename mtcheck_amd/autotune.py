@@ -1,6 +1,7 @@
-"""Schedule autotuner: per (chunk size, hash) picks the leaf kernel's tiles per
-block, the table load factor and the sweep fuse depth by timing incremental
-checkpoints of a synthetic sample buffer on the device.  The choice never changes
+"""Schedule autotuner: per chunk size it picks the sweep fuse depth (and, for chunk
+sizes the leaf kernel does not stage through LDS, the leaf kernel's tiles per
+block) by timing incremental checkpoints of a synthetic sample buffer on the
+device.  The table load factor stays at its default.  The choice never changes
 results (diffs are identical for every schedule), only speed."""
 from __future__ import annotations
 
