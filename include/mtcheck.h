@@ -191,7 +191,10 @@ const char* mtc_batch_last_error(const mtc_batch* b);
  * A client's checkpoints run in submission order (the result equals its serial
  * sequence); the diff of a request stays in the client's handle
  * (mtc_server_client) until the client's next checkpoint runs.  mtc_server_submit
- * records an event on `stream`, so the buffer may still be in production there. */
+ * records an event on `stream`, so the buffer may still be in production there.
+ * After a result is posted, the lane may grow that client's digest table before
+ * taking its next checkpoint; reading the client's diff and statistics meanwhile
+ * is safe, and the grown bytes count against the memory budget. */
 typedef struct mtc_server mtc_server;
 int mtc_server_create(mtc_server** out, int device, uint64_t memory_budget, uint32_t lanes, uint32_t max_batch);
 int mtc_server_destroy(mtc_server* s);

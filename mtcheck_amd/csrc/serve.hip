@@ -9,7 +9,10 @@
 // them as one batched step.  Requests that arrive while lanes are busy join the
 // next step of whichever lane frees first, and steps of different lanes overlap on
 // the device.  A client's checkpoints run in submission order, so every result
-// equals the client's serial checkpoint sequence.
+// equals the client's serial checkpoint sequence.  After posting a step's results
+// a lane grows any of its clients' digest tables that the next checkpoint could
+// not run in without pausing, so that rehash falls outside requests; grown tables
+// count against the budget.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
