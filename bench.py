@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--sweep", action="store_true",
                    help="chunk sizes {64,128,512,4096} x {murmur3,md5} x change {0.1,2,20}%%, one JSON line each")
     p.add_argument("--no-batched", action="store_true", help="skip the batched multi-buffer measurement")
+    p.add_argument("--no-small-series", action="store_true", help="skip the 64 MiB ten-checkpoint series")
     p.add_argument("--only-batched", action="store_true", help="profiling aid: run the batched measurement alone")
     p.add_argument("--batch-count", type=int, default=256)
     p.add_argument("--batch-steps", type=int, default=10)
@@ -499,6 +500,33 @@ def sweep(args):
                                   "leaf_ms": r["roofline"]["avg_ms"], "diff_mib_mean": r["diff_mib_mean"]}), flush=True)
 
 
+def small_series():
+    """The 64 MiB, ten-checkpoint series (10% zero runs at checkpoint 0, then 2% fresh
+    runs and 1% copies per checkpoint, 128 B chunks, Murmur3): device throughput over
+    the nine incremental checkpoints, and the serial oracle's on the same series on
+    one host core."""
+    import numpy as np
+
+    from mtcheck_amd.workload import CheckpointSeries
+    from oracle import oracle as O
+    nbytes, C = 64 << 20, 128
+    r = plain_run(nbytes, C, "murmur3", 0.03, 9, 1)
+    series = CheckpointSeries.with_change(nbytes, C, 0.03)
+    orc = O.OracleDedup(C)
+    host = np.zeros(nbytes, np.uint8)
+    t = 0.0
+    for k in range(10):
+        series.apply_host(host, k)
+        t0 = time.perf_counter()
+        orc.checkpoint(host, make_baseline=(k == 0))
+        if k:
+            t += time.perf_counter() - t0
+    return {"workload": "64 MiB uint8, 128 B chunks, murmur3, 10 checkpoints: 10% zero runs at checkpoint 0, "
+                        "then 2% fresh runs and 1% copies per checkpoint; the 9 incremental ones timed",
+            "value": r["value"], "unit": "GB/s", "ms_per_step": r["ms_per_step"],
+            "leaf_frac": r["roofline"]["frac"], "cpu_serial_1core_gbs": round(nbytes * 9 / t / 1e9, 4)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -533,6 +561,8 @@ def main():
     if "host_path" in r:
         out["host_path"] = r["host_path"]
     hk = M.MD5 if args.hash == "md5" else M.MURMUR3
+    if not args.no_small_series:
+        out["small_series"] = small_series()
     if not args.no_batched:
         out["batched"] = batched_bench(args)
     if not args.no_restart:
