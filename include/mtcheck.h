@@ -34,7 +34,8 @@ typedef enum mtc_status {
 
 /* Approaches: the DedupMode enum, include/utils.hpp:39-49.  Every tree mode is
  * served by the tree deduplicator, as src/dedup_files.cpp:54-63 dispatches them;
- * Full, Basic and List return MTC_ERR_UNSUPPORTED_APPROACH. */
+ * Basic is the basic deduplicator; Full and List return
+ * MTC_ERR_UNSUPPORTED_APPROACH. */
 typedef enum mtc_approach {
   MTC_APPROACH_UNKNOWN = 0,
   MTC_APPROACH_FULL = 1,
@@ -84,9 +85,9 @@ typedef struct mtc_stats {
 
 /* TreeDeduplicator(uint32_t bytes_per_chunk), src/tree_approach.cpp:5-9, plus the
  * approach selection of src/utils.cpp:15-36.  `device` is a HIP device ordinal.
- * MTC_APPROACH_BASIC (BasicDeduplicator, src/basic_approach.cpp:468-549: the diff
- * lists every chunk whose digest changed since the previous checkpoint) is switched
- * off in this build and returns MTC_ERR_UNSUPPORTED_APPROACH. */
+ * MTC_APPROACH_BASIC gives BasicDeduplicator(bytes_per_chunk)
+ * (src/basic_approach.cpp:468-549): its diff lists every chunk whose digest changed
+ * since the previous checkpoint, ids ascending, then their bytes. */
 int mtc_create(mtc_dedup** out, uint32_t chunk_size, int approach, int hash_kind, int device);
 
 /* ~TreeDeduplicator, src/tree_approach.cpp (destructor). */
@@ -140,6 +141,11 @@ int mtc_restart(const void* const* diffs, const uint64_t* sizes, uint32_t count,
  * P = S = 0, the changed chunks' indices, then their bytes, C each). */
 int mtc_restart_basic(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t chkpt_id,
                       int diffs_on_device, void* dev_out, uint64_t out_len, int device, void* stream);
+
+/* The calling thread's last mtc_restart / mtc_restart_basic: the upload of a host
+ * chain and the whole restart, in milliseconds (restart_timers[0..1],
+ * src/tree_approach.cpp:755-758,1124-1127,1792-1793; 0 upload for device chains). */
+int mtc_restart_times(float* upload_ms, float* total_ms);
 
 /* calculate_digest_host, include/hash_functions.hpp:46-82: digest of a whole host
  * buffer with the selected hash (restart verification). */

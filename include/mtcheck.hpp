@@ -93,7 +93,7 @@ class TreeDeduplicator {
  public:
   explicit TreeDeduplicator(uint32_t bytes_per_chunk, int hash_kind = MTC_HASH_MURMUR3, int device = 0,
                             int approach = MTC_APPROACH_TREE)
-      : chunk_size(bytes_per_chunk), device_(device) {
+      : chunk_size(bytes_per_chunk), device_(device), approach_(approach) {
     check(mtc_create(&h_, bytes_per_chunk, approach, hash_kind, device));
   }
   ~TreeDeduplicator() { mtc_destroy(h_); }
@@ -233,22 +233,24 @@ class TreeDeduplicator {
   }
   void run_restart(const std::vector<const void*>& ptrs, const std::vector<uint64_t>& sizes, int on_device,
                    uint8_t* out, size_t len, uint32_t chkpt_id, void* stream) {
-    const auto t0 = std::chrono::steady_clock::now();
-    check(mtc_restart(ptrs.data(), sizes.data(), (uint32_t)ptrs.size(), chkpt_id, on_device, out, len, device_,
-                      stream));
-    restart_timers[0] = 0.0;
-    restart_timers[1] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    auto fn = approach_ == MTC_APPROACH_BASIC ? mtc_restart_basic : mtc_restart;
+    check(fn(ptrs.data(), sizes.data(), (uint32_t)ptrs.size(), chkpt_id, on_device, out, len, device_, stream));
+    float up = 0, total = 0;
+    mtc_restart_times(&up, &total);
+    restart_timers[0] = up / 1e3;
+    restart_timers[1] = total / 1e3;
   }
 
   mtc_dedup* h_ = nullptr;
   int device_;
+  int approach_;
 };
 
 // BasicDeduplicator's restart overloads (src/basic_approach.cpp:593-630) and its
 // restart log (:681-690) over mtc_restart_basic: restores chains of basic diffs,
 // such as the .basic.incr_chkpt files the reference writes.  It holds no
-// deduplicator handle; basic checkpoints themselves are not offered by this
-// library (mtc_create refuses MTC_APPROACH_BASIC).
+// deduplicator handle (basic checkpoints are taken by a TreeDeduplicator created
+// with MTC_APPROACH_BASIC, whose restart overloads call mtc_restart_basic too).
 class BasicRestarter {
  public:
   explicit BasicRestarter(uint32_t chunk_size, int device = 0) : chunk_size(chunk_size), device_(device) {}
@@ -294,11 +296,12 @@ class BasicRestarter {
       ptrs.push_back(c.empty() ? nullptr : c.data());
       sizes.push_back(c.size());
     }
-    const auto t0 = std::chrono::steady_clock::now();
     check(mtc_restart_basic(ptrs.data(), sizes.data(), (uint32_t)ptrs.size(), chkpt_id, 0, out, len, device_,
                             stream));
-    restart_timers[0] = 0.0;
-    restart_timers[1] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    float up = 0, total = 0;
+    mtc_restart_times(&up, &total);
+    restart_timers[0] = up / 1e3;
+    restart_timers[1] = total / 1e3;
   }
 
   int device_;

@@ -8,6 +8,7 @@ namespace mtc {
 void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_baseline, hipStream_t st);
 void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t k, bool on_device,
                   uint8_t* out, uint64_t out_len, hipStream_t st, bool basic);
+void last_restart_times(float* upload_ms, float* total_ms);
 void hash_chunks(const uint8_t* data, uint64_t len, uint32_t C, int hk, Digest* out, hipStream_t st);
 void checkpoint_batch_impl(mtc_dedup* const* hs, const uint8_t* const* datas, const uint64_t* lens,
                            const int* baselines, uint32_t count, StepCtx& ctx, hipStream_t st, bool use_graph);
@@ -78,11 +79,8 @@ int mtc_create(mtc_dedup** out, uint32_t chunk_size, int approach, int hash_kind
       case MTC_APPROACH_TREE_LOW_OFFSET:
       case MTC_APPROACH_TREE_LOW_ROOT_REF:
       case MTC_APPROACH_TREE_LOW_ROOT:
-        break;
       case MTC_APPROACH_BASIC:
-        // Switched off: the basic checkpoint's device path has not passed its parity
-        // test yet (mtc_restart_basic, which reads basic diffs, is available).
-        throw Status{MTC_ERR_UNSUPPORTED_APPROACH, "basic-approach checkpoints are switched off in this build"};
+        break;
       case MTC_APPROACH_FULL:
       case MTC_APPROACH_LIST:
         throw Status{MTC_ERR_UNSUPPORTED_APPROACH, "the tree approach is implemented"};
@@ -175,6 +173,10 @@ int mtc_copy_diff_async(mtc_dedup* h, void* host_dst, uint64_t capacity) {
       for (int i = 0; i < 2; i++) MTC_HIP(hipEventCreateWithFlags(&h->copy_done[i], hipEventDisableTiming));
     }
     h->async_copies = true;  // from the next checkpoint on, checkpoints alternate diff arenas
+    // The second arena is allocated here, before the copy is queued, while the
+    // device is idle (the checkpoint has returned), rather than inside the next
+    // checkpoint.
+    h->diff_alt.ensure(h->diff.count);
     // The diff is complete: mtc_checkpoint returns after its last kernel.
     MTC_HIP(hipMemcpyAsync(host_dst, h->diff_ptr(), h->last_diff_size, hipMemcpyDefault, h->copy_st));
     MTC_HIP(hipEventRecord(h->copy_done[h->diff_cur], h->copy_st));
@@ -230,6 +232,11 @@ int mtc_restart_basic(const void* const* diffs, const uint64_t* sizes, uint32_t 
     mtc::restart_impl(diffs, sizes, count, chkpt_id, diffs_on_device != 0, static_cast<uint8_t*>(dev_out), out_len,
                       static_cast<hipStream_t>(stream), true);
   });
+}
+
+int mtc_restart_times(float* upload_ms, float* total_ms) {
+  mtc::last_restart_times(upload_ms, total_ms);
+  return MTC_OK;
 }
 
 int mtc_digest_host(const void* host_data, uint64_t len, int hash_kind, uint8_t out[16]) {
@@ -335,6 +342,7 @@ int mtc_checkpoint_batch(mtc_batch* b, mtc_dedup* const* handles, const void* co
   if (!b || (count && (!handles || !dev_data || !lens))) return MTC_ERR_INVALID_ARG;
   int rc = guarded(nullptr, [&] {
     if (count == 0) return;
+    if (count > mtc::kMaxBatch) throw Status{MTC_ERR_INVALID_ARG, "more than 65536 members in one batch"};
     MTC_HIP(hipSetDevice(b->device));
     for (uint32_t i = 0; i < count; i++) {
       if (!handles[i] || !dev_data[i] || lens[i] == 0) throw Status{MTC_ERR_INVALID_ARG, "empty member in batch"};

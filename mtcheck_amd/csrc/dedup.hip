@@ -21,8 +21,9 @@
 #include <functional>
 #include <map>
 #include <tuple>
+#include <unordered_set>
 
-#include "internal.hpp"
+#include "args.hpp"
 
 namespace mtc {
 
@@ -43,26 +44,6 @@ constexpr uint32_t kTilesPerWaveA = MTC_TPW_A;  // stage-0 tiles per wavefront, 
 // ---------------------------------------------------------------------------
 // Batched launches
 // ---------------------------------------------------------------------------
-template <class A>
-struct Launch {
-  const A* args;           // one per member
-  const uint32_t* prefix;  // count + 1 block offsets
-  const uint16_t* group;   // member owning block 64*i (only when count > 1)
-  uint32_t count;
-};
-
-struct Where {
-  uint32_t m, blk, nblk;  // member, block within the member's share, share size
-};
-
-// Pointers inside argument blocks are loaded from memory, so the compiler cannot
-// tell that they address global memory and would emit generic (flat) accesses.
-// A round trip through the global address space lets it use global instructions.
-template <class T>
-__device__ __forceinline__ T* gl(T* p) {
-  return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)p;
-}
-#define MTC_GLOBAL(f) c.f = gl(c.f)
 
 // The member of this block: a table entry per 64 blocks, then a short forward walk
 // (members with no blocks are skipped by the walk).  All values are wave-uniform.
@@ -80,16 +61,6 @@ __device__ __forceinline__ Where where(const Launch<A>& L) {
 // ---------------------------------------------------------------------------
 // small utility kernels
 // ---------------------------------------------------------------------------
-struct FillSeg {
-  uint32_t* p;
-  uint64_t count;  // 32-bit words
-  uint32_t v;
-  __device__ FillSeg global() const {
-    FillSeg c = *this;
-    MTC_GLOBAL(p);
-    return c;
-  }
-};
 
 __global__ __launch_bounds__(kThreads) void k_fill(Launch<FillSeg> L) {
   const Where w = where(L);
@@ -175,19 +146,6 @@ __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t
 // sums, then the tiles with their offsets.  The element count may live on the
 // device (count_dev), bounded by count_max.
 // ---------------------------------------------------------------------------
-struct ScanSeg {
-  const uint32_t* in;
-  uint32_t* out;          // may equal in
-  const uint32_t* count_dev;
-  uint32_t count_max;
-  uint32_t* total;        // optional: sum of all elements
-  uint32_t* sums;         // ceil(count_max / kScanTile) words
-  __device__ ScanSeg global() const {
-    ScanSeg c = *this;
-    MTC_GLOBAL(in); MTC_GLOBAL(out); MTC_GLOBAL(count_dev); MTC_GLOBAL(total); MTC_GLOBAL(sums);
-    return c;
-  }
-};
 
 __device__ __forceinline__ uint32_t scan_count(const ScanSeg& s) {
   return s.count_dev ? min(*s.count_dev, s.count_max) : s.count_max;
@@ -364,23 +322,6 @@ __device__ __forceinline__ Digest staged_hash(const uint8_t* data, uint64_t firs
 // Leaf kernel: hash, fixed test, table insert (src/tree_approach.cpp:305-351 and
 // the serial loop of include/reference_impl.hpp:44-71).  One lane per chunk.
 // ---------------------------------------------------------------------------
-struct LeafArgs {
-  const uint8_t* data;
-  uint64_t len;
-  uint32_t n;
-  uint32_t C;
-  Digest* leaf_dg;
-  uint8_t* labels;
-  int baseline;       // 1: previous digests are zero (a reset); 2: every chunk counts as changed
-  uint32_t* list;     // chunks to insert, packed per block
-  uint32_t* blk_cnt;  // their count per block
-  uint64_t tiles_per_block;
-  __device__ LeafArgs global() const {
-    LeafArgs c = *this;
-    MTC_GLOBAL(data); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(labels); MTC_GLOBAL(list); MTC_GLOBAL(blk_cnt);
-    return c;
-  }
-};
 
 // Digest of chunk tile*256 + threadIdx.x (zero beyond n).  CB > 0 selects the
 // LDS-staged path for 16-byte-aligned data and chunk sizes that are multiples of
@@ -648,19 +589,6 @@ __global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
 }
 
 // Concatenate the per-block regions into the dense list of chunks to insert.
-struct CompactArgs {
-  const uint32_t* list;
-  const uint32_t* blk_cnt;
-  const uint32_t* blk_off;
-  uint64_t tiles_per_block;
-  uint32_t* dense;
-  uint32_t nblk;  // leaf blocks
-  __device__ CompactArgs global() const {
-    CompactArgs c = *this;
-    MTC_GLOBAL(list); MTC_GLOBAL(blk_cnt); MTC_GLOBAL(blk_off); MTC_GLOBAL(dense);
-    return c;
-  }
-};
 
 constexpr uint32_t kCompactPer = 32;  // leaf blocks per compaction block
 
@@ -693,29 +621,6 @@ __global__ __launch_bounds__(kThreads) void k_compact(Launch<CompactArgs> L) {
     dst[e] = a.list[(uint64_t)(b0 + lo) * a.tiles_per_block * kThreads + (e - s_pre[lo])];
   }
 }
-
-struct DenseArgs {
-  const uint32_t* dense;  // chunks to insert, ascending within each wavefront's run
-  const uint32_t* total;  // their number
-  uint32_t n;
-  const Digest* leaf_dg;
-  uint32_t* leaf_slot;
-  uint8_t* labels;
-  Slot* tab;
-  uint64_t mask;
-  uint32_t cur;
-  Ctl* ctl;
-  unsigned long long* ctr;
-  Geo g;
-  uint32_t R0;          // levels of the first sweep stage
-  uint8_t* tile_first;  // per stage-0 tile: holds a first-occurrence leaf
-  uint8_t* tile_busy;   // per stage-0 tile: holds a leaf that is not a fixed duplicate
-  __device__ DenseArgs global() const {
-    DenseArgs c = *this;
-    MTC_GLOBAL(dense); MTC_GLOBAL(total); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(labels); MTC_GLOBAL(tab); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(tile_first); MTC_GLOBAL(tile_busy);
-    return c;
-  }
-};
 
 // Stage-0 sweep tile of a chunk: bottom-level chunks by position, depth D-1 leaves
 // by their position one level up.
@@ -852,39 +757,6 @@ __global__ __launch_bounds__(kThreads) void k_hash_chunks(const uint8_t* data, u
 // level and, when n is not a power of two, the depth D-1 leaves); later stages
 // read the previous stage's frontier (one entry per tile root).
 // ---------------------------------------------------------------------------
-struct SweepArgs {
-  Geo g;
-  uint32_t cur;
-  uint32_t d_in, d_out;
-  int stage0;
-  const Digest* leaf_dg;
-  uint8_t* labels;
-  const uint32_t* leaf_slot;
-  const uint8_t* in_lab;
-  const Digest* in_dg;
-  const uint32_t* in_slot;
-  uint8_t* out_lab;
-  Digest* out_dg;
-  uint32_t* out_slot;
-  Slot* tab;
-  uint64_t mask;
-  Ctl* ctl;
-  unsigned long long* ctr;
-  uint64_t* staging;
-  uint32_t* cnt_f;
-  uint32_t* cnt_s;
-  const uint32_t* seg_base;
-  Digest* dbg_tree;
-  uint8_t* dbg_labels;
-  const uint8_t* tile_first;  // stage 0: tile holds a first-occurrence leaf
-  const uint8_t* tile_busy;   // stage 0: tile holds a leaf that is not a fixed duplicate
-  uint32_t tiles_per_wave;     // sweep A stage 0: consecutive tiles one wavefront walks (<= 64)
-  __device__ SweepArgs global() const {
-    SweepArgs c = *this;
-    MTC_GLOBAL(leaf_dg); MTC_GLOBAL(labels); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(in_lab); MTC_GLOBAL(in_dg); MTC_GLOBAL(in_slot); MTC_GLOBAL(out_lab); MTC_GLOBAL(out_dg); MTC_GLOBAL(out_slot); MTC_GLOBAL(tab); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(staging); MTC_GLOBAL(cnt_f); MTC_GLOBAL(cnt_s); MTC_GLOBAL(seg_base); MTC_GLOBAL(dbg_tree); MTC_GLOBAL(dbg_labels); MTC_GLOBAL(tile_first); MTC_GLOBAL(tile_busy);
-    return c;
-  }
-};
 
 __device__ __forceinline__ uint64_t bottom_chunk(const Geo& g, uint64_t p) { return (1ull << g.D) - g.n + p; }
 
@@ -1339,7 +1211,6 @@ __global__ __launch_bounds__(kThreads) void k_sweep_b(Launch<SweepArgs> L) {
   sweep_b_tile<HK>(a, tile, load_lab4(a, tile << R, T, lane), lane);
 }
 
-
 // A single chunk: the leaf is the root.
 __global__ void k_single_leaf(Launch<SweepArgs> L) {
   const Where w = where(L);
@@ -1355,18 +1226,6 @@ __global__ void k_single_leaf(Launch<SweepArgs> L) {
 // ---------------------------------------------------------------------------
 // Root compaction: one thread per (depth, block) segment, in segment order.
 // ---------------------------------------------------------------------------
-struct RootArgs {
-  const uint64_t* staging;
-  const uint32_t *cnt_f, *cnt_s, *off_f, *off_s, *seg_base;
-  uint32_t ndepth, nseg;
-  uint32_t* first_roots;
-  uint64_t* shift_tmp;
-  __device__ RootArgs global() const {
-    RootArgs c = *this;
-    MTC_GLOBAL(staging); MTC_GLOBAL(cnt_f); MTC_GLOBAL(cnt_s); MTC_GLOBAL(off_f); MTC_GLOBAL(off_s); MTC_GLOBAL(seg_base); MTC_GLOBAL(first_roots); MTC_GLOBAL(shift_tmp);
-    return c;
-  }
-};
 
 __global__ __launch_bounds__(kThreads) void k_scatter_roots(Launch<RootArgs> L) {
   const Where w = where(L);
@@ -1393,26 +1252,6 @@ __global__ __launch_bounds__(kThreads) void k_scatter_roots(Launch<RootArgs> L) 
 // (src/tree_approach.cpp:574-580,670-696), then a stable sort by source.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kLdsBins = 4096;  // source-checkpoint bins counted in LDS
-
-struct ShiftArgs {
-  const uint64_t* shift_tmp;
-  const uint32_t* S;
-  const uint32_t* leaf_slot;
-  uint32_t n;
-  const Slot* tab;
-  uint64_t mask;
-  uint32_t ref, cur;
-  uint32_t* key;
-  uint64_t* val;
-  uint32_t* src_hist;
-  uint32_t nbins;
-  Ctl* ctl;
-  __device__ ShiftArgs global() const {
-    ShiftArgs c = *this;
-    MTC_GLOBAL(shift_tmp); MTC_GLOBAL(S); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(tab); MTC_GLOBAL(key); MTC_GLOBAL(val); MTC_GLOBAL(src_hist); MTC_GLOBAL(ctl);
-    return c;
-  }
-};
 
 __global__ __launch_bounds__(kThreads) void k_shift_src(Launch<ShiftArgs> L) {
   __shared__ uint32_t h[kLdsBins];
@@ -1456,20 +1295,6 @@ __global__ __launch_bounds__(kThreads) void k_shift_src(Launch<ShiftArgs> L) {
 // contiguous range of the S entries (S read on the device), counts its digits,
 // and after a scan of the digit-major (digit, block) counts moves its entries in
 // order.
-struct SortArgs {
-  const uint32_t* key_in;
-  const uint64_t* val_in;
-  uint32_t* key_out;
-  uint64_t* val_out;
-  const uint32_t* S;
-  uint32_t shift, nb;  // digit shift and number of digit values (a power of two <= 256)
-  uint32_t* hist;      // nb * G counts, then their exclusive scan
-  __device__ SortArgs global() const {
-    SortArgs c = *this;
-    MTC_GLOBAL(key_in); MTC_GLOBAL(val_in); MTC_GLOBAL(key_out); MTC_GLOBAL(val_out); MTC_GLOBAL(S); MTC_GLOBAL(hist);
-    return c;
-  }
-};
 
 __device__ __forceinline__ void sort_range(uint32_t S, uint32_t g, uint32_t G, uint32_t& lo, uint32_t& hi) {
   const uint32_t per = ((S + G - 1) / G + kThreads - 1) / kThreads * kThreads;
@@ -1541,17 +1366,6 @@ __global__ __launch_bounds__(kThreads) void k_sort_move(Launch<SortArgs> L) {
 }
 
 // (chkpt, count) table in ascending checkpoint id (src/tree_approach.cpp:657-665).
-struct PtableArgs {
-  const uint32_t* src_hist;
-  uint32_t nbins, ref;
-  uint32_t* ptable;
-  Ctl* ctl;
-  __device__ PtableArgs global() const {
-    PtableArgs c = *this;
-    MTC_GLOBAL(src_hist); MTC_GLOBAL(ptable); MTC_GLOBAL(ctl);
-    return c;
-  }
-};
 
 // The (chkpt, count) table (src/tree_approach.cpp:657-665): the source checkpoints
 // with at least one shifted entry, ascending.  One whole block.
@@ -1572,20 +1386,6 @@ __device__ void build_ptable(const PtableArgs& a) {
   if (threadIdx.x == 0) a.ctl->P = base;
 }
 
-struct RegionArgs {
-  const uint32_t* first_roots;
-  const uint32_t* F;
-  Geo g;
-  uint32_t* sizes;
-  PtableArgs pt;  // the (chkpt, count) table, built by block 0
-  __device__ RegionArgs global() const {
-    RegionArgs c = *this;
-    MTC_GLOBAL(first_roots); MTC_GLOBAL(F); MTC_GLOBAL(sizes);
-    c.pt = pt.global();
-    return c;
-  }
-};
-
 __global__ __launch_bounds__(kThreads) void k_region_sizes(Launch<RegionArgs> L) {
   const Where w = where(L);
   const RegionArgs a = L.args[w.m].global();
@@ -1599,30 +1399,6 @@ __global__ __launch_bounds__(kThreads) void k_region_sizes(Launch<RegionArgs> L)
 // Diff assembly (src/tree_approach.cpp:593,622-706): header, F roots, P table,
 // S pairs, then the gathered chunks.
 // ---------------------------------------------------------------------------
-struct DiffArgs {
-  uint8_t* diff;
-  const uint32_t* first_roots;
-  const uint32_t* ptable;
-  const uint64_t* shift_sorted;  // (prev << 32 | node)
-  const uint32_t* region_off;
-  const uint32_t* total_chunks;
-  const uint8_t* data;
-  uint64_t len;
-  uint32_t n, C;
-  uint32_t ref_id, chkpt_id;
-  Geo g;
-  Ctl* ctl;
-  int aligned;
-  const unsigned long long* ctr;  // striped counters
-  const uint32_t* changed;        // chunks inserted by the leaf pass
-  Ctl* out;                       // the member's entry of the step's result array
-  __device__ DiffArgs global() const {
-    DiffArgs c = *this;
-    MTC_GLOBAL(diff); MTC_GLOBAL(first_roots); MTC_GLOBAL(ptable); MTC_GLOBAL(shift_sorted); MTC_GLOBAL(region_off); MTC_GLOBAL(total_chunks); MTC_GLOBAL(data); MTC_GLOBAL(ctl);
-    MTC_GLOBAL(ctr); MTC_GLOBAL(changed); MTC_GLOBAL(out);
-    return c;
-  }
-};
 
 __device__ void diff_meta(const DiffArgs& a, const Where& w) {
   const uint32_t F = a.ctl->F, P = a.ctl->P, S = a.ctl->S;
@@ -1653,7 +1429,6 @@ __device__ void diff_meta(const DiffArgs& a, const Where& w) {
     a.ctl->diff_size = data_off + chunks * a.C;
   }
 }
-
 
 // The diff in one launch: metadata (every block, grid-stride), then block 0 folds
 // the counters and copies the control block out (its diff size is final once
@@ -1720,24 +1495,6 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Launch<DiffArgs> L) {
 // short last chunk zero padded.  The reference orders the indices as its atomics
 // land; here they are ascending.  Block 0 also folds the counters and copies the
 // control block out, as k_assemble does for tree diffs.
-struct BasicDiffArgs {
-  uint8_t* diff;
-  const uint32_t* dense;  // changed chunks, ascending
-  const uint32_t* total;  // their number
-  const uint8_t* data;
-  uint64_t len;
-  uint32_t n, C;
-  uint32_t ref_id, chkpt_id;
-  Ctl* ctl;
-  int aligned;
-  const unsigned long long* ctr;
-  Ctl* out;
-  __device__ BasicDiffArgs global() const {
-    BasicDiffArgs c = *this;
-    MTC_GLOBAL(diff); MTC_GLOBAL(dense); MTC_GLOBAL(total); MTC_GLOBAL(data); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(out);
-    return c;
-  }
-};
 
 __global__ __launch_bounds__(kThreads) void k_basic_diff(Launch<BasicDiffArgs> L) {
   const Where wh = where(L);
@@ -2038,7 +1795,7 @@ static void ensure_buffers(mtc_dedup* h, hipStream_t st) {
 // MTC_REHASH_PROBE_LIMIT=k ends the in-window probe after k slots; both exist to
 // test the less used paths (results never depend on them).
 static void rehash_to(mtc_dedup* h, uint64_t cap, hipStream_t st) {
-  if (cap > (1ull << 31)) throw Status{MTC_ERR_TABLE_FULL, "table capacity above 2^31 slots"};
+  if (cap > kMaxTableSlots) throw Status{MTC_ERR_TABLE_FULL, "table capacity above 2^31 slots"};
   const uint64_t old_cap = h->table_cap;
   h->table_tmp.ensure(cap + 1);
   uint32_t cap_log = 0;
@@ -2092,11 +1849,26 @@ static bool table_fits_step(const mtc_dedup* h, uint64_t cap) {
   return h->table_size + nodes < cap && h->table_size <= cap / 10 * 8;
 }
 
+// Slots of a baseline's digest table (without the overflow slot): a power of two
+// holding n entries at the load factor, at least 1024.
+uint64_t table_slots(uint64_t n, double load_factor) {
+  const double lf = load_factor > 0.05 ? load_factor : 0.125;
+  return next_pow2(std::max<uint64_t>((uint64_t)((double)n / lf), 1024));
+}
+
+// Device bytes a deduplicator of n chunks holds (the buffers of ensure_buffers,
+// the table at the given load factor and the diff arena).
+uint64_t state_bytes(uint64_t n, uint32_t chunk_size, double load_factor) {
+  const uint64_t table = 32 * (table_slots(n, load_factor) + 1);
+  const uint64_t per_chunk = 16 + 1 + 4 + 4 + 4 + 8 * 2 + 4 + 8 + 2 * (4 + 8) + 4 + 12 + 1;
+  return table + per_chunk * n + (uint64_t)chunk_size * n + (8u << 20);
+}
+
 static bool prepare_table(mtc_dedup* h, bool reset, hipStream_t st) {
-  const double lf = h->tune.load_factor > 0.05 ? h->tune.load_factor : 0.125;
   (void)st;
   if (reset) {
-    const uint64_t cap = next_pow2(std::max<uint64_t>((uint64_t)(h->n / lf), 1024));
+    const uint64_t cap = table_slots(h->n, h->tune.load_factor);
+    if (cap > kMaxTableSlots) throw Status{MTC_ERR_TABLE_FULL, "digest table above 2^31 slots"};
     h->table.ensure(cap + 1);
     h->table_cap = cap;
     h->table_size = 0;
@@ -2175,13 +1947,13 @@ struct ArenaImage {
     r.args = put(args.data(), sizeof(A) * args.size());
     r.prefix = put(prefix.data(), sizeof(uint32_t) * prefix.size());
     if (r.count > 1) {
-      std::vector<uint16_t> group((r.grid + 63) / 64);
+      std::vector<uint32_t> group((r.grid + 63) / 64);
       uint32_t m = 0;
       for (uint32_t gi = 0; gi < group.size(); gi++) {
         while (prefix[m + 1] <= 64 * gi) m++;
-        group[gi] = (uint16_t)m;
+        group[gi] = m;
       }
-      r.group = put(group.data(), sizeof(uint16_t) * group.size());
+      r.group = put(group.data(), sizeof(uint32_t) * group.size());
     }
     return r;
   }
@@ -2190,7 +1962,7 @@ struct ArenaImage {
 template <class A>
 Launch<A> bind(const LaunchRec& r, uint8_t* base) {
   return Launch<A>{reinterpret_cast<const A*>(base + r.args), reinterpret_cast<const uint32_t*>(base + r.prefix),
-                   r.count > 1 ? reinterpret_cast<const uint16_t*>(base + r.group) : nullptr, r.count};
+                   r.count > 1 ? reinterpret_cast<const uint32_t*>(base + r.group) : nullptr, r.count};
 }
 
 struct Member {
@@ -2228,16 +2000,30 @@ struct Ops {
 
 }  // namespace
 
+// Checks a member against its handle before anything of the step changes: a step
+// that is refused leaves every handle as it was.
+static void validate_member(Member& M) {
+  const mtc_dedup* h = M.h;
+  if (M.len == 0) throw Status{MTC_ERR_INVALID_ARG, "empty checkpoint buffer"};
+  const uint64_t C = h->chunk_size;
+  const uint64_t n64 = (M.len + C - 1) / C;
+  if (n64 > (1ull << 31)) throw Status{MTC_ERR_INVALID_ARG, "more than 2^31 chunks"};
+  M.baseline = M.baseline || h->current_id == 0 || !h->has_state;
+  if (!M.baseline && M.len != h->data_len)
+    throw Status{MTC_ERR_STATE, "incremental checkpoint with a different length than the previous one"};
+  if (M.baseline && h->approach != MTC_APPROACH_BASIC && table_slots(n64, h->tune.load_factor) > kMaxTableSlots)
+    throw Status{MTC_ERR_TABLE_FULL, "a digest table for " + std::to_string(n64) +
+                                         " chunks needs more than 2^31 slots at this load factor"};
+}
+
+// Sets a validated member up: geometry, buffers, table and launch shape.  The
+// handle holds no state until the step has completed (run_step sets has_state).
 static void prepare_member(Member& M, hipStream_t st) {
   mtc_dedup* h = M.h;
   const uint64_t C = h->chunk_size;
   const uint64_t n64 = (M.len + C - 1) / C;
-  if (n64 > (1ull << 31)) throw Status{MTC_ERR_INVALID_ARG, "more than 2^31 chunks"};
-  if (M.len == 0) throw Status{MTC_ERR_INVALID_ARG, "empty checkpoint buffer"};
-  M.baseline = M.baseline || h->current_id == 0 || !h->has_state;
-  if (!M.baseline && M.len != h->data_len)
-    throw Status{MTC_ERR_STATE, "incremental checkpoint with a different length than the previous one"};
   if (M.baseline) {
+    h->has_state = false;
     h->n = (uint32_t)n64;
     h->data_len = M.len;
     plan_tree(h);
@@ -2245,18 +2031,15 @@ static void prepare_member(Member& M, hipStream_t st) {
   ensure_buffers(h, st);
   M.split = h->approach == MTC_APPROACH_BASIC ? false : prepare_table(h, M.baseline, st);
   if (M.baseline) h->baseline_id = h->current_id;
-  h->has_state = true;
   M.n = h->n;
   M.cur = h->current_id;
   M.g = make_geo(M.n);
   M.cb = staged_cb(h->chunk_size, (uintptr_t)M.data % 16 == 0);
   // With asynchronous copies, checkpoints alternate diff arenas; this step waits
   // (on the device) for the copy still reading the arena it is about to write.
+  // The pending flag is cleared once the step has completed.
   M.arena = h->async_copies ? 1 - h->diff_cur : 0;
-  if (h->copy_pending[M.arena]) {
-    MTC_HIP(hipStreamWaitEvent(st, h->copy_done[M.arena], 0));
-    h->copy_pending[M.arena] = false;
-  }
+  if (h->copy_pending[M.arena]) MTC_HIP(hipStreamWaitEvent(st, h->copy_done[M.arena], 0));
   const uint64_t ntiles = (M.n + kThreads - 1) / kThreads;
   M.tpb = std::max<uint64_t>(1, h->tune.leaf_tiles_per_block);
   M.nblk_leaf = (uint32_t)((ntiles + M.tpb - 1) / M.tpb);
@@ -2639,9 +2422,26 @@ StepCtx::~StepCtx() {
     if (e) (void)hipEventDestroy(e);
 }
 
-// Runs one checkpoint of every member.  use_graph: replay the step's launches from
-// a captured graph when their shape matches the previous step's.
+static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_graph);
+
+// Runs one checkpoint of every member.  Every member is checked before any handle
+// changes.  Once work is under way, a failure of any kind (a HIP error, or a
+// member whose kernels report one) leaves every member of the step without state,
+// so that each handle's next checkpoint is a new baseline: the step may already
+// have overwritten leaf digests and added table entries of every member.
 void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_graph) {
+  for (auto& M : ms) validate_member(M);
+  try {
+    run_step_body(ctx, ms, st, use_graph);
+  } catch (...) {
+    for (auto& M : ms) M.h->has_state = false;
+    throw;
+  }
+}
+
+// use_graph: replay the step's launches from a captured graph when their shape
+// matches the previous step's.
+static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_graph) {
   HostTrace ht;
   const size_t B = ms.size();
   if (!ctx.ev[0])
@@ -2792,20 +2592,24 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
   MTC_HIP(hipEventElapsedTime(&ms_setup, ctx.ev[0], ctx.ev[1]));
   MTC_HIP(hipEventElapsedTime(&ms_total, ctx.ev[1], ctx.ev[3]));
   MTC_HIP(hipEventElapsedTime(&ms_leaf, ctx.ev[1], ctx.ev[4]));
-  std::string err;
+  // MTC_TEST_FAIL_MEMBER=i makes member i report an inconsistent state (tests of
+  // the failure path; nothing else reads it).
+  const char* inject = std::getenv("MTC_TEST_FAIL_MEMBER");
+  const long fail_member = inject ? std::atol(inject) : -1;
+  for (size_t i = 0; i < B; i++) {
+    const uint32_t e = ctx.ctl_h[i].err | ((long)i == fail_member ? (uint32_t)ERR_CORRUPT : 0u);
+    if (e & (ERR_TABLE_FULL | ERR_TABLE_SPIN | ERR_OVERFLOW_SLOT))
+      throw Status{MTC_ERR_TABLE_FULL, "member " + std::to_string(i) + ": digest table insert failed (err bits " +
+                                           std::to_string(e) + ")"};
+    if (e)
+      throw Status{MTC_ERR_STATE, "member " + std::to_string(i) + ": inconsistent table state (err bits " +
+                                      std::to_string(e) + ")"};
+  }
   for (size_t i = 0; i < B; i++) {
     mtc_dedup* h = ms[i].h;
     const Ctl& c = ctx.ctl_h[i];
-    if (c.err & (ERR_TABLE_FULL | ERR_TABLE_SPIN | ERR_OVERFLOW_SLOT)) {
-      h->has_state = false;  // the next checkpoint of this handle starts a new baseline
-      throw Status{MTC_ERR_TABLE_FULL, "member " + std::to_string(i) + ": digest table insert failed (err bits " +
-                                           std::to_string(c.err) + ")"};
-    }
-    if (c.err) {
-      h->has_state = false;
-      throw Status{MTC_ERR_STATE, "member " + std::to_string(i) + ": inconsistent table state (err bits " +
-                                      std::to_string(c.err) + ")"};
-    }
+    h->has_state = true;
+    h->copy_pending[ms[i].arena] = false;  // the step waited for that copy and has completed
     *h->ctl_h = c;
     h->table_size += c.claims;
     h->last_diff_size = c.diff_size;
@@ -2827,10 +2631,13 @@ void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_
 
 void checkpoint_batch_impl(mtc_dedup* const* hs, const uint8_t* const* datas, const uint64_t* lens,
                            const int* baselines, uint32_t count, StepCtx& ctx, hipStream_t st, bool use_graph) {
+  if (count > kMaxBatch) throw Status{MTC_ERR_INVALID_ARG, "more than 65536 members in one batch"};
   std::vector<Member> ms;
+  ms.reserve(count);
+  std::unordered_set<const mtc_dedup*> seen;
+  seen.reserve(2 * (size_t)count);
   for (uint32_t i = 0; i < count; i++) {
-    for (uint32_t j = 0; j < i; j++)
-      if (hs[j] == hs[i]) throw Status{MTC_ERR_INVALID_ARG, "a handle appears twice in one batch"};
+    if (!seen.insert(hs[i]).second) throw Status{MTC_ERR_INVALID_ARG, "a handle appears twice in one batch"};
     ms.push_back(Member{hs[i], datas[i], lens[i], baselines && baselines[i] != 0});
   }
   run_step(ctx, ms, st, use_graph);

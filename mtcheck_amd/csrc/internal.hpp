@@ -122,6 +122,16 @@ struct StepCtx {
   void reset_graph();
 };
 
+// Table slot indices are 32-bit and index `capacity` is the overflow slot, so a
+// table holds at most 2^31 slots.
+constexpr uint64_t kMaxTableSlots = 1ull << 31;
+// Most members of one batched step (mtc_checkpoint_batch, a serving lane's step).
+constexpr uint32_t kMaxBatch = 1u << 16;
+// Slots of a baseline's table for n chunks at a load factor (a power of two).
+uint64_t table_slots(uint64_t n, double load_factor);
+// Device bytes a deduplicator of n chunks holds at a load factor.
+uint64_t state_bytes(uint64_t n, uint32_t chunk_size, double load_factor);
+
 // Schedule parameters (set by mtc_set_tuning or the autotuner).
 struct Tune {
   uint64_t leaf_tiles_per_block = 1;  // 256-chunk tiles each leaf block streams

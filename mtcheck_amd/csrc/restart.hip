@@ -234,6 +234,7 @@ __global__ __launch_bounds__(kRThreads) void k_r_copy(const uint64_t* cover, uin
 }
 
 struct RestartScratch {
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};  // start, chain uploaded, done
   Workspace ws;
   DevBuf<uint32_t> err, region_off, set_vals;
   DevBuf<uint64_t> set_keys, cover, meta;
@@ -274,6 +275,15 @@ static RestartScratch& scratch_for(int dev) {
   return *r;
 }
 
+// The calling thread's last restart: upload of a host chain, and the whole restart
+// (restart_timers of src/tree_approach.cpp:1792-1793).
+static thread_local float g_restart_ms[2] = {0, 0};
+
+void last_restart_times(float* upload_ms, float* total_ms) {
+  if (upload_ms) *upload_ms = g_restart_ms[0];
+  if (total_ms) *total_ms = g_restart_ms[1];
+}
+
 static void check_err(uint32_t* err_d, hipStream_t st) {
   uint32_t e = 0;
   MTC_HIP(hipMemcpyAsync(&e, err_d, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -291,6 +301,10 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
   MTC_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> guard(scratch_mutex());
   RestartScratch& rs = scratch_for(dev);
+  g_restart_ms[0] = g_restart_ms[1] = 0;
+  if (!rs.ev[0])
+    for (auto& e : rs.ev) MTC_HIP(hipEventCreate(&e));
+  MTC_HIP(hipEventRecord(rs.ev[0], st));
   auto present = [&](uint32_t idx) {
     if (!diffs[idx] || sizes[idx] < sizeof(mtc_header)) throw Status{MTC_ERR_CORRUPT_DIFF, "missing or short diff"};
   };
@@ -361,6 +375,7 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
     sumF += d.h.num_first_ocur;
     pt_total += 8ull * d.h.num_prior_chkpts;
   }
+  MTC_HIP(hipEventRecord(rs.ev[1], st));
   // The prior-checkpoint tables (in bounds: off_data <= size was checked above).
   std::vector<std::vector<uint32_t>> ptables(L);
   if (on_device && pt_total) pinned_reserve(rs.hstage, rs.hstage_cap, pt_total);
@@ -491,7 +506,11 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
   k_r_copy<<<rgrid(n, 8192), kRThreads, 0, st>>>(cover_p, n, (uint32_t)C, hk.datalen, meta_p + 2 * L, meta_p + 3 * L,
                                                  out, vec, err_p);
   MTC_HIP(hipGetLastError());
+  MTC_HIP(hipEventRecord(rs.ev[2], st));
   check_err(err_p, st);
+  MTC_HIP(hipEventElapsedTime(&g_restart_ms[0], rs.ev[0], rs.ev[1]));
+  MTC_HIP(hipEventElapsedTime(&g_restart_ms[1], rs.ev[0], rs.ev[2]));
+  if (on_device) g_restart_ms[0] = 0;  // nothing was uploaded
 }
 
 }  // namespace mtc

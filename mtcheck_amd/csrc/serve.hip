@@ -27,8 +27,6 @@
 namespace mtc {
 void checkpoint_batch_impl(mtc_dedup* const* hs, const uint8_t* const* datas, const uint64_t* lens,
                            const int* baselines, uint32_t count, StepCtx& ctx, hipStream_t st, bool use_graph);
-uint64_t state_bytes(uint64_t n, uint32_t chunk_size, double load_factor);
-uint64_t table_slots(uint64_t n, double load_factor);
 void pregrow(mtc_dedup* h, hipStream_t st);
 }  // namespace mtc
 
@@ -54,6 +52,7 @@ struct Request {
 
 struct Client {
   mtc_dedup* h = nullptr;
+  uint64_t max_len = 0;      // longest buffer the client may submit (its state estimate's size)
   uint64_t state_est = 0;
   uint64_t table_slots = 0;  // table slots counted in state_est
   uint64_t table_extra = 0;  // bytes of table grown beyond them (counted while resident)
@@ -217,28 +216,6 @@ struct mtc_server {
   }
 };
 
-namespace mtc {
-
-// Device bytes a deduplicator of n chunks holds (the buffers of ensure_buffers,
-// the table at the given load factor and the diff arena).
-// Slots of a baseline's digest table (without the overflow slot), as the
-// deduplicator sizes it.
-uint64_t table_slots(uint64_t n, double load_factor) {
-  const double lf = load_factor > 0.05 ? load_factor : 0.125;
-  const uint64_t want = std::max<uint64_t>((uint64_t)((double)n / lf), 1024);
-  uint64_t cap = 1;
-  while (cap < want) cap <<= 1;
-  return cap;
-}
-
-uint64_t state_bytes(uint64_t n, uint32_t chunk_size, double load_factor) {
-  const uint64_t table = 32 * (table_slots(n, load_factor) + 1);
-  const uint64_t per_chunk = 16 + 1 + 4 + 4 + 4 + 8 * 2 + 4 + 8 + 2 * (4 + 8) + 4 + 12 + 1;
-  return table + per_chunk * n + (uint64_t)chunk_size * n + (8u << 20);
-}
-
-}  // namespace mtc
-
 static thread_local std::string g_serve_error;
 
 template <class F>
@@ -270,7 +247,8 @@ static int serve_guarded(mtc_server* s, F&& body) {
 extern "C" {
 
 int mtc_server_create(mtc_server** out, int device, uint64_t memory_budget, uint32_t lanes, uint32_t max_batch) {
-  if (!out || lanes == 0 || lanes > 8 || max_batch == 0 || memory_budget == 0) return MTC_ERR_INVALID_ARG;
+  if (!out || lanes == 0 || lanes > 8 || max_batch == 0 || max_batch > mtc::kMaxBatch || memory_budget == 0)
+    return MTC_ERR_INVALID_ARG;
   *out = nullptr;
   mtc_server* s = new mtc_server();
   s->device = device;
@@ -325,16 +303,28 @@ int mtc_server_destroy(mtc_server* s) {
 }
 
 int mtc_server_add_client(mtc_server* s, uint32_t chunk_size, int hash_kind, uint64_t max_len, uint32_t* client) {
-  if (!s || !client || max_len == 0) return MTC_ERR_INVALID_ARG;
+  if (!s || !client || max_len == 0 || chunk_size == 0) return MTC_ERR_INVALID_ARG;
+  // A client whose resident state could never fit the budget would wait forever.
+  const uint64_t n = (max_len + chunk_size - 1) / chunk_size;
+  const double lf = mtc::Tune{}.load_factor;
+  if (n > (1ull << 31)) return MTC_ERR_INVALID_ARG;
+  if (mtc::table_slots(n, lf) > mtc::kMaxTableSlots) return MTC_ERR_TABLE_FULL;
+  const uint64_t est = mtc::state_bytes(n, chunk_size, lf);
+  if (est > s->budget) {
+    g_serve_error = "the client's resident state (" + std::to_string(est) + " bytes) exceeds the memory budget";
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->last_error = g_serve_error;
+    return MTC_ERR_OUT_OF_MEMORY;
+  }
   mtc_dedup* h = nullptr;
   int rc = mtc_create(&h, chunk_size, MTC_APPROACH_TREE, hash_kind, s->device);
   if (rc != MTC_OK) return rc;
   std::lock_guard<std::mutex> lk(s->mu);
   Client c;
   c.h = h;
-  const uint64_t n = (max_len + chunk_size - 1) / chunk_size;
-  c.state_est = mtc::state_bytes(n, chunk_size, h->tune.load_factor);
-  c.table_slots = mtc::table_slots(n, h->tune.load_factor) + 1;
+  c.max_len = max_len;
+  c.state_est = est;
+  c.table_slots = mtc::table_slots(n, lf) + 1;
   s->clients.push_back(c);
   *client = (uint32_t)(s->clients.size() - 1);
   return MTC_OK;
@@ -382,6 +372,8 @@ int mtc_server_submit(mtc_server* s, uint32_t client, const void* dev_data, uint
       std::lock_guard<std::mutex> lk(s->mu);
       if (client >= s->clients.size() || s->clients[client].removed)
         throw mtc::Status{MTC_ERR_INVALID_ARG, "unknown client"};
+      if (len > s->clients[client].max_len)
+        throw mtc::Status{MTC_ERR_INVALID_ARG, "submission longer than the client's registered max_len"};
       if (!s->free_events.empty()) {
         ev = s->free_events.back();
         s->free_events.pop_back();

@@ -9,7 +9,6 @@ from __future__ import annotations
 import ctypes
 import os
 import struct
-import time
 from typing import Sequence
 
 from . import _lib
@@ -190,9 +189,8 @@ class TreeDeduplicator:
 
     # -- restart overloads (src/tree_approach.cpp:1786-1840) ---------------------
     def restart(self, out, chkpts, chkpt_id: int, logname: str | None = None, stream=None) -> None:
-        t0 = time.perf_counter()
         restart_into(out, chkpts, chkpt_id, device=self.device, stream=stream, approach=self.approach)
-        self.restart_timers = [0.0, time.perf_counter() - t0]
+        self.restart_timers = last_restart_times()
         if logname:
             self.write_restart_log(chkpt_id, f"{logname}.chunk_size.{self.chunk_size}.restart_timing.csv")
 
@@ -203,9 +201,8 @@ class TreeDeduplicator:
             suffix = ".basic.incr_chkpt" if self.approach == BASIC else ".hashtree.incr_chkpt"
             with open(name + suffix, "rb") as f:
                 diffs.append(f.read())
-        t0 = time.perf_counter()
         restart_into(out, diffs, chkpt_id, device=self.device, stream=stream, approach=self.approach)
-        self.restart_timers = [0.0, time.perf_counter() - t0]
+        self.restart_timers = last_restart_times()
         if logname:
             self.write_restart_log(chkpt_id, logname)
 
@@ -303,6 +300,14 @@ def metadata_breakdown(header, diff: bytes, num_chkpts: int) -> str:
             out.append(",0")
     out.append("\n")
     return "".join(out)
+
+
+def last_restart_times() -> list:
+    """[upload, whole restart] of this thread's last restart, in seconds
+    (restart_timers, src/tree_approach.cpp:1792-1793; upload 0 for device chains)."""
+    up, total = ctypes.c_float(), ctypes.c_float()
+    check(lib().mtc_restart_times(ctypes.byref(up), ctypes.byref(total)))
+    return [up.value / 1e3, total.value / 1e3]
 
 
 def restart_into(out, chkpts, chkpt_id: int, device: int = 0, stream=None, approach: int = TREE) -> None:

@@ -2,9 +2,11 @@
 
 mtc_restart_basic is checked against the oracle's basic diffs: every checkpoint
 of a series must restart, from host and from device diffs, to the buffer it was
-taken of.  The basic checkpoint itself is switched off in this build (mtc_create
-refuses MTC_APPROACH_BASIC); its parity tests are kept, skipped, until its device
-path has passed them.  Every test prints its case first.
+taken of.  Basic checkpoints are checked byte for byte against the oracle.  Their
+first device runs faulted (a baseline's digest-table reset went through the null
+table pointer of a basic handle); tests/test_host_launches.py shows that launch on
+the CPU and checks every launch of basic steps there.  Every test prints its case
+first.
 """
 import numpy as np
 import pytest
@@ -17,10 +19,6 @@ import mtcheck_amd as M  # noqa: E402
 from mtcheck_amd.workload import CheckpointSeries  # noqa: E402
 
 pytestmark = pytest.mark.gpu
-
-SWITCHED_OFF = ("basic-approach checkpoints are switched off: their first device run wrote the digest-table "
-                "reset through a null table pointer (basic handles keep no table); the reset now skips them, "
-                "and this test is the check that re-enables the path")
 
 
 def say(msg):
@@ -93,13 +91,6 @@ def test_basic_restart_rejects_tree_diffs():
         M.restart_into(out, [d0, d1], 1, approach=M.BASIC)
 
 
-def test_basic_checkpoint_is_switched_off():
-    say("basic handle creation")
-    with pytest.raises(M.MtcError):
-        M.TreeDeduplicator(64, approach=M.BASIC)
-
-
-@pytest.mark.skip(reason=SWITCHED_OFF)
 @pytest.mark.parametrize("nbytes,C,hk,base_at", SERIES)
 def test_basic_series(nbytes, C, hk, base_at):
     say(f"basic series nbytes={nbytes} C={C} hk={hk} baselines={base_at}")
@@ -128,7 +119,6 @@ def test_basic_series(nbytes, C, hk, base_at):
         assert np.array_equal(out.cpu().numpy(), versions[k]), ("device diffs", k)
 
 
-@pytest.mark.skip(reason=SWITCHED_OFF)
 def test_basic_rewrite_with_same_bytes_is_not_listed():
     """A chunk rewritten with its previous bytes has the previous digest: not listed."""
     C = 64

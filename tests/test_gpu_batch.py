@@ -85,3 +85,40 @@ def test_batch_rejects_duplicate_handle():
     a = torch.zeros(4096, dtype=torch.uint8, device="cuda:0")
     with pytest.raises(M.MtcError):
         M.Batch().checkpoint([d, d], [a, a], make_baseline=True)
+
+
+def test_failed_batch_step_restarts_every_member_from_a_baseline(monkeypatch):
+    """A member whose step reports an error (forced with MTC_TEST_FAIL_MEMBER) fails
+    the whole batch; every member's next checkpoint is a new baseline, equal to the
+    oracle's, and restarts exactly."""
+    mem = [(150_001, 128, M.MURMUR3), (64 * 1000, 64, M.MURMUR3), (40_000, 512, M.MD5)]
+    series = [CheckpointSeries(nb, C, seed=40 + i) for i, (nb, C, _) in enumerate(mem)]
+    bufs = [torch.empty(nb, dtype=torch.uint8, device="cuda:0") for nb, _, _ in mem]
+    ds = [M.TreeDeduplicator(C, hash_kind=hk) for _, C, hk in mem]
+    orc = [O.OracleDedup(C, hk) for _, C, hk in mem]
+    batch = M.Batch()
+    for k in range(2):
+        for s, b in zip(series, bufs):
+            s.apply_device(b, k)
+        torch.cuda.synchronize()
+        say(f"failure injection: step {k}")
+        got = batch.checkpoint(ds, bufs, make_baseline=(k == 0), copy=True)
+        for i, g in enumerate(got):
+            assert g == orc[i].checkpoint(bufs[i].cpu().numpy(), make_baseline=(k == 0)), (k, i)
+    for s, b in zip(series, bufs):
+        s.apply_device(b, 2)
+    torch.cuda.synchronize()
+    say("failure injection: failing step (member 1)")
+    monkeypatch.setenv("MTC_TEST_FAIL_MEMBER", "1")
+    with pytest.raises(M.MtcError):
+        batch.checkpoint(ds, bufs)
+    monkeypatch.delenv("MTC_TEST_FAIL_MEMBER")
+    say("failure injection: retried step")
+    got = batch.checkpoint(ds, bufs, copy=True)
+    for i, (g, (nb, C, hk)) in enumerate(zip(got, mem)):
+        h = O.parse_header(g)
+        assert (h["ref_id"], h["chkpt_id"]) == (2, 2), (i, h)
+        assert g == orc[i].checkpoint(bufs[i].cpu().numpy(), make_baseline=True), i
+        out = torch.zeros_like(bufs[i])
+        M.restart_into(out, [None, None, g], 2)
+        assert torch.equal(out, bufs[i]), i
