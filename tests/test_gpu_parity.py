@@ -209,7 +209,7 @@ def test_device_generator_matches_host():
 def test_errors_are_reported():
     say("error paths")
     with pytest.raises(M.MtcError) as e:
-        M.TreeDeduplicator(128, approach=M.BASIC)
+        M.TreeDeduplicator(128, approach=M.FULL)
     assert e.value.code == 2
     with pytest.raises(M.MtcError):
         M.TreeDeduplicator(0)
