@@ -142,6 +142,19 @@ int mtc_restart(const void* const* diffs, const uint64_t* sizes, uint32_t count,
 int mtc_restart_basic(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t chkpt_id,
                       int diffs_on_device, void* dev_out, uint64_t out_len, int device, void* stream);
 
+/* No counterpart in the reference: mtc_restart for callers that cannot allocate
+ * device memory (code that runs where hipMalloc is not allowed, tight serving
+ * loops, harnesses that forbid allocation).  It calls no hipMalloc, hipFree,
+ * hipHostMalloc or other allocating device routine: its scratch is one static
+ * 64 MiB device array per GPU.  Device-resident tree chains only (dev_diffs[i]
+ * device pointers), of at most 256 diffs from the chain's ref_id to chkpt_id;
+ * a restart whose scratch (about 8 bytes per chunk plus 16 per first-occurrence
+ * root) exceeds 64 MiB -- roughly 7.5 million chunks -- returns
+ * MTC_ERR_INVALID_ARG.  Calls on one GPU are serialised by the library; the
+ * stream is synchronised before the call returns. */
+int mtc_restart_noalloc(const void* const* dev_diffs, const uint64_t* sizes, uint32_t count, uint32_t chkpt_id,
+                        void* dev_out, uint64_t out_len, int device, void* stream);
+
 /* The calling thread's last mtc_restart / mtc_restart_basic: the upload of a host
  * chain and the whole restart, in milliseconds (restart_timers[0..1],
  * src/tree_approach.cpp:755-758,1124-1127,1792-1793; 0 upload for device chains). */

@@ -9,6 +9,8 @@ void checkpoint_impl(mtc_dedup* h, const uint8_t* data, uint64_t len, bool make_
 void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t k, bool on_device,
                   uint8_t* out, uint64_t out_len, hipStream_t st, bool basic);
 void last_restart_times(float* upload_ms, float* total_ms);
+void restart_noalloc_impl(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t k, uint8_t* out,
+                          uint64_t out_len, hipStream_t st);
 void hash_chunks(const uint8_t* data, uint64_t len, uint32_t C, int hk, Digest* out, hipStream_t st);
 void checkpoint_batch_impl(mtc_dedup* const* hs, const uint8_t* const* datas, const uint64_t* lens,
                            const int* baselines, uint32_t count, StepCtx& ctx, hipStream_t st, bool use_graph);
@@ -231,6 +233,16 @@ int mtc_restart_basic(const void* const* diffs, const uint64_t* sizes, uint32_t 
     use_device(device);
     mtc::restart_impl(diffs, sizes, count, chkpt_id, diffs_on_device != 0, static_cast<uint8_t*>(dev_out), out_len,
                       static_cast<hipStream_t>(stream), true);
+  });
+}
+
+int mtc_restart_noalloc(const void* const* dev_diffs, const uint64_t* sizes, uint32_t count, uint32_t chkpt_id,
+                        void* dev_out, uint64_t out_len, int device, void* stream) {
+  return guarded(nullptr, [&] {
+    if (!dev_out) throw Status{MTC_ERR_INVALID_ARG, "no output buffer"};
+    use_device(device);
+    mtc::restart_noalloc_impl(dev_diffs, sizes, count, chkpt_id, static_cast<uint8_t*>(dev_out), out_len,
+                              static_cast<hipStream_t>(stream));
   });
 }
 

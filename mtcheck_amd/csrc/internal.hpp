@@ -95,6 +95,11 @@ struct Workspace {
 
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t count, uint32_t* total, Workspace& ws,
                         hipStream_t st);
+// The same scan with caller-provided tile-sum buffers (no allocation): sums[d]
+// holds at least scan_sums_words(count, d) words.
+uint64_t scan_sums_words(uint64_t count, int depth);
+void exclusive_scan_u32_fixed(const uint32_t* in, uint32_t* out, uint64_t count, uint32_t* total,
+                              uint32_t* const sums[3], hipStream_t st);
 
 // Launch state of one step (a checkpoint of one or many deduplicators): the
 // uploaded argument arena, the step's result array and, when steps are replayed

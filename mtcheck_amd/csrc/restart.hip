@@ -513,4 +513,232 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
   if (on_device) g_restart_ms[0] = 0;  // nothing was uploaded
 }
 
+// ---------------------------------------------------------------------------
+// Restart without allocation (mtc_restart_noalloc).  Every scratch buffer of the
+// restart above -- the cover array, the (position, root) set, region offsets, the
+// per-position tables and scan partials -- is carved from one static device array
+// per GPU, and the host side uses no pinned staging: headers come back through one
+// gather kernel and one small copy, and the running counts of shifted entries by
+// source checkpoint are built on the device.  Chains are device-resident tree
+// diffs of at most kNoAllocMaxChain checkpoints.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kNoAllocScratch = 64ull << 20;  // bytes of `scratch` on each GPU
+constexpr uint32_t kNoAllocMaxChain = 256;         // diffs from ref_id to the restarted checkpoint
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratch];
+
+// Per GPU: the device address of `scratch` (taken once, with that GPU current) and
+// the lock that keeps restarts on one GPU from sharing the array at the same time.
+struct NoAllocDevice {
+  std::mutex mu;
+  uint8_t* ptr = nullptr;
+};
+static NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+// The 32-byte headers of the chain's diffs, gathered next to each other.
+__global__ void k_r_gather_headers(const uint64_t* ptrs, uint32_t L, uint32_t* out) {
+  const uint32_t i = blockIdx.x * kRThreads + threadIdx.x;
+  if (i >= 8 * L) return;
+  out[i] = ld32(reinterpret_cast<const uint8_t*>(ptrs[i / 8]) + 4 * (i % 8));
+}
+
+// One block per chain position p: the inclusive running count, by checkpoint id
+// ref..ref+p, of the shifted entries of diff p (its prior-checkpoint table,
+// src/tree_approach.cpp:872-883), checked against the diff's S.
+__global__ __launch_bounds__(kRThreads) void k_r_cum(const uint64_t* table_ptr, const uint64_t* ps,
+                                                     const uint64_t* cum_off, uint32_t ref, uint32_t* cum,
+                                                     uint32_t* err) {
+  __shared__ uint32_t cnt[kNoAllocMaxChain];
+  const uint32_t p = blockIdx.x, t = threadIdx.x;
+  const uint32_t P = (uint32_t)(ps[p] >> 32), S = (uint32_t)ps[p];
+  const uint8_t* tab = reinterpret_cast<const uint8_t*>(table_ptr[p]);
+  cnt[t] = 0;
+  __syncthreads();
+  for (uint32_t i = t; i < P; i += kRThreads) {
+    const uint32_t c = ld32(tab + 8ull * i), m = ld32(tab + 8ull * i + 4);
+    if (c < ref || c > ref + p) atomicOr(err, ERR_CORRUPT);
+    else atomicAdd(&cnt[c - ref], m);
+  }
+  __syncthreads();
+  for (uint32_t o = 1; o < kNoAllocMaxChain; o <<= 1) {  // inclusive scan
+    const uint32_t v = (t >= o) ? cnt[t - o] : 0u;
+    __syncthreads();
+    cnt[t] += v;
+    __syncthreads();
+  }
+  if (t <= p) cum[cum_off[p] + t] = cnt[t];
+  if (t == p && cnt[t] != S) atomicOr(err, ERR_CORRUPT);
+}
+
+// k_r_cover_first with the region total read on the device (region_off[F]) and
+// checked against the data section's capacity.
+__global__ void k_r_cover_first_dev(const uint8_t* first, uint32_t F, const uint32_t* region_off,
+                                    uint64_t data_chunks, Geo g, uint32_t pos, uint64_t* cover, uint32_t* err) {
+  const uint64_t total = region_off[F];
+  if (total > data_chunks) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, ERR_CORRUPT);
+    return;
+  }
+  for (uint64_t j = (uint64_t)blockIdx.x * kRThreads + threadIdx.x; j < total; j += (uint64_t)gridDim.x * kRThreads) {
+    uint32_t lo = 0, hi = F;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (region_off[mid] <= j) lo = mid;
+      else hi = mid;
+    }
+    uint32_t st[2], ct[2];
+    const int r = leaf_runs(g, ld32(first + 4ull * lo), st, ct);
+    const uint32_t chunk = run_at(r, st, ct, (uint32_t)(j - region_off[lo]));
+    cover[chunk] = ((uint64_t)pos << kSlotBits) | j;
+  }
+}
+
+// Bump carving of the scratch array (256-byte aligned pieces).
+struct Carve {
+  uint64_t off = 0;
+  uint64_t take(uint64_t bytes) {
+    const uint64_t o = off;
+    off = (off + bytes + 255) & ~255ull;
+    return o;
+  }
+};
+
+void restart_noalloc_impl(const void* const* diffs, const uint64_t* sizes, uint32_t count, uint32_t k, uint8_t* out,
+                          uint64_t out_len, hipStream_t st) {
+  if (!diffs || !sizes || k >= count || !diffs[k]) throw Status{MTC_ERR_INVALID_ARG, "bad diff chain arguments"};
+  int dev = 0;
+  MTC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) throw Status{MTC_ERR_INVALID_ARG, "device ordinal above 63"};
+  NoAllocDevice& nd = g_noalloc[dev];
+  std::lock_guard<std::mutex> guard(nd.mu);
+  if (!nd.ptr) {  // once per GPU, with it current: the device copy of `scratch`
+    void* p = nullptr;
+    MTC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Status{MTC_ERR_HIP, "no device address for the restart scratch"};
+    nd.ptr = static_cast<uint8_t*>(p);
+  }
+  uint8_t* const base = nd.ptr;
+  g_restart_ms[0] = g_restart_ms[1] = 0;
+  if (sizes[k] < sizeof(mtc_header)) throw Status{MTC_ERR_CORRUPT_DIFF, "missing or short diff"};
+  mtc_header hk;
+  MTC_HIP(hipMemcpyAsync(&hk, diffs[k], sizeof(hk), hipMemcpyDeviceToHost, st));
+  MTC_HIP(hipStreamSynchronize(st));
+  if (hk.chkpt_id != k || hk.ref_id > k || hk.chunk_size == 0 || hk.datalen == 0)
+    throw Status{MTC_ERR_CORRUPT_DIFF, "bad header of the requested diff"};
+  if (out_len < hk.datalen) throw Status{MTC_ERR_BUFFER_TOO_SMALL, "output shorter than datalen"};
+  const uint64_t C = hk.chunk_size;
+  const uint64_t n64 = (hk.datalen + C - 1) / C;
+  if (n64 > (1ull << 31)) throw Status{MTC_ERR_CORRUPT_DIFF, "too many chunks"};
+  const uint32_t n = (uint32_t)n64, ref = hk.ref_id, L = k - ref + 1;
+  if (L > kNoAllocMaxChain)
+    throw Status{MTC_ERR_INVALID_ARG, "the no-allocation restart takes chains of at most 256 diffs"};
+  const Geo g = make_geo(n);
+
+  // Headers of the chain: one gather kernel, one copy.
+  Carve cv;
+  uint32_t* err_p = reinterpret_cast<uint32_t*>(base + cv.take(4));
+  uint64_t* ptrs_d = reinterpret_cast<uint64_t*>(base + cv.take(8ull * L));
+  uint32_t* hdr_d = reinterpret_cast<uint32_t*>(base + cv.take(32ull * L));
+  uint64_t ptrs_h[kNoAllocMaxChain];
+  for (uint32_t p = 0; p < L; p++) {
+    const uint32_t idx = ref + p;
+    if (!diffs[idx] || sizes[idx] < sizeof(mtc_header)) throw Status{MTC_ERR_CORRUPT_DIFF, "missing or short diff"};
+    ptrs_h[p] = reinterpret_cast<uint64_t>(diffs[idx]);
+  }
+  MTC_HIP(hipMemcpyAsync(ptrs_d, ptrs_h, 8ull * L, hipMemcpyHostToDevice, st));
+  k_r_gather_headers<<<rgrid(8ull * L), kRThreads, 0, st>>>(ptrs_d, L, hdr_d);
+  MTC_HIP(hipGetLastError());
+  mtc_header hs[kNoAllocMaxChain];
+  MTC_HIP(hipMemcpyAsync(hs, hdr_d, 32ull * L, hipMemcpyDeviceToHost, st));
+  MTC_HIP(hipStreamSynchronize(st));
+
+  // Chain layout (the same checks as the allocating restart).
+  uint64_t off_first[kNoAllocMaxChain], off_shift[kNoAllocMaxChain], off_table[kNoAllocMaxChain];
+  uint64_t meta_h[7 * kNoAllocMaxChain];  // F-root, region base, data, data chunks, table, (P, S), cum offset
+  uint64_t sumF = 0, cum_words = 0;
+  uint32_t maxF = 0;
+  for (uint32_t p = 0; p < L; p++) {
+    const uint32_t idx = ref + p;
+    const mtc_header& h = hs[p];
+    if (h.chkpt_id != idx || h.ref_id != ref || h.datalen != hk.datalen || h.chunk_size != hk.chunk_size)
+      throw Status{MTC_ERR_CORRUPT_DIFF, "diff " + std::to_string(idx) + " does not belong to the chain"};
+    off_first[p] = 32;
+    off_table[p] = off_first[p] + 4ull * h.num_first_ocur;
+    off_shift[p] = off_table[p] + 8ull * h.num_prior_chkpts;
+    const uint64_t off_data = off_shift[p] + 8ull * h.num_shift_dupl;
+    if (off_data > sizes[idx]) throw Status{MTC_ERR_CORRUPT_DIFF, "diff shorter than its metadata"};
+    if (h.num_prior_chkpts > p + 1) throw Status{MTC_ERR_CORRUPT_DIFF, "prior-checkpoint table longer than the chain"};
+    const uint8_t* b = static_cast<const uint8_t*>(diffs[idx]);
+    meta_h[p] = reinterpret_cast<uint64_t>(b + off_first[p]);
+    meta_h[L + p] = sumF + p;
+    meta_h[2 * L + p] = reinterpret_cast<uint64_t>(b + off_data);
+    meta_h[3 * L + p] = (sizes[idx] - off_data) / C;
+    meta_h[4 * L + p] = reinterpret_cast<uint64_t>(b + off_table[p]);
+    meta_h[5 * L + p] = ((uint64_t)h.num_prior_chkpts << 32) | h.num_shift_dupl;
+    meta_h[6 * L + p] = cum_words;
+    cum_words += p + 1;
+    sumF += h.num_first_ocur;
+    maxF = std::max(maxF, h.num_first_ocur);
+  }
+  uint64_t set_cap = 16;
+  while (set_cap < 2 * sumF) set_cap <<= 1;
+  uint64_t* cover_p = reinterpret_cast<uint64_t*>(base + cv.take(8ull * n));
+  uint64_t* keys_p = reinterpret_cast<uint64_t*>(base + cv.take(8 * set_cap));
+  uint32_t* vals_p = reinterpret_cast<uint32_t*>(base + cv.take(4 * set_cap));
+  const uint64_t off_region = cv.take(4 * (sumF + L));
+  const uint64_t off_meta = cv.take(8ull * 7 * L);
+  const uint64_t off_cum = cv.take(4 * cum_words);
+  uint64_t off_sums[3];
+  for (int d = 0; d < 3; d++) off_sums[d] = cv.take(4 * scan_sums_words(maxF, d));
+  if (cv.off > kNoAllocScratch)
+    throw Status{MTC_ERR_INVALID_ARG, "this restart needs " + std::to_string(cv.off) +
+                                          " bytes of scratch; the no-allocation restart has 64 MiB per GPU"};
+  uint32_t* region_p = reinterpret_cast<uint32_t*>(base + off_region);
+  uint64_t* meta_p = reinterpret_cast<uint64_t*>(base + off_meta);
+  uint32_t* cum_p = reinterpret_cast<uint32_t*>(base + off_cum);
+  uint32_t* const sums[3] = {reinterpret_cast<uint32_t*>(base + off_sums[0]),
+                             reinterpret_cast<uint32_t*>(base + off_sums[1]),
+                             reinterpret_cast<uint32_t*>(base + off_sums[2])};
+  MTC_HIP(hipMemcpyAsync(meta_p, meta_h, 8ull * 7 * L, hipMemcpyHostToDevice, st));
+  // The array keeps the previous call's contents: everything read before it is
+  // written starts from a fill.
+  k_r_fill32<<<1, kRThreads, 0, st>>>(err_p, 1, 0);
+  k_r_fill64<<<rgrid(set_cap, 65536), kRThreads, 0, st>>>(keys_p, set_cap, kNoCover);
+  k_r_fill64<<<rgrid(n, 65536), kRThreads, 0, st>>>(cover_p, n, kNoCover);
+  k_r_cum<<<L, kRThreads, 0, st>>>(meta_p + 4 * L, meta_p + 5 * L, meta_p + 6 * L, ref, cum_p, err_p);
+  MTC_HIP(hipGetLastError());
+  for (uint32_t p = 0; p < L; p++) {
+    const uint32_t F = hs[p].num_first_ocur;
+    const uint8_t* b = static_cast<const uint8_t*>(diffs[ref + p]);
+    uint32_t* ro = region_p + meta_h[L + p];
+    if (F) {
+      k_r_roots<<<rgrid(F), kRThreads, 0, st>>>(b + off_first[p], F, g, ro, p, keys_p, vals_p, set_cap - 1, err_p, 0);
+      MTC_HIP(hipGetLastError());
+    }
+    exclusive_scan_u32_fixed(ro, ro, F, ro + F, sums, st);
+  }
+  check_err(err_p, st);  // every root lies inside the tree before any cover is written
+  for (uint32_t p = 0; p < L; p++) {
+    const uint32_t F = hs[p].num_first_ocur, S = hs[p].num_shift_dupl;
+    const uint8_t* b = static_cast<const uint8_t*>(diffs[ref + p]);
+    if (F) {
+      k_r_cover_first_dev<<<rgrid(std::max<uint64_t>(meta_h[3 * L + p], 1), 65536), kRThreads, 0, st>>>(
+          b + off_first[p], F, region_p + meta_h[L + p], meta_h[3 * L + p], g, p, cover_p, err_p);
+      MTC_HIP(hipGetLastError());
+    }
+    if (S) {
+      ShiftArgs sa{b + off_shift[p], S, cum_p + meta_h[6 * L + p], p + 1, ref, p, g, keys_p, vals_p, set_cap - 1,
+                   meta_p, meta_p + L, region_p, cover_p, err_p};
+      k_r_cover_shift<<<rgrid(S), kRThreads, 0, st>>>(sa);
+      MTC_HIP(hipGetLastError());
+    }
+  }
+  const int vec = (C % 16 == 0) && ((uintptr_t)out % 16 == 0);
+  k_r_copy<<<rgrid(n, 8192), kRThreads, 0, st>>>(cover_p, n, (uint32_t)C, hk.datalen, meta_p + 2 * L, meta_p + 3 * L,
+                                                 out, vec, err_p);
+  MTC_HIP(hipGetLastError());
+  check_err(err_p, st);
+}
+
 }  // namespace mtc

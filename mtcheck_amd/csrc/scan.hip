@@ -63,27 +63,47 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* in,
   if (total && !tile_base && threadIdx.x == 0) *total = tot;
 }
 
-static void scan_level(const uint32_t* in, uint32_t* out, uint64_t count, uint32_t* total, Workspace& ws, int depth,
-                       hipStream_t st) {
+// One level of the scan.  Tile sums go to sums[depth]: the workspace's buffers
+// (grown on demand) or, when fixed is given, caller-provided buffers of at least
+// scan_sums_words(count, depth) words.
+static void scan_level(const uint32_t* in, uint32_t* out, uint64_t count, uint32_t* total, Workspace* ws,
+                       uint32_t* const* fixed, int depth, hipStream_t st) {
   const uint64_t tiles = (count + kScanTile - 1) / kScanTile;
   if (tiles <= 1) {
     k_scan_apply<<<1, kScanThreads, 0, st>>>(in, out, count, nullptr, total);
     MTC_HIP(hipGetLastError());
     return;
   }
-  DevBuf<uint32_t>& sums = depth == 0 ? ws.u32a : (depth == 1 ? ws.u32b : ws.u32c);
   if (depth > 2) throw HipError{hipErrorInvalidValue, "scan depth"};
-  sums.reserve(tiles);
-  k_scan_sums<<<(uint32_t)tiles, kScanThreads, 0, st>>>(in, count, sums.p);
+  uint32_t* sums = nullptr;
+  if (fixed) {
+    sums = fixed[depth];
+  } else {
+    DevBuf<uint32_t>& buf = depth == 0 ? ws->u32a : (depth == 1 ? ws->u32b : ws->u32c);
+    buf.reserve(tiles);
+    sums = buf.p;
+  }
+  k_scan_sums<<<(uint32_t)tiles, kScanThreads, 0, st>>>(in, count, sums);
   MTC_HIP(hipGetLastError());
-  scan_level(sums.p, sums.p, tiles, total, ws, depth + 1, st);
-  k_scan_apply<<<(uint32_t)tiles, kScanThreads, 0, st>>>(in, out, count, sums.p, nullptr);
+  scan_level(sums, sums, tiles, total, ws, fixed, depth + 1, st);
+  k_scan_apply<<<(uint32_t)tiles, kScanThreads, 0, st>>>(in, out, count, sums, nullptr);
   MTC_HIP(hipGetLastError());
 }
 
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t count, uint32_t* total, Workspace& ws,
                         hipStream_t st) {
-  scan_level(in, out, count, total, ws, 0, st);  // count == 0 still writes *total = 0
+  scan_level(in, out, count, total, &ws, nullptr, 0, st);  // count == 0 still writes *total = 0
+}
+
+uint64_t scan_sums_words(uint64_t count, int depth) {
+  uint64_t t = count;
+  for (int d = 0; d <= depth; d++) t = (t + kScanTile - 1) / kScanTile;
+  return t;
+}
+
+void exclusive_scan_u32_fixed(const uint32_t* in, uint32_t* out, uint64_t count, uint32_t* total,
+                              uint32_t* const sums[3], hipStream_t st) {
+  scan_level(in, out, count, total, nullptr, sums, 0, st);
 }
 
 }  // namespace mtc

@@ -367,7 +367,7 @@ void __hipRegisterFunction(void**, const void* hostFunction, char*, const char* 
   std::lock_guard<std::mutex> g(S().mu);
   S().names[hostFunction] = deviceName ? deviceName : "";
 }
-void __hipRegisterVar(void*, void* var, char*, const char*, int, size_t size, int, int) {
+void __hipRegisterVar(void**, void* var, char*, char*, int, size_t size, int, int) {
   // The device copy of a variable: an allocation of its own in the fake heap.
   void* p = nullptr;
   hipMalloc(&p, size);
