@@ -87,8 +87,9 @@ __device__ __forceinline__ unsigned long long* stripe(unsigned long long* ctr, u
 // contiguous result array (one device-to-host copy for the whole batch).  Run by
 // one whole block of kStripes threads, after every counter and control field of
 // the checkpoint is final.
-__device__ void finish_member(const unsigned long long* ctr, const uint32_t* changed, Ctl* ctl, Ctl* out) {
-  __shared__ unsigned long long s[CTR_KINDS];
+// s: CTR_KINDS words of LDS.
+__device__ void finish_member(const unsigned long long* ctr, const uint32_t* changed, Ctl* ctl, Ctl* out,
+                              unsigned long long* s) {
   if (threadIdx.x < CTR_KINDS) s[threadIdx.x] = 0;
   __syncthreads();
   for (uint32_t k = 0; k < CTR_KINDS; k++) {
@@ -151,9 +152,8 @@ __device__ __forceinline__ uint32_t scan_count(const ScanSeg& s) {
   return s.count_dev ? min(*s.count_dev, s.count_max) : s.count_max;
 }
 
-__global__ __launch_bounds__(kThreads) void k_scan_reduce(Launch<ScanSeg> L) {
-  __shared__ uint32_t ws[kThreads / 64];
-  const Where w = where(L);
+__device__ __forceinline__ void ph_scan_reduce(const Launch<ScanSeg>& L, const Where& w, uint8_t* lds) {
+  uint32_t* ws = reinterpret_cast<uint32_t*>(lds);
   const ScanSeg s = L.args[w.m].global();
   const uint32_t cnt = scan_count(s), ntiles = (cnt + kScanTile - 1) / kScanTile;
   for (uint32_t tile = w.blk; tile < ntiles; tile += w.nblk) {
@@ -170,9 +170,13 @@ __global__ __launch_bounds__(kThreads) void k_scan_reduce(Launch<ScanSeg> L) {
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_scan_top(Launch<ScanSeg> L) {
-  __shared__ uint32_t ws[kThreads / 64];
-  const Where w = where(L);
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(Launch<ScanSeg> L) {
+  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * (kThreads / 64)];
+  ph_scan_reduce(L, where(L), lds);
+}
+
+__device__ __forceinline__ void ph_scan_top(const Launch<ScanSeg>& L, const Where& w, uint8_t* lds) {
+  uint32_t* ws = reinterpret_cast<uint32_t*>(lds);
   const ScanSeg s = L.args[w.m].global();
   const uint32_t cnt = scan_count(s), ntiles = (cnt + kScanTile - 1) / kScanTile;
   uint32_t carry = 0;
@@ -187,9 +191,13 @@ __global__ __launch_bounds__(kThreads) void k_scan_top(Launch<ScanSeg> L) {
   if (threadIdx.x == 0 && s.total) *s.total = carry;
 }
 
-__global__ __launch_bounds__(kThreads) void k_scan_down(Launch<ScanSeg> L) {
-  __shared__ uint32_t ws[kThreads / 64];
-  const Where w = where(L);
+__global__ __launch_bounds__(kThreads) void k_scan_top(Launch<ScanSeg> L) {
+  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * (kThreads / 64)];
+  ph_scan_top(L, where(L), lds);
+}
+
+__device__ __forceinline__ void ph_scan_down(const Launch<ScanSeg>& L, const Where& w, uint8_t* lds) {
+  uint32_t* ws = reinterpret_cast<uint32_t*>(lds);
   const ScanSeg s = L.args[w.m].global();
   const uint32_t cnt = scan_count(s), ntiles = (cnt + kScanTile - 1) / kScanTile;
   for (uint32_t tile = w.blk; tile < ntiles; tile += w.nblk) {
@@ -210,11 +218,15 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(Launch<ScanSeg> L) {
   }
 }
 
+__global__ __launch_bounds__(kThreads) void k_scan_down(Launch<ScanSeg> L) {
+  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * (kThreads / 64)];
+  ph_scan_down(L, where(L), lds);
+}
+
 // A scan whose every segment fits one tile (count_max <= kScanTile): one block per
 // segment scans it and writes its total, in one launch instead of three.
-__global__ __launch_bounds__(kThreads) void k_scan_one(Launch<ScanSeg> L) {
-  __shared__ uint32_t ws[kThreads / 64];
-  const Where w = where(L);
+__device__ __forceinline__ void ph_scan_one(const Launch<ScanSeg>& L, const Where& w, uint8_t* lds) {
+  uint32_t* ws = reinterpret_cast<uint32_t*>(lds);
   const ScanSeg s = L.args[w.m].global();
   const uint32_t cnt = scan_count(s);
   const uint32_t base = 8u * threadIdx.x;
@@ -232,6 +244,11 @@ __global__ __launch_bounds__(kThreads) void k_scan_one(Launch<ScanSeg> L) {
   for (uint32_t j = 0; j < 8; j++)
     if (base + j < cnt) s.out[base + j] = off + v[j];
   if (threadIdx.x == 0 && s.total) *s.total = tot;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_one(Launch<ScanSeg> L) {
+  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * (kThreads / 64)];
+  ph_scan_one(L, where(L), lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -595,9 +612,8 @@ constexpr uint32_t kCompactPer = 32;  // leaf blocks per compaction block
 // The regions of kCompactPer consecutive leaf blocks land contiguously in the dense
 // list (their offsets are a prefix sum), so each thread copies entries by their
 // position in the group and finds the source region by a search in LDS.
-__global__ __launch_bounds__(kThreads) void k_compact(Launch<CompactArgs> L) {
-  __shared__ uint32_t s_pre[kCompactPer + 1];
-  const Where w = where(L);
+__device__ __forceinline__ void ph_compact(const Launch<CompactArgs>& L, const Where& w, uint8_t* lds) {
+  uint32_t* s_pre = reinterpret_cast<uint32_t*>(lds);
   const CompactArgs a = L.args[w.m].global();
   const uint32_t b0 = w.blk * kCompactPer, nb = min(kCompactPer, a.nblk - b0);
   if (threadIdx.x == 0) {
@@ -622,6 +638,11 @@ __global__ __launch_bounds__(kThreads) void k_compact(Launch<CompactArgs> L) {
   }
 }
 
+__global__ __launch_bounds__(kThreads) void k_compact(Launch<CompactArgs> L) {
+  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * (kCompactPer + 1)];
+  ph_compact(L, where(L), lds);
+}
+
 // Stage-0 sweep tile of a chunk: bottom-level chunks by position, depth D-1 leaves
 // by their position one level up.
 __device__ __forceinline__ uint64_t chunk_tile(const Geo& g, uint32_t R0, uint64_t c) {
@@ -639,8 +660,8 @@ __device__ __forceinline__ uint64_t chunk_tile(const Geo& g, uint32_t R0, uint64
 // first atomics would serialise on one address; after a reset, the heads of all
 // zero-filled runs meet there.  Consecutive entries with the same digest (one
 // zero-filled run) share the first one's table access.
-__global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
-  const Where w = where(L);
+__device__ __forceinline__ void ph_insert(const Launch<DenseArgs>& L, const Where& w, uint8_t* lds) {
+  (void)lds;
   const DenseArgs a = L.args[w.m].global();
   const uint64_t total = *a.total;
   const uint32_t lane = lane_id();
@@ -694,14 +715,18 @@ __global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
   if (lane == 0 && wc) atomicAdd(stripe(a.ctr, CTR_CLAIMS), (unsigned long long)wc);
 }
 
+__global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
+  ph_insert(L, where(L), nullptr);
+}
+
 // Once every claimer has stored its value: the mins of the leaves whose insert
 // matched an entry of this checkpoint (digests that several changed chunks share).
 // The first match below the entry's holder sees the holder's value and relabels it
 // a shifted duplicate (a claimer is labelled first by k_insert); later, lower
 // matches do the same to the match they displace, whose own label k_resolve then
 // decides from the final value either way.  Fixed leaves keep their label.
-__global__ __launch_bounds__(kThreads) void k_fixmin(Launch<DenseArgs> L) {
-  const Where w = where(L);
+__device__ __forceinline__ void ph_fixmin(const Launch<DenseArgs>& L, const Where& w, uint8_t* lds) {
+  (void)lds;
   const DenseArgs a = L.args[w.m].global();
   const uint64_t total = *a.total;
   for (uint64_t i = (uint64_t)w.blk * kThreads + threadIdx.x; i < total; i += (uint64_t)w.nblk * kThreads) {
@@ -722,10 +747,14 @@ __global__ __launch_bounds__(kThreads) void k_fixmin(Launch<DenseArgs> L) {
   }
 }
 
+__global__ __launch_bounds__(kThreads) void k_fixmin(Launch<DenseArgs> L) {
+  ph_fixmin(L, where(L), nullptr);
+}
+
 // Once every insert has landed: a pending leaf is FIRST when its digest's entry is
 // (cur, itself), SHIFT otherwise (include/reference_impl.hpp:52-69).
-__global__ __launch_bounds__(kThreads) void k_resolve(Launch<DenseArgs> L) {
-  const Where w = where(L);
+__device__ __forceinline__ void ph_resolve(const Launch<DenseArgs>& L, const Where& w, uint8_t* lds) {
+  (void)lds;
   const DenseArgs a = L.args[w.m].global();
   const uint64_t total = *a.total;
   for (uint64_t i = (uint64_t)w.blk * kThreads + threadIdx.x; i < total; i += (uint64_t)w.nblk * kThreads) {
@@ -737,6 +766,10 @@ __global__ __launch_bounds__(kThreads) void k_resolve(Launch<DenseArgs> L) {
     a.labels[c] = l;
     if (a.R0 && l == L_FIRST) a.tile_first[chunk_tile(a.g, a.R0, c)] = 1;
   }
+}
+
+__global__ __launch_bounds__(kThreads) void k_resolve(Launch<DenseArgs> L) {
+  ph_resolve(L, where(L), nullptr);
 }
 
 // Standalone chunk hashing (mtc_hash_chunks).
@@ -937,10 +970,9 @@ __device__ __forceinline__ void sweep_a_tile(const SweepArgs& a, uint64_t tile, 
 // Stage 0: a wavefront owns tiles_per_wave consecutive tiles, reads their
 // first-occurrence flags with one load per lane and walks only the flagged ones.
 template <int HK>
-__global__ __launch_bounds__(kThreads) void k_sweep_a(Launch<SweepArgs> L) {
-  __shared__ Digest rec_dg[kThreads / 64][256];
-  __shared__ uint32_t rec_node[kThreads / 64][256];
-  const Where wh = where(L);
+__device__ __forceinline__ void ph_sweep_a(const Launch<SweepArgs>& L, const Where& wh, uint8_t* lds) {
+  auto& rec_dg = *reinterpret_cast<Digest(*)[kThreads / 64][256]>(lds);
+  auto& rec_node = *reinterpret_cast<uint32_t(*)[kThreads / 64][256]>(lds + sizeof(Digest) * (kThreads / 64) * 256);
   const SweepArgs a = L.args[wh.m].global();
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
   const uint64_t ntiles = 1ull << a.d_out;
@@ -961,6 +993,12 @@ __global__ __launch_bounds__(kThreads) void k_sweep_a(Launch<SweepArgs> L) {
     todo &= todo - 1;
     sweep_a_tile<HK>(a, t0 + i, lane, rec_dg[w], rec_node[w]);
   }
+}
+
+template <int HK>
+__global__ __launch_bounds__(kThreads) void k_sweep_a(Launch<SweepArgs> L) {
+  __shared__ __align__(16) uint8_t lds[(sizeof(Digest) + sizeof(uint32_t)) * (kThreads / 64) * 256];
+  ph_sweep_a<HK>(L, where(L), lds);
 }
 
 // Sweep B: shifted-duplicate forest and region roots (include/reference_impl.hpp
@@ -1201,8 +1239,8 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
 
 // One wavefront per tile.
 template <int HK>
-__global__ __launch_bounds__(kThreads) void k_sweep_b(Launch<SweepArgs> L) {
-  const Where wh = where(L);
+__device__ __forceinline__ void ph_sweep_b(const Launch<SweepArgs>& L, const Where& wh, uint8_t* lds) {
+  (void)lds;
   const SweepArgs a = L.args[wh.m].global();
   const uint32_t lane = lane_id();
   const uint64_t tile = (uint64_t)wh.blk * (kThreads / 64) + (threadIdx.x >> 6);
@@ -1211,9 +1249,13 @@ __global__ __launch_bounds__(kThreads) void k_sweep_b(Launch<SweepArgs> L) {
   sweep_b_tile<HK>(a, tile, load_lab4(a, tile << R, T, lane), lane);
 }
 
+template <int HK>
+__global__ __launch_bounds__(kThreads) void k_sweep_b(Launch<SweepArgs> L) {
+  ph_sweep_b<HK>(L, where(L), nullptr);
+}
+
 // A single chunk: the leaf is the root.
-__global__ void k_single_leaf(Launch<SweepArgs> L) {
-  const Where w = where(L);
+__device__ __forceinline__ void ph_single_leaf(const Launch<SweepArgs>& L, const Where& w) {
   if (threadIdx.x != 0) return;
   const SweepArgs a = L.args[w.m].global();
   const uint8_t l = a.labels[0];
@@ -1223,12 +1265,14 @@ __global__ void k_single_leaf(Launch<SweepArgs> L) {
   a.cnt_s[0] = (root && l == L_SHIFT) ? 1u : 0u;
 }
 
+__global__ void k_single_leaf(Launch<SweepArgs> L) { ph_single_leaf(L, where(L)); }
+
 // ---------------------------------------------------------------------------
 // Root compaction: one thread per (depth, block) segment, in segment order.
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kThreads) void k_scatter_roots(Launch<RootArgs> L) {
-  const Where w = where(L);
+__device__ __forceinline__ void ph_scatter_roots(const Launch<RootArgs>& L, const Where& w, uint8_t* lds) {
+  (void)lds;
   const RootArgs a = L.args[w.m].global();
   const uint32_t seg = w.blk * kThreads + threadIdx.x;
   if (seg >= a.nseg) return;
@@ -1247,18 +1291,21 @@ __global__ __launch_bounds__(kThreads) void k_scatter_roots(Launch<RootArgs> L) 
   }
 }
 
+__global__ __launch_bounds__(kThreads) void k_scatter_roots(Launch<RootArgs> L) {
+  ph_scatter_roots(L, where(L), nullptr);
+}
+
 // ---------------------------------------------------------------------------
 // Shifted entries: source checkpoint and previous node from the table
 // (src/tree_approach.cpp:574-580,670-696), then a stable sort by source.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kLdsBins = 4096;  // source-checkpoint bins counted in LDS
 
-__global__ __launch_bounds__(kThreads) void k_shift_src(Launch<ShiftArgs> L) {
-  __shared__ uint32_t h[kLdsBins];
-  const Where w = where(L);
+__device__ __forceinline__ void ph_shift_src(const Launch<ShiftArgs>& L, const Where& w, uint8_t* lds) {
+  uint32_t* h = reinterpret_cast<uint32_t*>(lds);
   const ShiftArgs a = L.args[w.m].global();
-  const bool lds = a.nbins <= kLdsBins;
-  if (lds)
+  const bool in_lds = a.nbins <= kLdsBins;
+  if (in_lds)
     for (uint32_t b = threadIdx.x; b < a.nbins; b += kThreads) h[b] = 0;
   __syncthreads();
   const uint32_t S = *a.S;
@@ -1282,13 +1329,18 @@ __global__ __launch_bounds__(kThreads) void k_shift_src(Launch<ShiftArgs> L) {
     }
     a.key[i] = k;
     a.val[i] = ((uint64_t)prev << 32) | node;
-    if (lds) atomicAdd(&h[k], 1u);
+    if (in_lds) atomicAdd(&h[k], 1u);
     else atomicAdd(&a.src_hist[k], 1u);
   }
   __syncthreads();
-  if (lds)
+  if (in_lds)
     for (uint32_t b = threadIdx.x; b < a.nbins; b += kThreads)
       if (h[b]) atomicAdd(&a.src_hist[b], h[b]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_shift_src(Launch<ShiftArgs> L) {
+  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * kLdsBins];
+  ph_shift_src(L, where(L), lds);
 }
 
 // Stable counting sort by one 8-bit digit with a fixed grid: block g of G owns a
@@ -1302,9 +1354,8 @@ __device__ __forceinline__ void sort_range(uint32_t S, uint32_t g, uint32_t G, u
   hi = min(S, lo + per);
 }
 
-__global__ __launch_bounds__(kThreads) void k_sort_count(Launch<SortArgs> L) {
-  __shared__ uint32_t h[256];
-  const Where w = where(L);
+__device__ __forceinline__ void ph_sort_count(const Launch<SortArgs>& L, const Where& w, uint8_t* lds) {
+  uint32_t* h = reinterpret_cast<uint32_t*>(lds);
   const SortArgs a = L.args[w.m].global();
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -1315,10 +1366,14 @@ __global__ __launch_bounds__(kThreads) void k_sort_count(Launch<SortArgs> L) {
   if (threadIdx.x < a.nb) a.hist[threadIdx.x * w.nblk + w.blk] = h[threadIdx.x];
 }
 
-__global__ __launch_bounds__(kThreads) void k_sort_move(Launch<SortArgs> L) {
-  __shared__ uint32_t cnt[kThreads / 64][256];
-  __shared__ uint32_t run[256];
-  const Where wh = where(L);
+__global__ __launch_bounds__(kThreads) void k_sort_count(Launch<SortArgs> L) {
+  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * 256];
+  ph_sort_count(L, where(L), lds);
+}
+
+__device__ __forceinline__ void ph_sort_move(const Launch<SortArgs>& L, const Where& wh, uint8_t* lds) {
+  auto& cnt = *reinterpret_cast<uint32_t(*)[kThreads / 64][256]>(lds);
+  uint32_t* run = reinterpret_cast<uint32_t*>(lds + sizeof(uint32_t) * (kThreads / 64) * 256);
   const SortArgs a = L.args[wh.m].global();
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint64_t lt = (1ull << lane) - 1;
@@ -1365,12 +1420,16 @@ __global__ __launch_bounds__(kThreads) void k_sort_move(Launch<SortArgs> L) {
   }
 }
 
+__global__ __launch_bounds__(kThreads) void k_sort_move(Launch<SortArgs> L) {
+  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * ((kThreads / 64) * 256 + 256)];
+  ph_sort_move(L, where(L), lds);
+}
+
 // (chkpt, count) table in ascending checkpoint id (src/tree_approach.cpp:657-665).
 
 // The (chkpt, count) table (src/tree_approach.cpp:657-665): the source checkpoints
 // with at least one shifted entry, ascending.  One whole block.
-__device__ void build_ptable(const PtableArgs& a) {
-  __shared__ uint32_t s_ws[kThreads / 64];
+__device__ void build_ptable(const PtableArgs& a, uint32_t* s_ws) {
   uint32_t base = 0;
   for (uint32_t b0 = 0; b0 < a.nbins; b0 += kThreads) {
     const uint32_t b = b0 + threadIdx.x;
@@ -1386,13 +1445,18 @@ __device__ void build_ptable(const PtableArgs& a) {
   if (threadIdx.x == 0) a.ctl->P = base;
 }
 
-__global__ __launch_bounds__(kThreads) void k_region_sizes(Launch<RegionArgs> L) {
-  const Where w = where(L);
+__device__ __forceinline__ void ph_region_sizes(const Launch<RegionArgs>& L, const Where& w, uint8_t* lds) {
+  uint32_t* s_ws = reinterpret_cast<uint32_t*>(lds);
   const RegionArgs a = L.args[w.m].global();
-  if (w.blk == 0) build_ptable(a.pt);
+  if (w.blk == 0) build_ptable(a.pt, s_ws);
   const uint32_t F = *a.F;
   for (uint32_t i = w.blk * kThreads + threadIdx.x; i < F; i += w.nblk * kThreads)
     a.sizes[i] = leaf_count(a.g, a.first_roots[i]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_region_sizes(Launch<RegionArgs> L) {
+  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * (kThreads / 64)];
+  ph_region_sizes(L, where(L), lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1435,15 +1499,15 @@ __device__ void diff_meta(const DiffArgs& a, const Where& w) {
 // block 0's own metadata thread has written it), then the chunks of every
 // first-occurrence region are gathered, regions in root order and leaves in
 // ascending chunk order; the short last chunk is zero padded (:636-650).
-__global__ __launch_bounds__(kThreads) void k_assemble(Launch<DiffArgs> L) {
+__device__ __forceinline__ void ph_assemble(const Launch<DiffArgs>& L, const Where& wh, uint8_t* lds) {
+  uint32_t* s_src = reinterpret_cast<uint32_t*>(lds);
+  unsigned long long* s_fold = reinterpret_cast<unsigned long long*>(lds + sizeof(uint32_t) * kThreads);
   static_assert(kThreads == kStripes, "one block folds the striped counters");
-  __shared__ uint32_t s_src[kThreads];
-  const Where wh = where(L);
   const DiffArgs a = L.args[wh.m].global();
   diff_meta(a, wh);
   if (wh.blk == 0) {
     __syncthreads();  // the header thread's control fields
-    finish_member(a.ctr, a.changed, a.ctl, a.out);
+    finish_member(a.ctr, a.changed, a.ctl, a.out, s_fold);
   }
   const uint32_t F = a.ctl->F;
   if (F == 0) return;
@@ -1490,14 +1554,19 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Launch<DiffArgs> L) {
   }
 }
 
+__global__ __launch_bounds__(kThreads) void k_assemble(Launch<DiffArgs> L) {
+  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * kThreads + sizeof(unsigned long long) * CTR_KINDS];
+  ph_assemble(L, where(L), lds);
+}
+
 // Diff of the basic approach (src/basic_approach.cpp:54-142): the header (F = the
 // changed chunks, P = S = 0), their chunk indices, then their bytes, C each, the
 // short last chunk zero padded.  The reference orders the indices as its atomics
 // land; here they are ascending.  Block 0 also folds the counters and copies the
 // control block out, as k_assemble does for tree diffs.
 
-__global__ __launch_bounds__(kThreads) void k_basic_diff(Launch<BasicDiffArgs> L) {
-  const Where wh = where(L);
+__device__ __forceinline__ void ph_basic_diff(const Launch<BasicDiffArgs>& L, const Where& wh, uint8_t* lds) {
+  unsigned long long* s_fold = reinterpret_cast<unsigned long long*>(lds);
   const BasicDiffArgs a = L.args[wh.m].global();
   const uint32_t F = *a.total;
   uint32_t* out = reinterpret_cast<uint32_t*>(a.diff);
@@ -1522,7 +1591,7 @@ __global__ __launch_bounds__(kThreads) void k_basic_diff(Launch<BasicDiffArgs> L
       a.ctl->diff_size = 32 + 4ull * F + (uint64_t)F * a.C;
     }
     __syncthreads();
-    finish_member(a.ctr, a.total, a.ctl, a.out);
+    finish_member(a.ctr, a.total, a.ctl, a.out, s_fold);
   }
   uint8_t* dst0 = a.diff + 32 + 4ull * F;
   const uint32_t C = a.C;
@@ -1541,6 +1610,128 @@ __global__ __launch_bounds__(kThreads) void k_basic_diff(Launch<BasicDiffArgs> L
       }
     } else {
       dst[w] = (so + w < a.len) ? a.data[so + w] : 0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_basic_diff(Launch<BasicDiffArgs> L) {
+  __shared__ __align__(16) uint8_t lds[sizeof(unsigned long long) * CTR_KINDS];
+  ph_basic_diff(L, where(L), lds);
+}
+
+// ---------------------------------------------------------------------------
+// The post-leaf pipeline as one persistent launch.  Every launch of a step after
+// the leaf kernel becomes a phase: its blocks are work items, numbered phase by
+// phase into one ticket sequence.  A resident block takes the next ticket, waits
+// until its member's previous phase has finished every item (a per-member count
+// in memory), runs the item with the phase's body -- the same code as the
+// phase's own kernel -- and adds one to its member's count of the phase.
+//
+// Tickets are taken in order, so a block only ever waits for items with smaller
+// tickets, every one of which a running block already holds; the earliest
+// unfinished item never waits.  The launch therefore finishes whatever number of
+// its blocks fit on the GPU at once, next to other kernels.  Members are
+// independent, so a member's next phase starts as soon as its own previous phase
+// is done, while other members are still in earlier phases.
+// ---------------------------------------------------------------------------
+#ifndef MTC_PIPE_WAVES
+#define MTC_PIPE_WAVES 4  // 128 VGPRs: four blocks per CU
+#endif
+constexpr uint32_t kPipeLds = (sizeof(Digest) + sizeof(uint32_t)) * (kThreads / 64) * 256;  // the largest body's
+constexpr long long kPipeWaitTicks = 200000000;  // 2 s of the 100 MHz wall clock: a wait that long aborts the step
+
+template <class A>
+__device__ __forceinline__ Launch<A> phase_launch(const PipeArgs& P, const PhaseRec& R) {
+  return Launch<A>{reinterpret_cast<const A*>(P.base + R.args), reinterpret_cast<const uint32_t*>(P.base + R.prefix),
+                   R.count > 1 ? reinterpret_cast<const uint32_t*>(P.base + R.group) : nullptr, R.count};
+}
+
+__device__ __forceinline__ void run_phase(const PipeArgs& P, const PhaseRec& R, const Where& w, uint8_t* lds) {
+  switch (R.kind) {
+    case PK_SCAN_REDUCE: ph_scan_reduce(phase_launch<ScanSeg>(P, R), w, lds); break;
+    case PK_SCAN_TOP: ph_scan_top(phase_launch<ScanSeg>(P, R), w, lds); break;
+    case PK_SCAN_DOWN: ph_scan_down(phase_launch<ScanSeg>(P, R), w, lds); break;
+    case PK_SCAN_ONE: ph_scan_one(phase_launch<ScanSeg>(P, R), w, lds); break;
+    case PK_COMPACT: ph_compact(phase_launch<CompactArgs>(P, R), w, lds); break;
+    case PK_INSERT: ph_insert(phase_launch<DenseArgs>(P, R), w, lds); break;
+    case PK_FIXMIN: ph_fixmin(phase_launch<DenseArgs>(P, R), w, lds); break;
+    case PK_RESOLVE: ph_resolve(phase_launch<DenseArgs>(P, R), w, lds); break;
+    case PK_SINGLE_LEAF: ph_single_leaf(phase_launch<SweepArgs>(P, R), w); break;
+    case PK_SWEEP_A_M3: ph_sweep_a<HASH_MURMUR3>(phase_launch<SweepArgs>(P, R), w, lds); break;
+    case PK_SWEEP_A_MD5: ph_sweep_a<HASH_MD5>(phase_launch<SweepArgs>(P, R), w, lds); break;
+    case PK_SWEEP_B_M3: ph_sweep_b<HASH_MURMUR3>(phase_launch<SweepArgs>(P, R), w, lds); break;
+    case PK_SWEEP_B_MD5: ph_sweep_b<HASH_MD5>(phase_launch<SweepArgs>(P, R), w, lds); break;
+    case PK_SCATTER_ROOTS: ph_scatter_roots(phase_launch<RootArgs>(P, R), w, lds); break;
+    case PK_SHIFT_SRC: ph_shift_src(phase_launch<ShiftArgs>(P, R), w, lds); break;
+    case PK_SORT_COUNT: ph_sort_count(phase_launch<SortArgs>(P, R), w, lds); break;
+    case PK_SORT_MOVE: ph_sort_move(phase_launch<SortArgs>(P, R), w, lds); break;
+    case PK_REGION_SIZES: ph_region_sizes(phase_launch<RegionArgs>(P, R), w, lds); break;
+    case PK_ASSEMBLE: ph_assemble(phase_launch<DiffArgs>(P, R), w, lds); break;
+    case PK_BASIC_DIFF: ph_basic_diff(phase_launch<BasicDiffArgs>(P, R), w, lds); break;
+    default: break;
+  }
+}
+
+__global__ __launch_bounds__(kThreads, MTC_PIPE_WAVES) void k_pipeline(PipeArgs P) {
+  __shared__ __align__(16) uint8_t lds[kPipeLds];
+  __shared__ uint32_t s_t, s_p, s_skip;
+  while (true) {
+    if (threadIdx.x == 0) {
+      const uint32_t t = atomicAdd(&P.ctl[0], 1u);
+      uint32_t p = 0;
+      if (t < P.total)
+        while (P.ph[p].first + P.ph[p].items <= t) p++;
+      s_t = t;
+      s_p = p;
+    }
+    __syncthreads();
+    const uint32_t t = s_t, p = s_p;
+    if (t >= P.total) break;  // block-uniform: every later ticket is past the end too
+    const PhaseRec R = P.ph[p];
+    const uint32_t b = t - R.first;
+    uint32_t m = 0;
+    if (R.count > 1) {
+      const uint32_t* group = reinterpret_cast<const uint32_t*>(P.base + R.group);
+      const uint32_t* prefix = reinterpret_cast<const uint32_t*>(P.base + R.prefix);
+      m = group[b >> 6];
+      while (prefix[m + 1] <= b) m++;
+    }
+    const uint32_t* prefix = reinterpret_cast<const uint32_t*>(P.base + R.prefix);
+    const Where w{m, b - prefix[m], prefix[m + 1] - prefix[m]};
+    const uint32_t mem = reinterpret_cast<const uint32_t*>(P.base + R.owner)[m];
+    const uint32_t cell = mem * P.nph + p;
+    if (threadIdx.x == 0) {
+      const uint32_t q = P.wait[cell];
+      if (q == ~0u) {
+        atomicMin(&P.stamp[3 * mem], (unsigned long long)wall_clock64());
+      } else {
+        const uint32_t want = P.need[mem * P.nph + q];
+        const uint32_t* cnt = &P.done[mem * P.nph + q];
+        const long long t0 = wall_clock64();
+        while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
+          if (__hip_atomic_load(&P.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+          if (wall_clock64() - t0 > kPipeWaitTicks) {
+            __hip_atomic_store(&P.ctl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      // After an abort no item runs: its inputs may be incomplete.
+      s_skip = __hip_atomic_load(&P.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (!s_skip) run_phase(P, R, w, lds);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      const uint32_t d = atomicAdd(&P.done[cell], 1u) + 1;
+      if (d == P.need[cell]) {
+        const unsigned long long now = (unsigned long long)wall_clock64();
+        if (p == P.sweeps_end[mem]) P.stamp[3 * mem + 1] = now;
+        atomicMax(&P.stamp[3 * mem + 2], now);
+      }
     }
   }
 }
@@ -1924,7 +2115,7 @@ struct HostTrace {
 namespace {
 
 struct LaunchRec {
-  uint64_t args = 0, prefix = 0, group = 0;
+  uint64_t args = 0, prefix = 0, group = 0, owner = 0;
   uint32_t count = 0, grid = 0;
 };
 
@@ -1937,8 +2128,11 @@ struct ArenaImage {
     if (size) std::memcpy(bytes.data() + off, p, size);
     return off;
   }
+  // One batched launch: an argument block and a block count per launch member, and
+  // the step member that owns each launch member (the pipeline tracks progress per
+  // step member).
   template <class A>
-  LaunchRec add(const std::vector<A>& args, const std::vector<uint32_t>& blocks) {
+  LaunchRec add(const std::vector<A>& args, const std::vector<uint32_t>& blocks, const std::vector<uint32_t>& owners) {
     LaunchRec r;
     r.count = (uint32_t)args.size();
     std::vector<uint32_t> prefix(r.count + 1, 0);
@@ -1946,6 +2140,7 @@ struct ArenaImage {
     r.grid = prefix[r.count];
     r.args = put(args.data(), sizeof(A) * args.size());
     r.prefix = put(prefix.data(), sizeof(uint32_t) * prefix.size());
+    r.owner = put(owners.data(), sizeof(uint32_t) * owners.size());
     if (r.count > 1) {
       std::vector<uint32_t> group((r.grid + 63) / 64);
       uint32_t m = 0;
@@ -1970,6 +2165,7 @@ struct Member {
   const uint8_t* data;
   uint64_t len;
   bool baseline;
+  uint32_t idx = 0;        // position in the step
   Ctl* ctl_out = nullptr;  // this member's entry of the step's result array
   int arena = 0;           // diff arena this step writes
   // derived
@@ -1985,18 +2181,30 @@ struct Member {
   uint32_t* ws_reg = nullptr;
 };
 
-// Recorded launches of a step: run eagerly or captured into a graph.
+// Recorded launches of a step: run eagerly, captured into a graph, or (the
+// phases after the leaf kernel) run as one persistent pipeline launch.
 struct Ops {
   std::vector<std::function<void(uint8_t*, hipStream_t)>> fn;
   std::vector<uint64_t> sig;
+  std::vector<uint32_t> kind;  // PhaseKind of each launch (PK_NONE: not a pipeline phase)
+  std::vector<LaunchRec> rec;
   template <class A, class K>
-  void add(uint32_t tag, const LaunchRec& r, K launch) {
+  void add(uint32_t tag, uint32_t pk, const LaunchRec& r, K launch) {
     if (r.grid == 0) return;
     sig.push_back(((uint64_t)tag << 32) | r.grid);
     sig.push_back((r.args << 32) ^ (r.prefix << 8) ^ r.group ^ r.count);
     fn.push_back([r, launch](uint8_t* base, hipStream_t st) { launch(bind<A>(r, base), r.grid, st); });
+    kind.push_back(pk);
+    rec.push_back(r);
   }
 };
+
+// Owners of a launch whose argument list holds one block per step member.
+static std::vector<uint32_t> owners_of(const std::vector<Member>& ms) {
+  std::vector<uint32_t> o;
+  for (const auto& M : ms) o.push_back(M.idx);
+  return o;
+}
 
 }  // namespace
 
@@ -2067,25 +2275,34 @@ static void prepare_member(Member& M, hipStream_t st) {
   }
 }
 
-static void add_scan(Ops& ops, ArenaImage& img, const std::vector<ScanSeg>& segs) {
+static void add_scan(Ops& ops, ArenaImage& img, const std::vector<ScanSeg>& segs, const std::vector<uint32_t>& owners) {
   if (segs.empty()) return;
   std::vector<uint32_t> tiles, ones(segs.size(), 1);
   if (std::all_of(segs.begin(), segs.end(), [](const ScanSeg& s) { return s.count_max <= kScanTile; })) {
-    ops.add<ScanSeg>(13, img.add(segs, ones), [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_one<<<g, kThreads, 0, st>>>(L); });
+    ops.add<ScanSeg>(13, PK_SCAN_ONE, img.add(segs, ones, owners), [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_one<<<g, kThreads, 0, st>>>(L); });
     return;
   }
   for (const auto& s : segs) tiles.push_back(std::max<uint32_t>(1, std::min<uint32_t>(512, (s.count_max + kScanTile - 1) / kScanTile)));
-  const LaunchRec rt = img.add(segs, tiles), r1 = img.add(segs, ones);
-  ops.add<ScanSeg>(10, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_reduce<<<g, kThreads, 0, st>>>(L); });
-  ops.add<ScanSeg>(11, r1, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_top<<<g, kThreads, 0, st>>>(L); });
-  ops.add<ScanSeg>(12, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_down<<<g, kThreads, 0, st>>>(L); });
+  const LaunchRec rt = img.add(segs, tiles, owners), r1 = img.add(segs, ones, owners);
+  ops.add<ScanSeg>(10, PK_SCAN_REDUCE, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_reduce<<<g, kThreads, 0, st>>>(L); });
+  ops.add<ScanSeg>(11, PK_SCAN_TOP, r1, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_top<<<g, kThreads, 0, st>>>(L); });
+  ops.add<ScanSeg>(12, PK_SCAN_DOWN, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_down<<<g, kThreads, 0, st>>>(L); });
 }
 
-// Builds the step's launch sequence for the members: phase 1 (fills, leaf kernel,
-// scan of the leaf blocks' counts), phase 2 (the rest), or both (phase 0).
-// Phase 2 of tree members: table passes, both forests, roots and the diff.
+// The scan of the leaf blocks' counts (the offsets of each block's list of
+// chunks to insert, and their total).
+static void add_leaf_scan(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
+  std::vector<ScanSeg> ss;
+  for (auto& M : ms)
+    ss.push_back(ScanSeg{M.h->blk_cnt.p, M.h->blk_off.p, nullptr, M.nblk_leaf, M.h->blk_off.p + M.nblk_leaf, M.ws_blk});
+  add_scan(ops, img, ss, owners_of(ms));
+}
+
+// After the leaf kernel, for tree members: table passes, both forests, roots and
+// the diff.
 static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
   const size_t B = ms.size();
+  const std::vector<uint32_t> own = owners_of(ms);
   // 3. dense table passes over the changed chunks
   {
     std::vector<CompactArgs> ca;
@@ -2101,11 +2318,11 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
                              h->tile_flags.p + M.ntile0});
       db.push_back(grid_for(M.n, 8 * kThreads, 16384));
     }
-    ops.add<CompactArgs>(20, img.add(ca, cb), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
-    const LaunchRec rd = img.add(da, db);
-    ops.add<DenseArgs>(22, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_insert<<<g, kThreads, 0, st>>>(L); });
-    ops.add<DenseArgs>(24, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_fixmin<<<g, kThreads, 0, st>>>(L); });
-    ops.add<DenseArgs>(23, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_resolve<<<g, kThreads, 0, st>>>(L); });
+    ops.add<CompactArgs>(20, PK_COMPACT, img.add(ca, cb, own), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
+    const LaunchRec rd = img.add(da, db, own);
+    ops.add<DenseArgs>(22, PK_INSERT, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_insert<<<g, kThreads, 0, st>>>(L); });
+    ops.add<DenseArgs>(24, PK_FIXMIN, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_fixmin<<<g, kThreads, 0, st>>>(L); });
+    ops.add<DenseArgs>(23, PK_RESOLVE, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_resolve<<<g, kThreads, 0, st>>>(L); });
   }
   // 4. sweeps: every stage of sweep A (and its inserts) before any stage of sweep B
   {
@@ -2142,14 +2359,18 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       for (size_t i = 0; i < B; i++)
         if (ms[i].n == 1) sb[i] = 1, any = true;
       if (any)
-        ops.add<SweepArgs>(30, img.add(base, sb), [](Launch<SweepArgs> L, uint32_t g, hipStream_t st) { k_single_leaf<<<g, 64, 0, st>>>(L); });
+        ops.add<SweepArgs>(30, PK_SINGLE_LEAF, img.add(base, sb, own), [](Launch<SweepArgs> L, uint32_t g, hipStream_t st) { k_single_leaf<<<g, 64, 0, st>>>(L); });
     }
     for (int pass = 0; pass < 2; pass++) {
       for (size_t s = 0; s < max_stages; s++) {
-        std::map<int, std::pair<std::vector<SweepArgs>, std::vector<uint32_t>>> byhk;
+        struct Group {
+          std::vector<SweepArgs> args;
+          std::vector<uint32_t> blocks, owners;
+        };
+        std::map<int, Group> byhk;
         for (size_t i = 0; i < B; i++) {
           mtc_dedup* h = ms[i].h;
-          auto& slot = byhk[h->hash_kind];
+          Group& slot = byhk[h->hash_kind];
           SweepArgs a = base[i];
           uint32_t blocks = 0;
           if (s < h->stages.size()) {
@@ -2170,20 +2391,21 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
             const uint64_t waves = ((1ull << a.d_out) + a.tiles_per_wave - 1) / a.tiles_per_wave;
             blocks = (uint32_t)((waves + 3) / 4);
           }
-          slot.first.push_back(a);
-          slot.second.push_back(blocks);
+          slot.args.push_back(a);
+          slot.blocks.push_back(blocks);
+          slot.owners.push_back(ms[i].idx);
         }
         for (auto& kv : byhk) {
           const int hk = kv.first;
           const uint32_t tag = 40 + 4 * (uint32_t)pass + (uint32_t)hk;
-          const LaunchRec r = img.add(kv.second.first, kv.second.second);
+          const LaunchRec r = img.add(kv.second.args, kv.second.blocks, kv.second.owners);
           if (pass == 0)
-            ops.add<SweepArgs>(tag, r, [hk](Launch<SweepArgs> L, uint32_t g, hipStream_t st) {
+            ops.add<SweepArgs>(tag, hk == HASH_MD5 ? PK_SWEEP_A_MD5 : PK_SWEEP_A_M3, r, [hk](Launch<SweepArgs> L, uint32_t g, hipStream_t st) {
               if (hk == HASH_MD5) k_sweep_a<HASH_MD5><<<g, kThreads, 0, st>>>(L);
               else k_sweep_a<HASH_MURMUR3><<<g, kThreads, 0, st>>>(L);
             });
           else
-            ops.add<SweepArgs>(tag, r, [hk](Launch<SweepArgs> L, uint32_t g, hipStream_t st) {
+            ops.add<SweepArgs>(tag, hk == HASH_MD5 ? PK_SWEEP_B_MD5 : PK_SWEEP_B_M3, r, [hk](Launch<SweepArgs> L, uint32_t g, hipStream_t st) {
               if (hk == HASH_MD5) k_sweep_b<HASH_MD5><<<g, kThreads, 0, st>>>(L);
               else k_sweep_b<HASH_MURMUR3><<<g, kThreads, 0, st>>>(L);
             });
@@ -2194,12 +2416,15 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
   // 5. root totals and compaction (segments are (depth, block) in node-id order)
   {
     std::vector<ScanSeg> ss;
+    std::vector<uint32_t> so;
     for (auto& M : ms) {
       mtc_dedup* h = M.h;
       ss.push_back(ScanSeg{h->seg_cnt_f.p, h->seg_off_f.p, nullptr, M.nseg, &h->ctl_d->F, M.ws_f});
       ss.push_back(ScanSeg{h->seg_cnt_s.p, h->seg_off_s.p, nullptr, M.nseg, &h->ctl_d->S, M.ws_s});
+      so.push_back(M.idx);
+      so.push_back(M.idx);
     }
-    add_scan(ops, img, ss);
+    add_scan(ops, img, ss, so);
     std::vector<RootArgs> ra;
     std::vector<uint32_t> rb;
     for (auto& M : ms) {
@@ -2208,7 +2433,7 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
                             M.g.D + 1, M.nseg, h->first_roots.p, h->shift_tmp.p});
       rb.push_back(grid_for(M.nseg));
     }
-    ops.add<RootArgs>(50, img.add(ra, rb), [](Launch<RootArgs> L, uint32_t g, hipStream_t st) { k_scatter_roots<<<g, kThreads, 0, st>>>(L); });
+    ops.add<RootArgs>(50, PK_SCATTER_ROOTS, img.add(ra, rb, own), [](Launch<RootArgs> L, uint32_t g, hipStream_t st) { k_scatter_roots<<<g, kThreads, 0, st>>>(L); });
   }
   // 6. shifted entries: source checkpoint, stable sort by source, (chkpt, count) table
   std::vector<const uint64_t*> sorted(B);
@@ -2225,11 +2450,12 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       max_passes = std::max(max_passes, M.passes);
       sorted[i] = h->sort_val[M.passes % 2].p;
     }
-    ops.add<ShiftArgs>(60, img.add(sa, sb), [](Launch<ShiftArgs> L, uint32_t g, hipStream_t st) { k_shift_src<<<g, kThreads, 0, st>>>(L); });
+    ops.add<ShiftArgs>(60, PK_SHIFT_SRC, img.add(sa, sb, own), [](Launch<ShiftArgs> L, uint32_t g, hipStream_t st) { k_shift_src<<<g, kThreads, 0, st>>>(L); });
     for (uint32_t p = 0; p < max_passes; p++) {
       std::vector<SortArgs> so;
       std::vector<uint32_t> sblk;
       std::vector<ScanSeg> hs;
+      std::vector<uint32_t> hs_own;
       for (auto& M : ms) {
         mtc_dedup* h = M.h;
         uint32_t bits = 0;
@@ -2239,12 +2465,15 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
         so.push_back(SortArgs{h->sort_key[p % 2].p, h->sort_val[p % 2].p, h->sort_key[1 - p % 2].p,
                               h->sort_val[1 - p % 2].p, &h->ctl_d->S, shift, nb, h->sort_hist.p});
         sblk.push_back(p < M.passes ? M.Gs : 0u);
-        if (p < M.passes) hs.push_back(ScanSeg{h->sort_hist.p, h->sort_hist.p, nullptr, nb * M.Gs, nullptr, M.ws_sort});
+        if (p < M.passes) {
+          hs.push_back(ScanSeg{h->sort_hist.p, h->sort_hist.p, nullptr, nb * M.Gs, nullptr, M.ws_sort});
+          hs_own.push_back(M.idx);
+        }
       }
-      const LaunchRec r = img.add(so, sblk);
-      ops.add<SortArgs>(61, r, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_count<<<g, kThreads, 0, st>>>(L); });
-      add_scan(ops, img, hs);
-      ops.add<SortArgs>(62, r, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_move<<<g, kThreads, 0, st>>>(L); });
+      const LaunchRec r = img.add(so, sblk, own);
+      ops.add<SortArgs>(61, PK_SORT_COUNT, r, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_count<<<g, kThreads, 0, st>>>(L); });
+      add_scan(ops, img, hs, hs_own);
+      ops.add<SortArgs>(62, PK_SORT_MOVE, r, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_move<<<g, kThreads, 0, st>>>(L); });
     }
   }
   // 7. region sizes and offsets, metadata, gathered data, counters
@@ -2259,8 +2488,8 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       rb.push_back(grid_for(M.n, 16 * kThreads, 256));
       ss.push_back(ScanSeg{h->region_off.p, h->region_off.p, &h->ctl_d->F, M.n, h->region_off.p + M.n, M.ws_reg});
     }
-    ops.add<RegionArgs>(70, img.add(ra, rb), [](Launch<RegionArgs> L, uint32_t g, hipStream_t st) { k_region_sizes<<<g, kThreads, 0, st>>>(L); });
-    add_scan(ops, img, ss);
+    ops.add<RegionArgs>(70, PK_REGION_SIZES, img.add(ra, rb, own), [](Launch<RegionArgs> L, uint32_t g, hipStream_t st) { k_region_sizes<<<g, kThreads, 0, st>>>(L); });
+    add_scan(ops, img, ss, own);
     std::vector<DiffArgs> da;
     std::vector<uint32_t> gb;
     for (size_t i = 0; i < B; i++) {
@@ -2288,13 +2517,15 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       da.push_back(d);
       gb.push_back(grid_for(M.n, 8 * kThreads, 8192));
     }
-    ops.add<DiffArgs>(72, img.add(da, gb), [](Launch<DiffArgs> L, uint32_t g, hipStream_t st) { k_assemble<<<g, kThreads, 0, st>>>(L); });
+    ops.add<DiffArgs>(72, PK_ASSEMBLE, img.add(da, gb, own), [](Launch<DiffArgs> L, uint32_t g, hipStream_t st) { k_assemble<<<g, kThreads, 0, st>>>(L); });
   }
 }
 
-// Phase 2 of basic members (src/basic_approach.cpp:14-142): the diff of the
-// changed chunks, whose ascending list the leaf pass and compaction left.
+// After the leaf kernel, for basic members (src/basic_approach.cpp:14-142): the
+// diff of the changed chunks, whose ascending list the leaf pass and compaction
+// leave.
 static void build_basic_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
+  const std::vector<uint32_t> own = owners_of(ms);
   // the leaf blocks' list regions, concatenated in block order (ascending)
   {
     std::vector<CompactArgs> ca;
@@ -2304,7 +2535,7 @@ static void build_basic_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) 
       ca.push_back(CompactArgs{h->list.p, h->blk_cnt.p, h->blk_off.p, M.tpb, h->dense.p, M.nblk_leaf});
       cb.push_back((M.nblk_leaf + kCompactPer - 1) / kCompactPer);
     }
-    ops.add<CompactArgs>(21, img.add(ca, cb), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
+    ops.add<CompactArgs>(21, PK_COMPACT, img.add(ca, cb, own), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
   }
   std::vector<BasicDiffArgs> da;
   std::vector<uint32_t> gb;
@@ -2327,46 +2558,78 @@ static void build_basic_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) 
     da.push_back(d);
     gb.push_back(grid_for(M.n, 8 * kThreads, 8192));
   }
-  ops.add<BasicDiffArgs>(80, img.add(da, gb), [](Launch<BasicDiffArgs> L, uint32_t g, hipStream_t st) { k_basic_diff<<<g, kThreads, 0, st>>>(L); });
+  ops.add<BasicDiffArgs>(80, PK_BASIC_DIFF, img.add(da, gb, own), [](Launch<BasicDiffArgs> L, uint32_t g, hipStream_t st) { k_basic_diff<<<g, kThreads, 0, st>>>(L); });
 }
 
-static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int phase) {
-  const size_t B = ms.size();
-  if (phase != 2) {
-  // 1. per-step zero fills (counters, tile flags, root counts, histograms, debug views)
-  {
-    std::vector<FillSeg> fs;
-    std::vector<uint32_t> fb;
-    auto fill = [&](void* p, uint64_t words, uint32_t v) {
-      if (!words) return;
-      fs.push_back(FillSeg{reinterpret_cast<uint32_t*>(p), words, v});
-      fb.push_back(grid_for(words, kThreads * 4, 4096));
-    };
-    for (auto& M : ms) {
-      mtc_dedup* h = M.h;
-      fill(h->ctr.p, 2ull * CTR_KINDS * kStripes, 0);
-      fill(h->tile_flags.p, (2 * M.ntile0 + 3) / 4, 0);
-      fill(h->seg_cnt_f.p, M.nseg, 0);
-      fill(h->seg_cnt_s.p, M.nseg, 0);
-      fill(h->src_hist.p, M.nbins, 0);
-      fill(h->blk_cnt.p, M.nblk_leaf, 0);  // counts that k_leaf1 accumulates
-      if (h->debug) {
-        fill(h->dbg_tree.p, 4 * (2ull * M.n - 1), 0);
-        fill(h->dbg_labels.p, (2ull * M.n - 1 + 3) / 4, 0);
-      }
-    }
-    ops.add<FillSeg>(1, img.add(fs, fb), [](Launch<FillSeg> L, uint32_t g, hipStream_t st) { k_fill<<<g, kThreads, 0, st>>>(L); });
+// Words of the pipeline's counters for a step: ticket and abort flag, per-member
+// per-phase counts, then (8-byte aligned) three 64-bit time stamps per member.
+static uint64_t pipe_words(size_t members, uint32_t max_phases) {
+  return ((4 + (uint64_t)members * max_phases + 1) & ~1ull) + 6 * members;
+}
+
+// Most phases a step of these members can have (sizes the pipeline's counters,
+// which the step's first fill clears before the phases are built).
+static uint32_t max_phases(const std::vector<Member>& ms) {
+  uint32_t stages = 0, passes = 0;
+  for (const auto& M : ms) {
+    stages = std::max<uint32_t>(stages, (uint32_t)M.h->stages.size());
+    passes = std::max(passes, M.passes);
   }
-  // 2. leaf kernel, one launch per (hash, staged chunk size) group
-  {
-    // (hash, staged chunk size, one tile per block with 128-byte chunks)
+  // leaf scan 3, table passes 4, single leaf 1, two sweeps per hash per stage,
+  // root scans 3 + scatter 1, shift 1, per sort pass count + scan 3 + move, region
+  // 1 + scan 3, assemble 1, basic compact and diff 2
+  return 3 + 4 + 1 + 4 * stages + 4 + 1 + 5 * passes + 4 + 1 + 2;
+}
+
+// The step's launch sequence.  Phase 1: the fills (counters, control blocks, tile
+// flags, histograms, debug views, the pipeline's counters) and the leaf kernels,
+// plus the scan of the leaf blocks' counts when `leaf_scan` (steps that read the
+// count of chunks to insert before phase 2).  Phase 2: the rest.
+static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int phase, bool leaf_scan,
+                      uint32_t* pipe, uint64_t pipe_len) {
+  const size_t B = ms.size();
+  if (phase == 1) {
+    {
+      std::vector<FillSeg> fs;
+      std::vector<uint32_t> fb, fo;
+      auto fill = [&](void* p, uint64_t words, uint32_t v, uint32_t owner) {
+        if (!words) return;
+        fs.push_back(FillSeg{reinterpret_cast<uint32_t*>(p), words, v});
+        fb.push_back(grid_for(words, kThreads * 4, 4096));
+        fo.push_back(owner);
+      };
+      for (auto& M : ms) {
+        mtc_dedup* h = M.h;
+        fill(h->ctl_d, sizeof(Ctl) / 4, 0, M.idx);
+        fill(h->ctr.p, 2ull * CTR_KINDS * kStripes, 0, M.idx);
+        fill(h->tile_flags.p, (2 * M.ntile0 + 3) / 4, 0, M.idx);
+        fill(h->seg_cnt_f.p, M.nseg, 0, M.idx);
+        fill(h->seg_cnt_s.p, M.nseg, 0, M.idx);
+        fill(h->src_hist.p, M.nbins, 0, M.idx);
+        fill(h->blk_cnt.p, M.nblk_leaf, 0, M.idx);  // counts that k_leaf1 accumulates
+        if (h->debug) {
+          fill(h->dbg_tree.p, 4 * (2ull * M.n - 1), 0, M.idx);
+          fill(h->dbg_labels.p, (2ull * M.n - 1 + 3) / 4, 0, M.idx);
+        }
+      }
+      if (pipe) {  // counters and stamps 0; the start stamps all ones (an atomic min)
+        const uint64_t stamps = pipe_len - 6 * B;
+        fill(pipe, stamps, 0, 0);
+        for (size_t i = 0; i < B; i++) {
+          fill(pipe + stamps + 6 * i, 2, ~0u, 0);
+          fill(pipe + stamps + 6 * i + 2, 4, 0, 0);
+        }
+      }
+      ops.add<FillSeg>(1, PK_NONE, img.add(fs, fb, fo), [](Launch<FillSeg> L, uint32_t g, hipStream_t st) { k_fill<<<g, kThreads, 0, st>>>(L); });
+    }
+    // leaf kernel, one launch per (hash, staged chunk size, single tile) group
     std::map<std::tuple<int, int, bool>, std::vector<size_t>> groups;
     for (size_t i = 0; i < B; i++)
       groups[{ms[i].h->hash_kind, ms[i].cb,
               ms[i].cb >= 16 && ms[i].tpb == 1 && ms[i].h->approach != MTC_APPROACH_BASIC}].push_back(i);
     for (auto& kv : groups) {
       std::vector<LeafArgs> la;
-      std::vector<uint32_t> lb;
+      std::vector<uint32_t> lb, lo;
       for (size_t i : kv.second) {
         Member& M = ms[i];
         mtc_dedup* h = M.h;
@@ -2375,11 +2638,12 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
         la.push_back(LeafArgs{M.data, M.len, M.n, h->chunk_size, h->leaf_dg.p, h->labels.p, reset,
                               h->list.p, h->blk_cnt.p, M.tpb});
         lb.push_back(M.nblk_leaf);
+        lo.push_back(M.idx);
       }
       const int hk = std::get<0>(kv.first), cb = std::get<1>(kv.first);
       const bool lean = std::get<2>(kv.first);
       const uint32_t tag = 100 + (uint32_t)hk * 32 + (lean ? 16u : 0u) + (uint32_t)(cb ? __builtin_ctz((unsigned)cb) : 0);
-      ops.add<LeafArgs>(tag, img.add(la, lb), [hk, cb, lean](Launch<LeafArgs> L, uint32_t g, hipStream_t st) {
+      ops.add<LeafArgs>(tag, PK_NONE, img.add(la, lb, lo), [hk, cb, lean](Launch<LeafArgs> L, uint32_t g, hipStream_t st) {
         if (lean) {
           if (hk == HASH_MD5) launch_leaf1<HASH_MD5>(cb, L, g, st);
           else launch_leaf1<HASH_MURMUR3>(cb, L, g, st);
@@ -2390,19 +2654,75 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
         }
       });
     }
+    if (leaf_scan) add_leaf_scan(ms, ops, img);
+    return;
   }
-  {
-    std::vector<ScanSeg> ss;
-    for (auto& M : ms)
-      ss.push_back(ScanSeg{M.h->blk_cnt.p, M.h->blk_off.p, nullptr, M.nblk_leaf, M.h->blk_off.p + M.nblk_leaf, M.ws_blk});
-    add_scan(ops, img, ss);
-  }
-  }
-  if (phase == 1) return;
+  if (!leaf_scan) add_leaf_scan(ms, ops, img);
   std::vector<Member> tree, basic;
   for (auto& M : ms) (M.h->approach == MTC_APPROACH_BASIC ? basic : tree).push_back(M);
   if (!tree.empty()) build_tree_ops(tree, ops, img);
   if (!basic.empty()) build_basic_ops(basic, ops, img);
+}
+
+// Appends to the arena the phase table of one pipeline launch over ops (every op a
+// phase, in order) and returns its arguments (base and table pointers are filled
+// in at launch, once the arena's address is known).  sweeps_end[m] is the last
+// phase of member m that builds its labels and roots.
+struct PipePlan {
+  PipeArgs args{};
+  uint64_t ph = 0, wait = 0, need = 0, sweeps_end = 0;
+  uint32_t grid = 0;
+};
+
+static uint32_t pipeline_blocks() {
+  static int cached = 0;
+  if (!cached) {
+    int dev = 0, cus = 0, per_cu = 0;
+    MTC_HIP(hipGetDevice(&dev));
+    MTC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    MTC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pipeline, kThreads, 0));
+    cached = std::max(1, cus) * std::max(1, std::min(per_cu, 8));
+  }
+  return (uint32_t)cached;
+}
+
+static PipePlan plan_pipeline(const Ops& ops, ArenaImage& img, size_t B, uint32_t* pipe, uint32_t max_ph) {
+  PipePlan plan;
+  const uint32_t nph = (uint32_t)ops.fn.size();
+  if (nph > max_ph) throw Status{MTC_ERR_STATE, "pipeline phase count above its bound"};
+  std::vector<PhaseRec> ph(nph);
+  std::vector<uint32_t> need((size_t)B * nph, 0), wait((size_t)B * nph, ~0u), last(B, ~0u), sweeps(B, ~0u);
+  uint32_t first = 0;
+  for (uint32_t p = 0; p < nph; p++) {
+    const LaunchRec& r = ops.rec[p];
+    ph[p] = PhaseRec{ops.kind[p], r.grid, first, r.count, r.args, r.prefix, r.group, r.owner};
+    first += r.grid;
+    // blocks per step member
+    std::vector<uint32_t> prefix(r.count + 1), owner(r.count);
+    std::memcpy(prefix.data(), img.bytes.data() + r.prefix, 4ull * (r.count + 1));
+    std::memcpy(owner.data(), img.bytes.data() + r.owner, 4ull * r.count);
+    for (uint32_t m = 0; m < r.count; m++) need[(size_t)owner[m] * nph + p] += prefix[m + 1] - prefix[m];
+    for (size_t i = 0; i < B; i++) {
+      if (!need[i * nph + p]) continue;
+      wait[i * nph + p] = last[i];
+      last[i] = p;
+      const uint32_t k = ops.kind[p];
+      if (k == PK_SWEEP_B_M3 || k == PK_SWEEP_B_MD5 || k == PK_SINGLE_LEAF) sweeps[i] = p;
+    }
+  }
+  plan.ph = img.put(ph.data(), sizeof(PhaseRec) * ph.size());
+  plan.wait = img.put(wait.data(), 4 * wait.size());
+  plan.need = img.put(need.data(), 4 * need.size());
+  plan.sweeps_end = img.put(sweeps.data(), 4 * sweeps.size());
+  PipeArgs& a = plan.args;
+  a.nph = nph;
+  a.nmem = (uint32_t)B;
+  a.total = first;
+  a.ctl = pipe;
+  a.done = pipe + 4;
+  a.stamp = reinterpret_cast<unsigned long long*>(pipe + pipe_words(B, max_ph) - 6 * B);
+  plan.grid = std::max<uint32_t>(1, std::min<uint32_t>(first, pipeline_blocks()));
+  return plan;
 }
 
 void StepCtx::reset_graph() {
@@ -2415,9 +2735,9 @@ StepCtx::~StepCtx() {
   reset_graph();
   if (arena_h) (void)hipHostFree(arena_h);
   if (lead_h) (void)hipHostFree(lead_h);
-  if (pre_h) (void)hipHostFree(pre_h);
   if (ctl_h) (void)hipHostFree(ctl_h);
   if (cnt_h) (void)hipHostFree(cnt_h);
+  if (stamp_h) (void)hipHostFree(stamp_h);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
 }
@@ -2439,39 +2759,40 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
   }
 }
 
-// use_graph: replay the step's launches from a captured graph when their shape
-// matches the previous step's.
+// MTC_PIPELINE=launches runs the phases after the leaf kernel as one launch each
+// instead of one persistent launch (the results are the same).
+static bool pipeline_enabled() {
+  const char* e = std::getenv("MTC_PIPELINE");
+  return !(e && std::strcmp(e, "launches") == 0);
+}
+
+static uint32_t wall_clock_khz() {
+  static int khz = 0;
+  if (!khz) {
+    int dev = 0;
+    MTC_HIP(hipGetDevice(&dev));
+    MTC_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    if (khz <= 0) khz = 100000;
+  }
+  return (uint32_t)khz;
+}
+
+// A step in two parts.  Part 1 (fills and leaf kernels) runs at once from its own
+// arena; part 2 is built while the leaf kernels stream.  Part 2 is one persistent
+// pipeline launch (or, with MTC_PIPELINE=launches, one launch per phase, replayed
+// from a graph when use_graph and the launch shape repeats).  A step whose table
+// might overflow pauses between the parts for the exact bound (the scan of the
+// leaf blocks' counts then runs in part 1).
 static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_graph) {
   HostTrace ht;
   const size_t B = ms.size();
   if (!ctx.ev[0])
     for (auto& e : ctx.ev) MTC_HIP(hipEventCreate(&e));
   MTC_HIP(hipEventRecord(ctx.ev[0], st));
-  // Control blocks are cleared before any table work of the step (a rehash
-  // reports into them).
-  {
-    ArenaImage pre;
-    std::vector<FillSeg> fs;
-    std::vector<uint32_t> fb;
-    for (auto& M : ms) {
-      fs.push_back(FillSeg{reinterpret_cast<uint32_t*>(M.h->ctl_d), sizeof(Ctl) / 4, 0});
-      fb.push_back(1);
-    }
-    const LaunchRec r = pre.add(fs, fb);
-    ctx.pre_d.ensure(pre.bytes.size());
-    if (ctx.pre_h_cap < pre.bytes.size()) {
-      if (ctx.pre_h) (void)hipHostFree(ctx.pre_h);
-      ctx.pre_h = nullptr;
-      MTC_HIP(hipHostMalloc((void**)&ctx.pre_h, pre.bytes.size()));
-      ctx.pre_h_cap = pre.bytes.size();
-    }
-    MTC_HIP(hipStreamSynchronize(st));  // the pinned staging may still feed the previous copy
-    std::memcpy(ctx.pre_h, pre.bytes.data(), pre.bytes.size());
-    MTC_HIP(hipMemcpyAsync(ctx.pre_d.p, ctx.pre_h, pre.bytes.size(), hipMemcpyHostToDevice, st));
-    k_fill<<<r.grid, kThreads, 0, st>>>(bind<FillSeg>(r, ctx.pre_d.p));
-    MTC_HIP(hipGetLastError());
+  for (size_t i = 0; i < B; i++) {
+    ms[i].idx = (uint32_t)i;
+    prepare_member(ms[i], st);
   }
-  for (auto& M : ms) prepare_member(M, st);
   // Reset tables of baseline members (basic members keep no table).
   for (auto& M : ms) {
     if (!M.baseline || M.h->approach == MTC_APPROACH_BASIC) continue;
@@ -2485,10 +2806,13 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
 
   const bool debug = std::any_of(ms.begin(), ms.end(), [](const Member& M) { return M.h->debug != 0; });
   const bool split = std::any_of(ms.begin(), ms.end(), [](const Member& M) { return M.split; });
+  const bool pipe = pipeline_enabled();
   ctx.ctl_all.ensure(B);
   for (size_t i = 0; i < B; i++) ms[i].ctl_out = ctx.ctl_all.p + i;
-  // Uploads the arena image and runs the launches, replayed from a graph or eagerly
-  // (the leaf launches always run eagerly, timed on their own: ev[1] .. ev[4]).
+  const uint32_t max_ph = max_phases(ms);
+  const uint64_t pipe_len = pipe_words(B, max_ph);
+  if (pipe) ctx.pipe.reserve(pipe_len);
+  // Uploads an arena image and runs its launches, replayed from a graph or eagerly.
   auto run_ops = [&](Ops& ops, ArenaImage& img, bool graph, DevBuf<uint8_t>& arena_d, uint8_t*& arena_h,
                      uint64_t& arena_h_cap) {
     const uint64_t bytes = img.bytes.size();
@@ -2522,37 +2846,19 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
     for (size_t i = 0; i < ops.fn.size(); i++) {
       ops.fn[i](base, st);
       MTC_HIP(hipGetLastError());
-      const bool leaf = (ops.sig[2 * i] >> 32) >= 100;
-      if (leaf && (i + 1 == ops.fn.size() || (ops.sig[2 * i + 2] >> 32) < 100))
-        MTC_HIP(hipEventRecord(ctx.ev[4], st));
-      if (i == 0 && (ops.sig[0] >> 32) == 1) MTC_HIP(hipEventRecord(ctx.ev[1], st));
+      if (ops.kind[i] == PK_SWEEP_B_M3 || ops.kind[i] == PK_SWEEP_B_MD5 || ops.kind[i] == PK_SINGLE_LEAF)
+        MTC_HIP(hipEventRecord(ctx.ev[2], st));
     }
   };
-  if (!split && use_graph && !debug) {
-    // The leading launches (fills, leaf kernels, scan of their counts) run at once,
-    // from their own arena; the rest is built while the leaf kernels stream and is
-    // replayed from a graph.
+  // Part 1: fills and leaf kernels.
+  {
     Ops ops1;
     ArenaImage img1;
-    build_ops(ms, ops1, img1, 1);
+    build_ops(ms, ops1, img1, 1, split, pipe ? ctx.pipe.p : nullptr, pipe_len);
     run_ops(ops1, img1, false, ctx.lead_d, ctx.lead_h, ctx.lead_h_cap);
-    Ops ops2;
-    ArenaImage img2;
-    build_ops(ms, ops2, img2, 2);
-    ht.mark("build");
-    run_ops(ops2, img2, true, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
-  } else if (!split) {
-    Ops ops;
-    ArenaImage img;
-    build_ops(ms, ops, img, 0);
-    ht.mark("build");
-    run_ops(ops, img, false, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
-  } else {
-    // Phase 1, then the exact table bound of every member, then phase 2.
-    Ops ops1;
-    ArenaImage img1;
-    build_ops(ms, ops1, img1, 1);
-    run_ops(ops1, img1, false, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
+    MTC_HIP(hipEventRecord(ctx.ev[4], st));
+  }
+  if (split) {  // the exact table bound of every member, then part 2
     if (ctx.cnt_h_cap < B) {
       if (ctx.cnt_h) (void)hipHostFree(ctx.cnt_h);
       ctx.cnt_h = nullptr;
@@ -2566,10 +2872,36 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
     ht.mark("phase 1");
     for (size_t i = 0; i < B; i++)
       if (ms[i].split) grow_exact(ms[i].h, ctx.cnt_h[i], st);
+  }
+  // Part 2.
+  bool ev2 = false;
+  {
     Ops ops2;
     ArenaImage img2;
-    build_ops(ms, ops2, img2, 2);
-    run_ops(ops2, img2, false, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
+    build_ops(ms, ops2, img2, 2, split, nullptr, 0);
+    if (pipe) {
+      const PipePlan plan = plan_pipeline(ops2, img2, B, ctx.pipe.p, max_ph);
+      Ops one;
+      LaunchRec none;
+      none.grid = plan.grid;
+      one.add<FillSeg>(90, PK_NONE, none, [](Launch<FillSeg>, uint32_t, hipStream_t) {});
+      one.fn[0] = [plan](uint8_t* base, hipStream_t s2) {
+        PipeArgs a = plan.args;
+        a.base = base;
+        a.ph = reinterpret_cast<const PhaseRec*>(base + plan.ph);
+        a.wait = reinterpret_cast<const uint32_t*>(base + plan.wait);
+        a.need = reinterpret_cast<const uint32_t*>(base + plan.need);
+        a.sweeps_end = reinterpret_cast<const uint32_t*>(base + plan.sweeps_end);
+        k_pipeline<<<plan.grid, kThreads, 0, s2>>>(a);
+      };
+      one.sig.push_back(plan.args.total);
+      ht.mark("build");
+      run_ops(one, img2, use_graph && !debug, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
+    } else {
+      ht.mark("build");
+      run_ops(ops2, img2, use_graph && !debug, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
+      ev2 = !(use_graph && !debug);
+    }
   }
   if (debug)
     for (auto& M : ms)
@@ -2580,18 +2912,33 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
   ht.mark("launch");
   if (ctx.ctl_h_cap < B) {
     if (ctx.ctl_h) (void)hipHostFree(ctx.ctl_h);
+    if (ctx.stamp_h) (void)hipHostFree(ctx.stamp_h);
     ctx.ctl_h = nullptr;
+    ctx.stamp_h = nullptr;
     MTC_HIP(hipHostMalloc((void**)&ctx.ctl_h, B * sizeof(Ctl)));
+    MTC_HIP(hipHostMalloc((void**)&ctx.stamp_h, B * 3 * sizeof(unsigned long long) + 16));
     ctx.ctl_h_cap = (uint32_t)B;
   }
   MTC_HIP(hipMemcpyAsync(ctx.ctl_h, ctx.ctl_all.p, B * sizeof(Ctl), hipMemcpyDeviceToHost, st));
+  if (pipe) {
+    MTC_HIP(hipMemcpyAsync(ctx.stamp_h, ctx.pipe.p + pipe_len - 6 * B, B * 3 * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, st));
+    MTC_HIP(hipMemcpyAsync(reinterpret_cast<uint8_t*>(ctx.stamp_h) + B * 3 * sizeof(unsigned long long),
+                           ctx.pipe.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  }
   MTC_HIP(hipEventRecord(ctx.ev[3], st));
   MTC_HIP(hipStreamSynchronize(st));
   ht.mark("sync");
-  float ms_setup = 0, ms_total = 0, ms_leaf = 0;
+  float ms_setup = 0, ms_total = 0, ms_leaf = 0, ms_sweeps = 0;
   MTC_HIP(hipEventElapsedTime(&ms_setup, ctx.ev[0], ctx.ev[1]));
   MTC_HIP(hipEventElapsedTime(&ms_total, ctx.ev[1], ctx.ev[3]));
   MTC_HIP(hipEventElapsedTime(&ms_leaf, ctx.ev[1], ctx.ev[4]));
+  if (ev2) MTC_HIP(hipEventElapsedTime(&ms_sweeps, ctx.ev[1], ctx.ev[2]));
+  if (pipe) {
+    uint32_t aborted = 0;
+    std::memcpy(&aborted, reinterpret_cast<uint8_t*>(ctx.stamp_h) + B * 3 * sizeof(unsigned long long), 4);
+    if (aborted) throw Status{MTC_ERR_HIP, "the step's pipeline waited over 2 s for a phase and stopped"};
+  }
   // MTC_TEST_FAIL_MEMBER=i makes member i report an inconsistent state (tests of
   // the failure path; nothing else reads it).
   const char* inject = std::getenv("MTC_TEST_FAIL_MEMBER");
@@ -2605,6 +2952,7 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
       throw Status{MTC_ERR_STATE, "member " + std::to_string(i) + ": inconsistent table state (err bits " +
                                       std::to_string(e) + ")"};
   }
+  const double khz = pipe ? (double)wall_clock_khz() : 1.0;
   for (size_t i = 0; i < B; i++) {
     mtc_dedup* h = ms[i].h;
     const Ctl& c = ctx.ctl_h[i];
@@ -2615,9 +2963,25 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
     h->last_diff_size = c.diff_size;
     h->diff_cur = ms[i].arena;
     h->last_header = mtc_header{h->baseline_id, ms[i].cur, ms[i].len, h->chunk_size, c.F, c.P, c.S};
+    // Timers (src/tree_approach.cpp:1640,1657,1672): set-up; labels and forests
+    // (the leaf kernels with the part-1 fills, then the pipeline up to the last
+    // sweep); the diff (roots compaction to assembly).
+    float dedup = ms_total, collect = 0;
+    if (pipe) {
+      const unsigned long long* t = ctx.stamp_h + 3 * i;
+      const double t0 = (double)t[0], t1 = (double)t[1], t2 = (double)t[2];
+      if (t[0] != ~0ull && t2 >= t0) {
+        const double mid = t[1] ? t1 : t0;
+        dedup = ms_leaf + (float)((mid - t0) / khz);
+        collect = (float)((t2 - mid) / khz);
+      }
+    } else if (ev2) {
+      dedup = ms_sweeps;
+      collect = ms_total - ms_sweeps;
+    }
     h->last_ms[0] = ms_setup;
-    h->last_ms[1] = ms_total;
-    h->last_ms[2] = 0;
+    h->last_ms[1] = dedup;
+    h->last_ms[2] = collect;
     h->last_ms[3] = 0;
     h->last_ms[4] = ms_leaf;
   }

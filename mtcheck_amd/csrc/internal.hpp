@@ -105,15 +105,15 @@ void exclusive_scan_u32_fixed(const uint32_t* in, uint32_t* out, uint64_t count,
 // uploaded argument arena, the step's result array and, when steps are replayed
 // from a graph, the captured graph and the launch shape it was captured for.
 struct StepCtx {
-  DevBuf<uint8_t> arena_d, pre_d, lead_d;
-  uint8_t* arena_h = nullptr;  // pinned staging of the arena
+  DevBuf<uint8_t> arena_d, lead_d;
+  uint8_t* arena_h = nullptr;  // pinned staging of the arena of the step's second part
   uint64_t arena_h_cap = 0;
-  uint8_t* lead_h = nullptr;  // pinned staging of the leading launches' arena (graph steps)
+  uint8_t* lead_h = nullptr;  // pinned staging of the arena of its first part (fills, leaf kernels)
   uint64_t lead_h_cap = 0;
-  uint8_t* pre_h = nullptr;
-  uint64_t pre_h_cap = 0;
+  DevBuf<uint32_t> pipe;  // counters of the persistent pipeline launch
   DevBuf<Ctl> ctl_all;
   Ctl* ctl_h = nullptr;  // pinned
+  unsigned long long* stamp_h = nullptr;  // pinned: the pipeline's time stamps per member, then its abort flag
   uint32_t ctl_h_cap = 0;
   uint32_t* cnt_h = nullptr;  // pinned: chunks to insert per member (steps that pause after the leaf kernel)
   uint32_t cnt_h_cap = 0;

@@ -10,7 +10,6 @@ require that no launch breaks that rule, and they check the host-side error
 paths (limits, refused steps, failed steps).  CPU only.
 """
 import ctypes
-import os
 
 import pytest
 
@@ -87,8 +86,18 @@ def test_checker_flags_the_unallocated_table_reset(fake):
     L.fakehip_reset()
 
 
+@pytest.fixture(params=["pipeline", "launches"])
+def mode(request, monkeypatch):
+    """The phases after the leaf kernel as one persistent launch, or one launch each."""
+    if request.param == "launches":
+        monkeypatch.setenv("MTC_PIPELINE", "launches")
+    else:
+        monkeypatch.delenv("MTC_PIPELINE", raising=False)
+    return request.param
+
+
 @pytest.mark.parametrize("C,size", [(128, 1000 * 128 + 5), (64, 64 * 64), (16, 16 * 3 + 1), (100, 100 * 777)])
-def test_basic_steps_launch_in_bounds(fake, C, size):
+def test_basic_steps_launch_in_bounds(fake, mode, C, size):
     L = fake
     L.fakehip_reset()
     h = create(L, C, BASIC)
@@ -98,7 +107,7 @@ def test_basic_steps_launch_in_bounds(fake, C, size):
         assert rc == OK, L.mtc_last_error(h)
         assert (hdr.ref_id, hdr.chkpt_id) == ((3 if k >= 3 else 0), k)
     names = launches(L)
-    assert any("k_basic_diff" in n for n in names)
+    assert any("basic_diff" in n for n in names)
     assert not any("k_fill_u64" in n for n in names), "a basic handle keeps no table to reset"
     assert violations(L) == []
     L.mtc_destroy(h)
@@ -113,7 +122,7 @@ def test_basic_steps_launch_in_bounds(fake, C, size):
     (100, 100 * 777, 1, (2, 0.125, 5)),
     (4096, 4096 * 300, 0, None),
 ])
-def test_tree_steps_launch_in_bounds(fake, C, size, hk, tuning):
+def test_tree_steps_launch_in_bounds(fake, mode, C, size, hk, tuning):
     L = fake
     L.fakehip_reset()
     h = create(L, C, TREE, hk)
@@ -125,12 +134,12 @@ def test_tree_steps_launch_in_bounds(fake, C, size, hk, tuning):
             L.mtc_set_debug(h, 1)
         rc, hdr = checkpoint(L, h, data, size, baseline=(k == 0))
         assert rc == OK, L.mtc_last_error(h)
-    assert any("k_assemble" in n for n in launches(L))
+    assert any("assemble" in n for n in launches(L))
     assert violations(L) == []
     L.mtc_destroy(h)
 
 
-def test_batch_of_tree_and_basic_members_with_graph_replay(fake):
+def test_batch_of_tree_and_basic_members_with_graph_replay(fake, mode):
     L = fake
     L.fakehip_reset()
     b = ctypes.c_void_p()
@@ -150,7 +159,7 @@ def test_batch_of_tree_and_basic_members_with_graph_replay(fake):
         rc = L.mtc_checkpoint_batch(b, H, D, N, base, B, 1, None, hdrs, sizes)
         assert rc == OK, L.mtc_batch_last_error(b)
         assert [x.chkpt_id for x in hdrs] == [k] * B
-    assert any("k_basic_diff" in n for n in launches(L))
+    assert any("basic_diff" in n for n in launches(L))
     assert violations(L) == []
     for h in hs:
         L.mtc_destroy(h)
