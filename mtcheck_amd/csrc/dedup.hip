@@ -1672,67 +1672,83 @@ __device__ __forceinline__ void run_phase(const PipeArgs& P, const PhaseRec& R, 
   }
 }
 
+// Thread 0's part of an iteration: count the block's previous item done, take
+// the next ticket, find its phase, item and member, and wait for the member's
+// previous phase.  Returns (ticket, phase) and whether the item is to be skipped.
+struct Ticket {
+  uint32_t t, p, m, blk, nblk, mem, skip;
+};
+
+__device__ __forceinline__ Ticket pipe_next(const PipeArgs& P, uint32_t prev_cell, uint32_t prev_p, uint32_t prev_mem) {
+  if (prev_cell != ~0u) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const uint32_t d = atomicAdd(&P.done[prev_cell], 1u) + 1;
+    if (d == P.need[prev_cell]) {
+      const unsigned long long now = (unsigned long long)wall_clock64();
+      if (prev_p == P.sweeps_end[prev_mem]) P.stamp[3 * prev_mem + 1] = now;
+      atomicMax(&P.stamp[3 * prev_mem + 2], now);
+    }
+  }
+  Ticket k{atomicAdd(&P.ctl[0], 1u), 0, 0, 0, 0, 0, 0};
+  if (k.t >= P.total) return k;
+  while (P.ph[k.p].first + P.ph[k.p].items <= k.t) k.p++;
+  const PhaseRec& R = P.ph[k.p];
+  const uint32_t b = k.t - R.first;
+  const uint32_t* prefix = reinterpret_cast<const uint32_t*>(P.base + R.prefix);
+  if (R.count > 1) {
+    k.m = reinterpret_cast<const uint32_t*>(P.base + R.group)[b >> 6];
+    while (prefix[k.m + 1] <= b) k.m++;
+  }
+  k.blk = b - prefix[k.m];
+  k.nblk = prefix[k.m + 1] - prefix[k.m];
+  k.mem = reinterpret_cast<const uint32_t*>(P.base + R.owner)[k.m];
+  const uint32_t q = P.wait[k.mem * P.nph + k.p];
+  if (q == ~0u) {
+    atomicMin(&P.stamp[3 * k.mem], (unsigned long long)wall_clock64());
+  } else {
+    const uint32_t want = P.need[k.mem * P.nph + q];
+    const uint32_t* cnt = &P.done[k.mem * P.nph + q];
+    const long long t0 = wall_clock64();
+    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
+      if (__hip_atomic_load(&P.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (wall_clock64() - t0 > kPipeWaitTicks) {
+        __hip_atomic_store(&P.ctl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  // After an abort no item runs: its inputs may be incomplete.
+  k.skip = __hip_atomic_load(&P.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return k;
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+// One iteration per item: thread 0 finishes the previous item and fetches the
+// next, a barrier publishes the ticket, the block runs the item, a barrier ends
+// it.  Every value that steers the loop is read back with readfirstlane, so the
+// compiler sees uniform control flow around the barriers (an exit it took for
+// divergent let it re-enter the item body without thread 0).
 __global__ __launch_bounds__(kThreads, MTC_PIPE_WAVES) void k_pipeline(PipeArgs P) {
   __shared__ __align__(16) uint8_t lds[kPipeLds];
-  __shared__ uint32_t s_t, s_p, s_skip;
-  while (true) {
-    if (threadIdx.x == 0) {
-      const uint32_t t = atomicAdd(&P.ctl[0], 1u);
-      uint32_t p = 0;
-      if (t < P.total)
-        while (P.ph[p].first + P.ph[p].items <= t) p++;
-      s_t = t;
-      s_p = p;
-    }
+  __shared__ Ticket s_k;
+  uint32_t prev_cell = ~0u, prev_p = 0, prev_mem = 0;  // meaningful in thread 0 only
+  for (;;) {
+    if (threadIdx.x == 0) s_k = pipe_next(P, prev_cell, prev_p, prev_mem);
     __syncthreads();
-    const uint32_t t = s_t, p = s_p;
-    if (t >= P.total) break;  // block-uniform: every later ticket is past the end too
-    const PhaseRec R = P.ph[p];
-    const uint32_t b = t - R.first;
-    uint32_t m = 0;
-    if (R.count > 1) {
-      const uint32_t* group = reinterpret_cast<const uint32_t*>(P.base + R.group);
-      const uint32_t* prefix = reinterpret_cast<const uint32_t*>(P.base + R.prefix);
-      m = group[b >> 6];
-      while (prefix[m + 1] <= b) m++;
-    }
-    const uint32_t* prefix = reinterpret_cast<const uint32_t*>(P.base + R.prefix);
-    const Where w{m, b - prefix[m], prefix[m + 1] - prefix[m]};
-    const uint32_t mem = reinterpret_cast<const uint32_t*>(P.base + R.owner)[m];
-    const uint32_t cell = mem * P.nph + p;
-    if (threadIdx.x == 0) {
-      const uint32_t q = P.wait[cell];
-      if (q == ~0u) {
-        atomicMin(&P.stamp[3 * mem], (unsigned long long)wall_clock64());
-      } else {
-        const uint32_t want = P.need[mem * P.nph + q];
-        const uint32_t* cnt = &P.done[mem * P.nph + q];
-        const long long t0 = wall_clock64();
-        while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
-          if (__hip_atomic_load(&P.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-          if (wall_clock64() - t0 > kPipeWaitTicks) {
-            __hip_atomic_store(&P.ctl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(2);
-        }
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      // After an abort no item runs: its inputs may be incomplete.
-      s_skip = __hip_atomic_load(&P.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    const uint32_t t = uniform(s_k.t);
+    if (t >= P.total) break;
+    const uint32_t p = uniform(s_k.p);
+    const uint32_t mem = uniform(s_k.mem);
+    const Where w{uniform(s_k.m), uniform(s_k.blk), uniform(s_k.nblk)};
+    const uint32_t skip = uniform(s_k.skip);
+    if (!skip) run_phase(P, P.ph[p], w, lds);
     __syncthreads();
-    if (!s_skip) run_phase(P, R, w, lds);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      const uint32_t d = atomicAdd(&P.done[cell], 1u) + 1;
-      if (d == P.need[cell]) {
-        const unsigned long long now = (unsigned long long)wall_clock64();
-        if (p == P.sweeps_end[mem]) P.stamp[3 * mem + 1] = now;
-        atomicMax(&P.stamp[3 * mem + 2], now);
-      }
-    }
+    prev_cell = mem * P.nph + p;
+    prev_p = p;
+    prev_mem = mem;
   }
 }
 
