@@ -261,33 +261,4 @@ struct BasicDiffArgs {
   }
 };
 
-// The phases of the persistent pipeline launch (k_pipeline): the launches of a
-// step after the leaf kernel, each run by its own body.
-enum PhaseKind : uint32_t {
-  PK_SCAN_REDUCE, PK_SCAN_TOP, PK_SCAN_DOWN, PK_SCAN_ONE,
-  PK_COMPACT, PK_INSERT, PK_FIXMIN, PK_RESOLVE, PK_SINGLE_LEAF,
-  PK_SWEEP_A_M3, PK_SWEEP_A_MD5, PK_SWEEP_B_M3, PK_SWEEP_B_MD5,
-  PK_SCATTER_ROOTS, PK_SHIFT_SRC, PK_SORT_COUNT, PK_SORT_MOVE,
-  PK_REGION_SIZES, PK_ASSEMBLE, PK_BASIC_DIFF,
-  PK_NONE
-};
-
-// One phase; offsets are bytes from the step arena's base.
-struct PhaseRec {
-  uint32_t kind, items, first, count;  // body, blocks, first ticket, launch members
-  uint64_t args, prefix, group, owner;  // the phase's Launch tables, and the step member of each launch member
-};
-
-struct PipeArgs {
-  const uint8_t* base;       // the step arena
-  const PhaseRec* ph;
-  uint32_t nph, nmem, total;  // phases, step members, tickets
-  const uint32_t* sweeps_end; // [nmem]: the phase after which a member's labels and roots are final (~0u: none)
-  const uint32_t* wait;       // [nmem][nph]: phase whose items must all be done first (~0u: none)
-  const uint32_t* need;       // [nmem][nph]: items of the member in the phase
-  uint32_t* ctl;              // [0] ticket counter, [1] abort flag; zeroed before the launch
-  uint32_t* done;             // [nmem][nph] finished items; zeroed before the launch
-  unsigned long long* stamp;  // [nmem][3]: first item started (filled with ones), sweeps done, all done
-};
-
 }  // namespace mtc

@@ -1619,139 +1619,6 @@ __global__ __launch_bounds__(kThreads) void k_basic_diff(Launch<BasicDiffArgs> L
   ph_basic_diff(L, where(L), lds);
 }
 
-// ---------------------------------------------------------------------------
-// The post-leaf pipeline as one persistent launch.  Every launch of a step after
-// the leaf kernel becomes a phase: its blocks are work items, numbered phase by
-// phase into one ticket sequence.  A resident block takes the next ticket, waits
-// until its member's previous phase has finished every item (a per-member count
-// in memory), runs the item with the phase's body -- the same code as the
-// phase's own kernel -- and adds one to its member's count of the phase.
-//
-// Tickets are taken in order, so a block only ever waits for items with smaller
-// tickets, every one of which a running block already holds; the earliest
-// unfinished item never waits.  The launch therefore finishes whatever number of
-// its blocks fit on the GPU at once, next to other kernels.  Members are
-// independent, so a member's next phase starts as soon as its own previous phase
-// is done, while other members are still in earlier phases.
-// ---------------------------------------------------------------------------
-#ifndef MTC_PIPE_WAVES
-#define MTC_PIPE_WAVES 4  // 128 VGPRs: four blocks per CU
-#endif
-constexpr uint32_t kPipeLds = (sizeof(Digest) + sizeof(uint32_t)) * (kThreads / 64) * 256;  // the largest body's
-constexpr long long kPipeWaitTicks = 200000000;  // 2 s of the 100 MHz wall clock: a wait that long aborts the step
-
-template <class A>
-__device__ __forceinline__ Launch<A> phase_launch(const PipeArgs& P, const PhaseRec& R) {
-  return Launch<A>{reinterpret_cast<const A*>(P.base + R.args), reinterpret_cast<const uint32_t*>(P.base + R.prefix),
-                   R.count > 1 ? reinterpret_cast<const uint32_t*>(P.base + R.group) : nullptr, R.count};
-}
-
-__device__ __forceinline__ void run_phase(const PipeArgs& P, const PhaseRec& R, const Where& w, uint8_t* lds) {
-  switch (R.kind) {
-    case PK_SCAN_REDUCE: ph_scan_reduce(phase_launch<ScanSeg>(P, R), w, lds); break;
-    case PK_SCAN_TOP: ph_scan_top(phase_launch<ScanSeg>(P, R), w, lds); break;
-    case PK_SCAN_DOWN: ph_scan_down(phase_launch<ScanSeg>(P, R), w, lds); break;
-    case PK_SCAN_ONE: ph_scan_one(phase_launch<ScanSeg>(P, R), w, lds); break;
-    case PK_COMPACT: ph_compact(phase_launch<CompactArgs>(P, R), w, lds); break;
-    case PK_INSERT: ph_insert(phase_launch<DenseArgs>(P, R), w, lds); break;
-    case PK_FIXMIN: ph_fixmin(phase_launch<DenseArgs>(P, R), w, lds); break;
-    case PK_RESOLVE: ph_resolve(phase_launch<DenseArgs>(P, R), w, lds); break;
-    case PK_SINGLE_LEAF: ph_single_leaf(phase_launch<SweepArgs>(P, R), w); break;
-    case PK_SWEEP_A_M3: ph_sweep_a<HASH_MURMUR3>(phase_launch<SweepArgs>(P, R), w, lds); break;
-    case PK_SWEEP_A_MD5: ph_sweep_a<HASH_MD5>(phase_launch<SweepArgs>(P, R), w, lds); break;
-    case PK_SWEEP_B_M3: ph_sweep_b<HASH_MURMUR3>(phase_launch<SweepArgs>(P, R), w, lds); break;
-    case PK_SWEEP_B_MD5: ph_sweep_b<HASH_MD5>(phase_launch<SweepArgs>(P, R), w, lds); break;
-    case PK_SCATTER_ROOTS: ph_scatter_roots(phase_launch<RootArgs>(P, R), w, lds); break;
-    case PK_SHIFT_SRC: ph_shift_src(phase_launch<ShiftArgs>(P, R), w, lds); break;
-    case PK_SORT_COUNT: ph_sort_count(phase_launch<SortArgs>(P, R), w, lds); break;
-    case PK_SORT_MOVE: ph_sort_move(phase_launch<SortArgs>(P, R), w, lds); break;
-    case PK_REGION_SIZES: ph_region_sizes(phase_launch<RegionArgs>(P, R), w, lds); break;
-    case PK_ASSEMBLE: ph_assemble(phase_launch<DiffArgs>(P, R), w, lds); break;
-    case PK_BASIC_DIFF: ph_basic_diff(phase_launch<BasicDiffArgs>(P, R), w, lds); break;
-    default: break;
-  }
-}
-
-// Thread 0's part of an iteration: count the block's previous item done, take
-// the next ticket, find its phase, item and member, and wait for the member's
-// previous phase.  Returns (ticket, phase) and whether the item is to be skipped.
-struct Ticket {
-  uint32_t t, p, m, blk, nblk, mem, skip;
-};
-
-__device__ __forceinline__ Ticket pipe_next(const PipeArgs& P, uint32_t prev_cell, uint32_t prev_p, uint32_t prev_mem) {
-  if (prev_cell != ~0u) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const uint32_t d = atomicAdd(&P.done[prev_cell], 1u) + 1;
-    if (d == P.need[prev_cell]) {
-      const unsigned long long now = (unsigned long long)wall_clock64();
-      if (prev_p == P.sweeps_end[prev_mem]) P.stamp[3 * prev_mem + 1] = now;
-      atomicMax(&P.stamp[3 * prev_mem + 2], now);
-    }
-  }
-  Ticket k{atomicAdd(&P.ctl[0], 1u), 0, 0, 0, 0, 0, 0};
-  if (k.t >= P.total) return k;
-  while (P.ph[k.p].first + P.ph[k.p].items <= k.t) k.p++;
-  const PhaseRec& R = P.ph[k.p];
-  const uint32_t b = k.t - R.first;
-  const uint32_t* prefix = reinterpret_cast<const uint32_t*>(P.base + R.prefix);
-  if (R.count > 1) {
-    k.m = reinterpret_cast<const uint32_t*>(P.base + R.group)[b >> 6];
-    while (prefix[k.m + 1] <= b) k.m++;
-  }
-  k.blk = b - prefix[k.m];
-  k.nblk = prefix[k.m + 1] - prefix[k.m];
-  k.mem = reinterpret_cast<const uint32_t*>(P.base + R.owner)[k.m];
-  const uint32_t q = P.wait[k.mem * P.nph + k.p];
-  if (q == ~0u) {
-    atomicMin(&P.stamp[3 * k.mem], (unsigned long long)wall_clock64());
-  } else {
-    const uint32_t want = P.need[k.mem * P.nph + q];
-    const uint32_t* cnt = &P.done[k.mem * P.nph + q];
-    const long long t0 = wall_clock64();
-    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
-      if (__hip_atomic_load(&P.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-      if (wall_clock64() - t0 > kPipeWaitTicks) {
-        __hip_atomic_store(&P.ctl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  // After an abort no item runs: its inputs may be incomplete.
-  k.skip = __hip_atomic_load(&P.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return k;
-}
-
-__device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
-
-// One iteration per item: thread 0 finishes the previous item and fetches the
-// next, a barrier publishes the ticket, the block runs the item, a barrier ends
-// it.  Every value that steers the loop is read back with readfirstlane, so the
-// compiler sees uniform control flow around the barriers (an exit it took for
-// divergent let it re-enter the item body without thread 0).
-__global__ __launch_bounds__(kThreads, MTC_PIPE_WAVES) void k_pipeline(PipeArgs P) {
-  __shared__ __align__(16) uint8_t lds[kPipeLds];
-  __shared__ Ticket s_k;
-  uint32_t prev_cell = ~0u, prev_p = 0, prev_mem = 0;  // meaningful in thread 0 only
-  for (;;) {
-    if (threadIdx.x == 0) s_k = pipe_next(P, prev_cell, prev_p, prev_mem);
-    __syncthreads();
-    const uint32_t t = uniform(s_k.t);
-    if (t >= P.total) break;
-    const uint32_t p = uniform(s_k.p);
-    const uint32_t mem = uniform(s_k.mem);
-    const Where w{uniform(s_k.m), uniform(s_k.blk), uniform(s_k.nblk)};
-    const uint32_t skip = uniform(s_k.skip);
-    if (!skip) run_phase(P, P.ph[p], w, lds);
-    __syncthreads();
-    prev_cell = mem * P.nph + p;
-    prev_p = p;
-    prev_mem = mem;
-  }
-}
-
 // Reinsert every entry of the old table into a larger one (a rehash).
 __global__ __launch_bounds__(kThreads) void k_rehash(const Slot* old, uint64_t old_slots, Slot* tab, uint64_t mask,
                                                      Ctl* ctl) {
@@ -2131,7 +1998,7 @@ struct HostTrace {
 namespace {
 
 struct LaunchRec {
-  uint64_t args = 0, prefix = 0, group = 0, owner = 0;
+  uint64_t args = 0, prefix = 0, group = 0;
   uint32_t count = 0, grid = 0;
 };
 
@@ -2144,11 +2011,9 @@ struct ArenaImage {
     if (size) std::memcpy(bytes.data() + off, p, size);
     return off;
   }
-  // One batched launch: an argument block and a block count per launch member, and
-  // the step member that owns each launch member (the pipeline tracks progress per
-  // step member).
+  // One batched launch: an argument block and a block count per launch member.
   template <class A>
-  LaunchRec add(const std::vector<A>& args, const std::vector<uint32_t>& blocks, const std::vector<uint32_t>& owners) {
+  LaunchRec add(const std::vector<A>& args, const std::vector<uint32_t>& blocks) {
     LaunchRec r;
     r.count = (uint32_t)args.size();
     std::vector<uint32_t> prefix(r.count + 1, 0);
@@ -2156,7 +2021,6 @@ struct ArenaImage {
     r.grid = prefix[r.count];
     r.args = put(args.data(), sizeof(A) * args.size());
     r.prefix = put(prefix.data(), sizeof(uint32_t) * prefix.size());
-    r.owner = put(owners.data(), sizeof(uint32_t) * owners.size());
     if (r.count > 1) {
       std::vector<uint32_t> group((r.grid + 63) / 64);
       uint32_t m = 0;
@@ -2181,7 +2045,6 @@ struct Member {
   const uint8_t* data;
   uint64_t len;
   bool baseline;
-  uint32_t idx = 0;        // position in the step
   Ctl* ctl_out = nullptr;  // this member's entry of the step's result array
   int arena = 0;           // diff arena this step writes
   // derived
@@ -2197,30 +2060,21 @@ struct Member {
   uint32_t* ws_reg = nullptr;
 };
 
-// Recorded launches of a step: run eagerly, captured into a graph, or (the
-// phases after the leaf kernel) run as one persistent pipeline launch.
+// Recorded launches of a step: run eagerly or captured into a graph.  A tag names
+// each launch's kernel (sweep B: 44 and 45, one chunk: 30, leaf kernels: 100 up).
 struct Ops {
   std::vector<std::function<void(uint8_t*, hipStream_t)>> fn;
   std::vector<uint64_t> sig;
-  std::vector<uint32_t> kind;  // PhaseKind of each launch (PK_NONE: not a pipeline phase)
-  std::vector<LaunchRec> rec;
+  std::vector<uint32_t> tag;
   template <class A, class K>
-  void add(uint32_t tag, uint32_t pk, const LaunchRec& r, K launch) {
+  void add(uint32_t t, const LaunchRec& r, K launch) {
     if (r.grid == 0) return;
-    sig.push_back(((uint64_t)tag << 32) | r.grid);
+    sig.push_back(((uint64_t)t << 32) | r.grid);
     sig.push_back((r.args << 32) ^ (r.prefix << 8) ^ r.group ^ r.count);
     fn.push_back([r, launch](uint8_t* base, hipStream_t st) { launch(bind<A>(r, base), r.grid, st); });
-    kind.push_back(pk);
-    rec.push_back(r);
+    tag.push_back(t);
   }
 };
-
-// Owners of a launch whose argument list holds one block per step member.
-static std::vector<uint32_t> owners_of(const std::vector<Member>& ms) {
-  std::vector<uint32_t> o;
-  for (const auto& M : ms) o.push_back(M.idx);
-  return o;
-}
 
 }  // namespace
 
@@ -2291,18 +2145,18 @@ static void prepare_member(Member& M, hipStream_t st) {
   }
 }
 
-static void add_scan(Ops& ops, ArenaImage& img, const std::vector<ScanSeg>& segs, const std::vector<uint32_t>& owners) {
+static void add_scan(Ops& ops, ArenaImage& img, const std::vector<ScanSeg>& segs) {
   if (segs.empty()) return;
   std::vector<uint32_t> tiles, ones(segs.size(), 1);
   if (std::all_of(segs.begin(), segs.end(), [](const ScanSeg& s) { return s.count_max <= kScanTile; })) {
-    ops.add<ScanSeg>(13, PK_SCAN_ONE, img.add(segs, ones, owners), [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_one<<<g, kThreads, 0, st>>>(L); });
+    ops.add<ScanSeg>(13, img.add(segs, ones), [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_one<<<g, kThreads, 0, st>>>(L); });
     return;
   }
   for (const auto& s : segs) tiles.push_back(std::max<uint32_t>(1, std::min<uint32_t>(512, (s.count_max + kScanTile - 1) / kScanTile)));
-  const LaunchRec rt = img.add(segs, tiles, owners), r1 = img.add(segs, ones, owners);
-  ops.add<ScanSeg>(10, PK_SCAN_REDUCE, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_reduce<<<g, kThreads, 0, st>>>(L); });
-  ops.add<ScanSeg>(11, PK_SCAN_TOP, r1, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_top<<<g, kThreads, 0, st>>>(L); });
-  ops.add<ScanSeg>(12, PK_SCAN_DOWN, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_down<<<g, kThreads, 0, st>>>(L); });
+  const LaunchRec rt = img.add(segs, tiles), r1 = img.add(segs, ones);
+  ops.add<ScanSeg>(10, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_reduce<<<g, kThreads, 0, st>>>(L); });
+  ops.add<ScanSeg>(11, r1, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_top<<<g, kThreads, 0, st>>>(L); });
+  ops.add<ScanSeg>(12, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_down<<<g, kThreads, 0, st>>>(L); });
 }
 
 // The scan of the leaf blocks' counts (the offsets of each block's list of
@@ -2311,14 +2165,13 @@ static void add_leaf_scan(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
   std::vector<ScanSeg> ss;
   for (auto& M : ms)
     ss.push_back(ScanSeg{M.h->blk_cnt.p, M.h->blk_off.p, nullptr, M.nblk_leaf, M.h->blk_off.p + M.nblk_leaf, M.ws_blk});
-  add_scan(ops, img, ss, owners_of(ms));
+  add_scan(ops, img, ss);
 }
 
 // After the leaf kernel, for tree members: table passes, both forests, roots and
 // the diff.
 static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
   const size_t B = ms.size();
-  const std::vector<uint32_t> own = owners_of(ms);
   // 3. dense table passes over the changed chunks
   {
     std::vector<CompactArgs> ca;
@@ -2334,11 +2187,11 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
                              h->tile_flags.p + M.ntile0});
       db.push_back(grid_for(M.n, 8 * kThreads, 16384));
     }
-    ops.add<CompactArgs>(20, PK_COMPACT, img.add(ca, cb, own), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
-    const LaunchRec rd = img.add(da, db, own);
-    ops.add<DenseArgs>(22, PK_INSERT, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_insert<<<g, kThreads, 0, st>>>(L); });
-    ops.add<DenseArgs>(24, PK_FIXMIN, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_fixmin<<<g, kThreads, 0, st>>>(L); });
-    ops.add<DenseArgs>(23, PK_RESOLVE, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_resolve<<<g, kThreads, 0, st>>>(L); });
+    ops.add<CompactArgs>(20, img.add(ca, cb), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
+    const LaunchRec rd = img.add(da, db);
+    ops.add<DenseArgs>(22, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_insert<<<g, kThreads, 0, st>>>(L); });
+    ops.add<DenseArgs>(24, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_fixmin<<<g, kThreads, 0, st>>>(L); });
+    ops.add<DenseArgs>(23, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_resolve<<<g, kThreads, 0, st>>>(L); });
   }
   // 4. sweeps: every stage of sweep A (and its inserts) before any stage of sweep B
   {
@@ -2375,13 +2228,13 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       for (size_t i = 0; i < B; i++)
         if (ms[i].n == 1) sb[i] = 1, any = true;
       if (any)
-        ops.add<SweepArgs>(30, PK_SINGLE_LEAF, img.add(base, sb, own), [](Launch<SweepArgs> L, uint32_t g, hipStream_t st) { k_single_leaf<<<g, 64, 0, st>>>(L); });
+        ops.add<SweepArgs>(30, img.add(base, sb), [](Launch<SweepArgs> L, uint32_t g, hipStream_t st) { k_single_leaf<<<g, 64, 0, st>>>(L); });
     }
     for (int pass = 0; pass < 2; pass++) {
       for (size_t s = 0; s < max_stages; s++) {
         struct Group {
           std::vector<SweepArgs> args;
-          std::vector<uint32_t> blocks, owners;
+          std::vector<uint32_t> blocks;
         };
         std::map<int, Group> byhk;
         for (size_t i = 0; i < B; i++) {
@@ -2409,19 +2262,18 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
           }
           slot.args.push_back(a);
           slot.blocks.push_back(blocks);
-          slot.owners.push_back(ms[i].idx);
         }
         for (auto& kv : byhk) {
           const int hk = kv.first;
           const uint32_t tag = 40 + 4 * (uint32_t)pass + (uint32_t)hk;
-          const LaunchRec r = img.add(kv.second.args, kv.second.blocks, kv.second.owners);
+          const LaunchRec r = img.add(kv.second.args, kv.second.blocks);
           if (pass == 0)
-            ops.add<SweepArgs>(tag, hk == HASH_MD5 ? PK_SWEEP_A_MD5 : PK_SWEEP_A_M3, r, [hk](Launch<SweepArgs> L, uint32_t g, hipStream_t st) {
+            ops.add<SweepArgs>(tag, r, [hk](Launch<SweepArgs> L, uint32_t g, hipStream_t st) {
               if (hk == HASH_MD5) k_sweep_a<HASH_MD5><<<g, kThreads, 0, st>>>(L);
               else k_sweep_a<HASH_MURMUR3><<<g, kThreads, 0, st>>>(L);
             });
           else
-            ops.add<SweepArgs>(tag, hk == HASH_MD5 ? PK_SWEEP_B_MD5 : PK_SWEEP_B_M3, r, [hk](Launch<SweepArgs> L, uint32_t g, hipStream_t st) {
+            ops.add<SweepArgs>(tag, r, [hk](Launch<SweepArgs> L, uint32_t g, hipStream_t st) {
               if (hk == HASH_MD5) k_sweep_b<HASH_MD5><<<g, kThreads, 0, st>>>(L);
               else k_sweep_b<HASH_MURMUR3><<<g, kThreads, 0, st>>>(L);
             });
@@ -2432,15 +2284,12 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
   // 5. root totals and compaction (segments are (depth, block) in node-id order)
   {
     std::vector<ScanSeg> ss;
-    std::vector<uint32_t> so;
     for (auto& M : ms) {
       mtc_dedup* h = M.h;
       ss.push_back(ScanSeg{h->seg_cnt_f.p, h->seg_off_f.p, nullptr, M.nseg, &h->ctl_d->F, M.ws_f});
       ss.push_back(ScanSeg{h->seg_cnt_s.p, h->seg_off_s.p, nullptr, M.nseg, &h->ctl_d->S, M.ws_s});
-      so.push_back(M.idx);
-      so.push_back(M.idx);
     }
-    add_scan(ops, img, ss, so);
+    add_scan(ops, img, ss);
     std::vector<RootArgs> ra;
     std::vector<uint32_t> rb;
     for (auto& M : ms) {
@@ -2449,7 +2298,7 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
                             M.g.D + 1, M.nseg, h->first_roots.p, h->shift_tmp.p});
       rb.push_back(grid_for(M.nseg));
     }
-    ops.add<RootArgs>(50, PK_SCATTER_ROOTS, img.add(ra, rb, own), [](Launch<RootArgs> L, uint32_t g, hipStream_t st) { k_scatter_roots<<<g, kThreads, 0, st>>>(L); });
+    ops.add<RootArgs>(50, img.add(ra, rb), [](Launch<RootArgs> L, uint32_t g, hipStream_t st) { k_scatter_roots<<<g, kThreads, 0, st>>>(L); });
   }
   // 6. shifted entries: source checkpoint, stable sort by source, (chkpt, count) table
   std::vector<const uint64_t*> sorted(B);
@@ -2466,12 +2315,11 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       max_passes = std::max(max_passes, M.passes);
       sorted[i] = h->sort_val[M.passes % 2].p;
     }
-    ops.add<ShiftArgs>(60, PK_SHIFT_SRC, img.add(sa, sb, own), [](Launch<ShiftArgs> L, uint32_t g, hipStream_t st) { k_shift_src<<<g, kThreads, 0, st>>>(L); });
+    ops.add<ShiftArgs>(60, img.add(sa, sb), [](Launch<ShiftArgs> L, uint32_t g, hipStream_t st) { k_shift_src<<<g, kThreads, 0, st>>>(L); });
     for (uint32_t p = 0; p < max_passes; p++) {
       std::vector<SortArgs> so;
       std::vector<uint32_t> sblk;
       std::vector<ScanSeg> hs;
-      std::vector<uint32_t> hs_own;
       for (auto& M : ms) {
         mtc_dedup* h = M.h;
         uint32_t bits = 0;
@@ -2481,15 +2329,12 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
         so.push_back(SortArgs{h->sort_key[p % 2].p, h->sort_val[p % 2].p, h->sort_key[1 - p % 2].p,
                               h->sort_val[1 - p % 2].p, &h->ctl_d->S, shift, nb, h->sort_hist.p});
         sblk.push_back(p < M.passes ? M.Gs : 0u);
-        if (p < M.passes) {
-          hs.push_back(ScanSeg{h->sort_hist.p, h->sort_hist.p, nullptr, nb * M.Gs, nullptr, M.ws_sort});
-          hs_own.push_back(M.idx);
-        }
+        if (p < M.passes) hs.push_back(ScanSeg{h->sort_hist.p, h->sort_hist.p, nullptr, nb * M.Gs, nullptr, M.ws_sort});
       }
-      const LaunchRec r = img.add(so, sblk, own);
-      ops.add<SortArgs>(61, PK_SORT_COUNT, r, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_count<<<g, kThreads, 0, st>>>(L); });
-      add_scan(ops, img, hs, hs_own);
-      ops.add<SortArgs>(62, PK_SORT_MOVE, r, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_move<<<g, kThreads, 0, st>>>(L); });
+      const LaunchRec r = img.add(so, sblk);
+      ops.add<SortArgs>(61, r, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_count<<<g, kThreads, 0, st>>>(L); });
+      add_scan(ops, img, hs);
+      ops.add<SortArgs>(62, r, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_move<<<g, kThreads, 0, st>>>(L); });
     }
   }
   // 7. region sizes and offsets, metadata, gathered data, counters
@@ -2504,8 +2349,8 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       rb.push_back(grid_for(M.n, 16 * kThreads, 256));
       ss.push_back(ScanSeg{h->region_off.p, h->region_off.p, &h->ctl_d->F, M.n, h->region_off.p + M.n, M.ws_reg});
     }
-    ops.add<RegionArgs>(70, PK_REGION_SIZES, img.add(ra, rb, own), [](Launch<RegionArgs> L, uint32_t g, hipStream_t st) { k_region_sizes<<<g, kThreads, 0, st>>>(L); });
-    add_scan(ops, img, ss, own);
+    ops.add<RegionArgs>(70, img.add(ra, rb), [](Launch<RegionArgs> L, uint32_t g, hipStream_t st) { k_region_sizes<<<g, kThreads, 0, st>>>(L); });
+    add_scan(ops, img, ss);
     std::vector<DiffArgs> da;
     std::vector<uint32_t> gb;
     for (size_t i = 0; i < B; i++) {
@@ -2533,7 +2378,7 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       da.push_back(d);
       gb.push_back(grid_for(M.n, 8 * kThreads, 8192));
     }
-    ops.add<DiffArgs>(72, PK_ASSEMBLE, img.add(da, gb, own), [](Launch<DiffArgs> L, uint32_t g, hipStream_t st) { k_assemble<<<g, kThreads, 0, st>>>(L); });
+    ops.add<DiffArgs>(72, img.add(da, gb), [](Launch<DiffArgs> L, uint32_t g, hipStream_t st) { k_assemble<<<g, kThreads, 0, st>>>(L); });
   }
 }
 
@@ -2541,7 +2386,6 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
 // diff of the changed chunks, whose ascending list the leaf pass and compaction
 // leave.
 static void build_basic_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
-  const std::vector<uint32_t> own = owners_of(ms);
   // the leaf blocks' list regions, concatenated in block order (ascending)
   {
     std::vector<CompactArgs> ca;
@@ -2551,7 +2395,7 @@ static void build_basic_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) 
       ca.push_back(CompactArgs{h->list.p, h->blk_cnt.p, h->blk_off.p, M.tpb, h->dense.p, M.nblk_leaf});
       cb.push_back((M.nblk_leaf + kCompactPer - 1) / kCompactPer);
     }
-    ops.add<CompactArgs>(21, PK_COMPACT, img.add(ca, cb, own), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
+    ops.add<CompactArgs>(21, img.add(ca, cb), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
   }
   std::vector<BasicDiffArgs> da;
   std::vector<uint32_t> gb;
@@ -2574,69 +2418,39 @@ static void build_basic_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) 
     da.push_back(d);
     gb.push_back(grid_for(M.n, 8 * kThreads, 8192));
   }
-  ops.add<BasicDiffArgs>(80, PK_BASIC_DIFF, img.add(da, gb, own), [](Launch<BasicDiffArgs> L, uint32_t g, hipStream_t st) { k_basic_diff<<<g, kThreads, 0, st>>>(L); });
+  ops.add<BasicDiffArgs>(80, img.add(da, gb), [](Launch<BasicDiffArgs> L, uint32_t g, hipStream_t st) { k_basic_diff<<<g, kThreads, 0, st>>>(L); });
 }
 
-// Words of the pipeline's counters for a step: ticket and abort flag, per-member
-// per-phase counts, then (8-byte aligned) three 64-bit time stamps per member.
-static uint64_t pipe_words(size_t members, uint32_t max_phases) {
-  return ((4 + (uint64_t)members * max_phases + 1) & ~1ull) + 6 * members;
-}
-
-// Most phases a step of these members can have (sizes the pipeline's counters,
-// which the step's first fill clears before the phases are built).
-static uint32_t max_phases(const std::vector<Member>& ms) {
-  uint32_t stages = 0, passes = 0;
-  for (const auto& M : ms) {
-    stages = std::max<uint32_t>(stages, (uint32_t)M.h->stages.size());
-    passes = std::max(passes, M.passes);
-  }
-  // leaf scan 3, table passes 4, single leaf 1, two sweeps per hash per stage,
-  // root scans 3 + scatter 1, shift 1, per sort pass count + scan 3 + move, region
-  // 1 + scan 3, assemble 1, basic compact and diff 2
-  return 3 + 4 + 1 + 4 * stages + 4 + 1 + 5 * passes + 4 + 1 + 2;
-}
-
-// The step's launch sequence.  Phase 1: the fills (counters, control blocks, tile
-// flags, histograms, debug views, the pipeline's counters) and the leaf kernels,
-// plus the scan of the leaf blocks' counts when `leaf_scan` (steps that read the
-// count of chunks to insert before phase 2).  Phase 2: the rest.
-static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int phase, bool leaf_scan,
-                      uint32_t* pipe, uint64_t pipe_len) {
+// The step's launch sequence.  Part 1: the fills (control blocks, counters, tile
+// flags, histograms, debug views) and the leaf kernels, plus the scan of the leaf
+// blocks' counts when `leaf_scan` (steps that read the count of chunks to insert
+// before part 2).  Part 2: the rest.  Part 0: both.
+static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int phase, bool leaf_scan) {
   const size_t B = ms.size();
-  if (phase == 1) {
+  if (phase != 2) {
     {
       std::vector<FillSeg> fs;
-      std::vector<uint32_t> fb, fo;
-      auto fill = [&](void* p, uint64_t words, uint32_t v, uint32_t owner) {
+      std::vector<uint32_t> fb;
+      auto fill = [&](void* p, uint64_t words, uint32_t v) {
         if (!words) return;
         fs.push_back(FillSeg{reinterpret_cast<uint32_t*>(p), words, v});
         fb.push_back(grid_for(words, kThreads * 4, 4096));
-        fo.push_back(owner);
       };
       for (auto& M : ms) {
         mtc_dedup* h = M.h;
-        fill(h->ctl_d, sizeof(Ctl) / 4, 0, M.idx);
-        fill(h->ctr.p, 2ull * CTR_KINDS * kStripes, 0, M.idx);
-        fill(h->tile_flags.p, (2 * M.ntile0 + 3) / 4, 0, M.idx);
-        fill(h->seg_cnt_f.p, M.nseg, 0, M.idx);
-        fill(h->seg_cnt_s.p, M.nseg, 0, M.idx);
-        fill(h->src_hist.p, M.nbins, 0, M.idx);
-        fill(h->blk_cnt.p, M.nblk_leaf, 0, M.idx);  // counts that k_leaf1 accumulates
+        fill(h->ctl_d, sizeof(Ctl) / 4, 0);
+        fill(h->ctr.p, 2ull * CTR_KINDS * kStripes, 0);
+        fill(h->tile_flags.p, (2 * M.ntile0 + 3) / 4, 0);
+        fill(h->seg_cnt_f.p, M.nseg, 0);
+        fill(h->seg_cnt_s.p, M.nseg, 0);
+        fill(h->src_hist.p, M.nbins, 0);
+        fill(h->blk_cnt.p, M.nblk_leaf, 0);  // counts that k_leaf1 accumulates
         if (h->debug) {
-          fill(h->dbg_tree.p, 4 * (2ull * M.n - 1), 0, M.idx);
-          fill(h->dbg_labels.p, (2ull * M.n - 1 + 3) / 4, 0, M.idx);
+          fill(h->dbg_tree.p, 4 * (2ull * M.n - 1), 0);
+          fill(h->dbg_labels.p, (2ull * M.n - 1 + 3) / 4, 0);
         }
       }
-      if (pipe) {  // counters and stamps 0; the start stamps all ones (an atomic min)
-        const uint64_t stamps = pipe_len - 6 * B;
-        fill(pipe, stamps, 0, 0);
-        for (size_t i = 0; i < B; i++) {
-          fill(pipe + stamps + 6 * i, 2, ~0u, 0);
-          fill(pipe + stamps + 6 * i + 2, 4, 0, 0);
-        }
-      }
-      ops.add<FillSeg>(1, PK_NONE, img.add(fs, fb, fo), [](Launch<FillSeg> L, uint32_t g, hipStream_t st) { k_fill<<<g, kThreads, 0, st>>>(L); });
+      ops.add<FillSeg>(1, img.add(fs, fb), [](Launch<FillSeg> L, uint32_t g, hipStream_t st) { k_fill<<<g, kThreads, 0, st>>>(L); });
     }
     // leaf kernel, one launch per (hash, staged chunk size, single tile) group
     std::map<std::tuple<int, int, bool>, std::vector<size_t>> groups;
@@ -2645,7 +2459,7 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
               ms[i].cb >= 16 && ms[i].tpb == 1 && ms[i].h->approach != MTC_APPROACH_BASIC}].push_back(i);
     for (auto& kv : groups) {
       std::vector<LeafArgs> la;
-      std::vector<uint32_t> lb, lo;
+      std::vector<uint32_t> lb;
       for (size_t i : kv.second) {
         Member& M = ms[i];
         mtc_dedup* h = M.h;
@@ -2654,12 +2468,11 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
         la.push_back(LeafArgs{M.data, M.len, M.n, h->chunk_size, h->leaf_dg.p, h->labels.p, reset,
                               h->list.p, h->blk_cnt.p, M.tpb});
         lb.push_back(M.nblk_leaf);
-        lo.push_back(M.idx);
       }
       const int hk = std::get<0>(kv.first), cb = std::get<1>(kv.first);
       const bool lean = std::get<2>(kv.first);
       const uint32_t tag = 100 + (uint32_t)hk * 32 + (lean ? 16u : 0u) + (uint32_t)(cb ? __builtin_ctz((unsigned)cb) : 0);
-      ops.add<LeafArgs>(tag, PK_NONE, img.add(la, lb, lo), [hk, cb, lean](Launch<LeafArgs> L, uint32_t g, hipStream_t st) {
+      ops.add<LeafArgs>(tag, img.add(la, lb), [hk, cb, lean](Launch<LeafArgs> L, uint32_t g, hipStream_t st) {
         if (lean) {
           if (hk == HASH_MD5) launch_leaf1<HASH_MD5>(cb, L, g, st);
           else launch_leaf1<HASH_MURMUR3>(cb, L, g, st);
@@ -2670,75 +2483,14 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
         }
       });
     }
-    if (leaf_scan) add_leaf_scan(ms, ops, img);
-    return;
+    if (leaf_scan || phase == 0) add_leaf_scan(ms, ops, img);
+    if (phase == 1) return;
   }
-  if (!leaf_scan) add_leaf_scan(ms, ops, img);
+  if (phase == 2 && !leaf_scan) add_leaf_scan(ms, ops, img);
   std::vector<Member> tree, basic;
   for (auto& M : ms) (M.h->approach == MTC_APPROACH_BASIC ? basic : tree).push_back(M);
   if (!tree.empty()) build_tree_ops(tree, ops, img);
   if (!basic.empty()) build_basic_ops(basic, ops, img);
-}
-
-// Appends to the arena the phase table of one pipeline launch over ops (every op a
-// phase, in order) and returns its arguments (base and table pointers are filled
-// in at launch, once the arena's address is known).  sweeps_end[m] is the last
-// phase of member m that builds its labels and roots.
-struct PipePlan {
-  PipeArgs args{};
-  uint64_t ph = 0, wait = 0, need = 0, sweeps_end = 0;
-  uint32_t grid = 0;
-};
-
-static uint32_t pipeline_blocks() {
-  static int cached = 0;
-  if (!cached) {
-    int dev = 0, cus = 0, per_cu = 0;
-    MTC_HIP(hipGetDevice(&dev));
-    MTC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    MTC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pipeline, kThreads, 0));
-    cached = std::max(1, cus) * std::max(1, std::min(per_cu, 8));
-  }
-  return (uint32_t)cached;
-}
-
-static PipePlan plan_pipeline(const Ops& ops, ArenaImage& img, size_t B, uint32_t* pipe, uint32_t max_ph) {
-  PipePlan plan;
-  const uint32_t nph = (uint32_t)ops.fn.size();
-  if (nph > max_ph) throw Status{MTC_ERR_STATE, "pipeline phase count above its bound"};
-  std::vector<PhaseRec> ph(nph);
-  std::vector<uint32_t> need((size_t)B * nph, 0), wait((size_t)B * nph, ~0u), last(B, ~0u), sweeps(B, ~0u);
-  uint32_t first = 0;
-  for (uint32_t p = 0; p < nph; p++) {
-    const LaunchRec& r = ops.rec[p];
-    ph[p] = PhaseRec{ops.kind[p], r.grid, first, r.count, r.args, r.prefix, r.group, r.owner};
-    first += r.grid;
-    // blocks per step member
-    std::vector<uint32_t> prefix(r.count + 1), owner(r.count);
-    std::memcpy(prefix.data(), img.bytes.data() + r.prefix, 4ull * (r.count + 1));
-    std::memcpy(owner.data(), img.bytes.data() + r.owner, 4ull * r.count);
-    for (uint32_t m = 0; m < r.count; m++) need[(size_t)owner[m] * nph + p] += prefix[m + 1] - prefix[m];
-    for (size_t i = 0; i < B; i++) {
-      if (!need[i * nph + p]) continue;
-      wait[i * nph + p] = last[i];
-      last[i] = p;
-      const uint32_t k = ops.kind[p];
-      if (k == PK_SWEEP_B_M3 || k == PK_SWEEP_B_MD5 || k == PK_SINGLE_LEAF) sweeps[i] = p;
-    }
-  }
-  plan.ph = img.put(ph.data(), sizeof(PhaseRec) * ph.size());
-  plan.wait = img.put(wait.data(), 4 * wait.size());
-  plan.need = img.put(need.data(), 4 * need.size());
-  plan.sweeps_end = img.put(sweeps.data(), 4 * sweeps.size());
-  PipeArgs& a = plan.args;
-  a.nph = nph;
-  a.nmem = (uint32_t)B;
-  a.total = first;
-  a.ctl = pipe;
-  a.done = pipe + 4;
-  a.stamp = reinterpret_cast<unsigned long long*>(pipe + pipe_words(B, max_ph) - 6 * B);
-  plan.grid = std::max<uint32_t>(1, std::min<uint32_t>(first, pipeline_blocks()));
-  return plan;
 }
 
 void StepCtx::reset_graph() {
@@ -2753,7 +2505,6 @@ StepCtx::~StepCtx() {
   if (lead_h) (void)hipHostFree(lead_h);
   if (ctl_h) (void)hipHostFree(ctl_h);
   if (cnt_h) (void)hipHostFree(cnt_h);
-  if (stamp_h) (void)hipHostFree(stamp_h);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
 }
@@ -2775,40 +2526,19 @@ void run_step(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_gr
   }
 }
 
-// MTC_PIPELINE=launches runs the phases after the leaf kernel as one launch each
-// instead of one persistent launch (the results are the same).
-static bool pipeline_enabled() {
-  const char* e = std::getenv("MTC_PIPELINE");
-  return !(e && std::strcmp(e, "launches") == 0);
-}
-
-static uint32_t wall_clock_khz() {
-  static int khz = 0;
-  if (!khz) {
-    int dev = 0;
-    MTC_HIP(hipGetDevice(&dev));
-    MTC_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-    if (khz <= 0) khz = 100000;
-  }
-  return (uint32_t)khz;
-}
-
-// A step in two parts.  Part 1 (fills and leaf kernels) runs at once from its own
-// arena; part 2 is built while the leaf kernels stream.  Part 2 is one persistent
-// pipeline launch (or, with MTC_PIPELINE=launches, one launch per phase, replayed
-// from a graph when use_graph and the launch shape repeats).  A step whose table
-// might overflow pauses between the parts for the exact bound (the scan of the
-// leaf blocks' counts then runs in part 1).
+// A step in two parts when it is replayed from a graph: part 1 (fills and leaf
+// kernels) runs at once from its own arena while part 2 is built, then part 2 is
+// replayed from a graph when its launch shape repeats.  Otherwise every launch
+// goes out from one arena.  A step whose table might overflow pauses between the
+// parts for the exact bound (the scan of the leaf blocks' counts then runs in
+// part 1).
 static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st, bool use_graph) {
   HostTrace ht;
   const size_t B = ms.size();
   if (!ctx.ev[0])
     for (auto& e : ctx.ev) MTC_HIP(hipEventCreate(&e));
   MTC_HIP(hipEventRecord(ctx.ev[0], st));
-  for (size_t i = 0; i < B; i++) {
-    ms[i].idx = (uint32_t)i;
-    prepare_member(ms[i], st);
-  }
+  for (auto& M : ms) prepare_member(M, st);
   // Reset tables of baseline members (basic members keep no table).
   for (auto& M : ms) {
     if (!M.baseline || M.h->approach == MTC_APPROACH_BASIC) continue;
@@ -2822,14 +2552,13 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
 
   const bool debug = std::any_of(ms.begin(), ms.end(), [](const Member& M) { return M.h->debug != 0; });
   const bool split = std::any_of(ms.begin(), ms.end(), [](const Member& M) { return M.split; });
-  const bool pipe = pipeline_enabled();
+  const bool graph = use_graph && !debug;
   ctx.ctl_all.ensure(B);
   for (size_t i = 0; i < B; i++) ms[i].ctl_out = ctx.ctl_all.p + i;
-  const uint32_t max_ph = max_phases(ms);
-  const uint64_t pipe_len = pipe_words(B, max_ph);
-  if (pipe) ctx.pipe.reserve(pipe_len);
-  // Uploads an arena image and runs its launches, replayed from a graph or eagerly.
-  auto run_ops = [&](Ops& ops, ArenaImage& img, bool graph, DevBuf<uint8_t>& arena_d, uint8_t*& arena_h,
+  bool ev2 = false;  // ev[2] marks the end of the forests (eager launches only)
+  // Uploads an arena image and runs its launches, replayed from a graph or eagerly
+  // (the end of the leaf kernels is ev[4]).
+  auto run_ops = [&](Ops& ops, ArenaImage& img, bool as_graph, DevBuf<uint8_t>& arena_d, uint8_t*& arena_h,
                      uint64_t& arena_h_cap) {
     const uint64_t bytes = img.bytes.size();
     const uint8_t* old_arena = arena_d.p;
@@ -2843,7 +2572,7 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
     std::memcpy(arena_h, img.bytes.data(), bytes);
     MTC_HIP(hipMemcpyAsync(arena_d.p, arena_h, bytes, hipMemcpyHostToDevice, st));
     uint8_t* base = arena_d.p;
-    if (graph) {
+    if (as_graph) {
       ops.sig.push_back(reinterpret_cast<uint64_t>(base));
       if (!ctx.exec || ctx.sig != ops.sig || old_arena != base) {
         ctx.reset_graph();
@@ -2862,62 +2591,45 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
     for (size_t i = 0; i < ops.fn.size(); i++) {
       ops.fn[i](base, st);
       MTC_HIP(hipGetLastError());
-      if (ops.kind[i] == PK_SWEEP_B_M3 || ops.kind[i] == PK_SWEEP_B_MD5 || ops.kind[i] == PK_SINGLE_LEAF)
+      const uint32_t t = ops.tag[i];
+      if (t >= 100 && (i + 1 == ops.fn.size() || ops.tag[i + 1] < 100)) MTC_HIP(hipEventRecord(ctx.ev[4], st));
+      if (t == 44 || t == 45 || t == 30) {
         MTC_HIP(hipEventRecord(ctx.ev[2], st));
+        ev2 = true;
+      }
     }
   };
-  // Part 1: fills and leaf kernels.
-  {
+  if (split || graph) {
     Ops ops1;
     ArenaImage img1;
-    build_ops(ms, ops1, img1, 1, split, pipe ? ctx.pipe.p : nullptr, pipe_len);
+    build_ops(ms, ops1, img1, 1, split);
     run_ops(ops1, img1, false, ctx.lead_d, ctx.lead_h, ctx.lead_h_cap);
-    MTC_HIP(hipEventRecord(ctx.ev[4], st));
-  }
-  if (split) {  // the exact table bound of every member, then part 2
-    if (ctx.cnt_h_cap < B) {
-      if (ctx.cnt_h) (void)hipHostFree(ctx.cnt_h);
-      ctx.cnt_h = nullptr;
-      MTC_HIP(hipHostMalloc((void**)&ctx.cnt_h, B * sizeof(uint32_t)));
-      ctx.cnt_h_cap = (uint32_t)B;
+    if (split) {  // the exact table bound of every member
+      if (ctx.cnt_h_cap < B) {
+        if (ctx.cnt_h) (void)hipHostFree(ctx.cnt_h);
+        ctx.cnt_h = nullptr;
+        MTC_HIP(hipHostMalloc((void**)&ctx.cnt_h, B * sizeof(uint32_t)));
+        ctx.cnt_h_cap = (uint32_t)B;
+      }
+      for (size_t i = 0; i < B; i++)
+        MTC_HIP(hipMemcpyAsync(ctx.cnt_h + i, ms[i].h->blk_off.p + ms[i].nblk_leaf, sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, st));
+      MTC_HIP(hipStreamSynchronize(st));
+      ht.mark("phase 1");
+      for (size_t i = 0; i < B; i++)
+        if (ms[i].split) grow_exact(ms[i].h, ctx.cnt_h[i], st);
     }
-    for (size_t i = 0; i < B; i++)
-      MTC_HIP(hipMemcpyAsync(ctx.cnt_h + i, ms[i].h->blk_off.p + ms[i].nblk_leaf, sizeof(uint32_t),
-                             hipMemcpyDeviceToHost, st));
-    MTC_HIP(hipStreamSynchronize(st));
-    ht.mark("phase 1");
-    for (size_t i = 0; i < B; i++)
-      if (ms[i].split) grow_exact(ms[i].h, ctx.cnt_h[i], st);
-  }
-  // Part 2.
-  bool ev2 = false;
-  {
     Ops ops2;
     ArenaImage img2;
-    build_ops(ms, ops2, img2, 2, split, nullptr, 0);
-    if (pipe) {
-      const PipePlan plan = plan_pipeline(ops2, img2, B, ctx.pipe.p, max_ph);
-      Ops one;
-      LaunchRec none;
-      none.grid = plan.grid;
-      one.add<FillSeg>(90, PK_NONE, none, [](Launch<FillSeg>, uint32_t, hipStream_t) {});
-      one.fn[0] = [plan](uint8_t* base, hipStream_t s2) {
-        PipeArgs a = plan.args;
-        a.base = base;
-        a.ph = reinterpret_cast<const PhaseRec*>(base + plan.ph);
-        a.wait = reinterpret_cast<const uint32_t*>(base + plan.wait);
-        a.need = reinterpret_cast<const uint32_t*>(base + plan.need);
-        a.sweeps_end = reinterpret_cast<const uint32_t*>(base + plan.sweeps_end);
-        k_pipeline<<<plan.grid, kThreads, 0, s2>>>(a);
-      };
-      one.sig.push_back(plan.args.total);
-      ht.mark("build");
-      run_ops(one, img2, use_graph && !debug, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
-    } else {
-      ht.mark("build");
-      run_ops(ops2, img2, use_graph && !debug, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
-      ev2 = !(use_graph && !debug);
-    }
+    build_ops(ms, ops2, img2, 2, split);
+    ht.mark("build");
+    run_ops(ops2, img2, graph && !split, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
+  } else {
+    Ops ops;
+    ArenaImage img;
+    build_ops(ms, ops, img, 0, false);
+    ht.mark("build");
+    run_ops(ops, img, false, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
   }
   if (debug)
     for (auto& M : ms)
@@ -2928,33 +2640,19 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
   ht.mark("launch");
   if (ctx.ctl_h_cap < B) {
     if (ctx.ctl_h) (void)hipHostFree(ctx.ctl_h);
-    if (ctx.stamp_h) (void)hipHostFree(ctx.stamp_h);
     ctx.ctl_h = nullptr;
-    ctx.stamp_h = nullptr;
     MTC_HIP(hipHostMalloc((void**)&ctx.ctl_h, B * sizeof(Ctl)));
-    MTC_HIP(hipHostMalloc((void**)&ctx.stamp_h, B * 3 * sizeof(unsigned long long) + 16));
     ctx.ctl_h_cap = (uint32_t)B;
   }
   MTC_HIP(hipMemcpyAsync(ctx.ctl_h, ctx.ctl_all.p, B * sizeof(Ctl), hipMemcpyDeviceToHost, st));
-  if (pipe) {
-    MTC_HIP(hipMemcpyAsync(ctx.stamp_h, ctx.pipe.p + pipe_len - 6 * B, B * 3 * sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, st));
-    MTC_HIP(hipMemcpyAsync(reinterpret_cast<uint8_t*>(ctx.stamp_h) + B * 3 * sizeof(unsigned long long),
-                           ctx.pipe.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  }
   MTC_HIP(hipEventRecord(ctx.ev[3], st));
   MTC_HIP(hipStreamSynchronize(st));
   ht.mark("sync");
-  float ms_setup = 0, ms_total = 0, ms_leaf = 0, ms_sweeps = 0;
+  float ms_setup = 0, ms_total = 0, ms_leaf = 0, ms_forest = 0;
   MTC_HIP(hipEventElapsedTime(&ms_setup, ctx.ev[0], ctx.ev[1]));
   MTC_HIP(hipEventElapsedTime(&ms_total, ctx.ev[1], ctx.ev[3]));
   MTC_HIP(hipEventElapsedTime(&ms_leaf, ctx.ev[1], ctx.ev[4]));
-  if (ev2) MTC_HIP(hipEventElapsedTime(&ms_sweeps, ctx.ev[1], ctx.ev[2]));
-  if (pipe) {
-    uint32_t aborted = 0;
-    std::memcpy(&aborted, reinterpret_cast<uint8_t*>(ctx.stamp_h) + B * 3 * sizeof(unsigned long long), 4);
-    if (aborted) throw Status{MTC_ERR_HIP, "the step's pipeline waited over 2 s for a phase and stopped"};
-  }
+  if (ev2) MTC_HIP(hipEventElapsedTime(&ms_forest, ctx.ev[1], ctx.ev[2]));
   // MTC_TEST_FAIL_MEMBER=i makes member i report an inconsistent state (tests of
   // the failure path; nothing else reads it).
   const char* inject = std::getenv("MTC_TEST_FAIL_MEMBER");
@@ -2968,7 +2666,6 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
       throw Status{MTC_ERR_STATE, "member " + std::to_string(i) + ": inconsistent table state (err bits " +
                                       std::to_string(e) + ")"};
   }
-  const double khz = pipe ? (double)wall_clock_khz() : 1.0;
   for (size_t i = 0; i < B; i++) {
     mtc_dedup* h = ms[i].h;
     const Ctl& c = ctx.ctl_h[i];
@@ -2979,25 +2676,13 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
     h->last_diff_size = c.diff_size;
     h->diff_cur = ms[i].arena;
     h->last_header = mtc_header{h->baseline_id, ms[i].cur, ms[i].len, h->chunk_size, c.F, c.P, c.S};
-    // Timers (src/tree_approach.cpp:1640,1657,1672): set-up; labels and forests
-    // (the leaf kernels with the part-1 fills, then the pipeline up to the last
-    // sweep); the diff (roots compaction to assembly).
-    float dedup = ms_total, collect = 0;
-    if (pipe) {
-      const unsigned long long* t = ctx.stamp_h + 3 * i;
-      const double t0 = (double)t[0], t1 = (double)t[1], t2 = (double)t[2];
-      if (t[0] != ~0ull && t2 >= t0) {
-        const double mid = t[1] ? t1 : t0;
-        dedup = ms_leaf + (float)((mid - t0) / khz);
-        collect = (float)((t2 - mid) / khz);
-      }
-    } else if (ev2) {
-      dedup = ms_sweeps;
-      collect = ms_total - ms_sweeps;
-    }
+    // Timers (src/tree_approach.cpp:1640,1657,1672): set-up (buffers, table reset);
+    // labels and forests (fills, leaf kernels, table passes, both sweeps); the diff
+    // (root compaction to assembly).  A graph-replayed step reports its whole time
+    // as the second.
     h->last_ms[0] = ms_setup;
-    h->last_ms[1] = dedup;
-    h->last_ms[2] = collect;
+    h->last_ms[1] = ev2 ? ms_forest : ms_total;
+    h->last_ms[2] = ev2 ? ms_total - ms_forest : 0.0f;
     h->last_ms[3] = 0;
     h->last_ms[4] = ms_leaf;
   }

@@ -110,10 +110,8 @@ struct StepCtx {
   uint64_t arena_h_cap = 0;
   uint8_t* lead_h = nullptr;  // pinned staging of the arena of its first part (fills, leaf kernels)
   uint64_t lead_h_cap = 0;
-  DevBuf<uint32_t> pipe;  // counters of the persistent pipeline launch
   DevBuf<Ctl> ctl_all;
   Ctl* ctl_h = nullptr;  // pinned
-  unsigned long long* stamp_h = nullptr;  // pinned: the pipeline's time stamps per member, then its abort flag
   uint32_t ctl_h_cap = 0;
   uint32_t* cnt_h = nullptr;  // pinned: chunks to insert per member (steps that pause after the leaf kernel)
   uint32_t cnt_h_cap = 0;

@@ -39,7 +39,7 @@ struct Alloc {
 struct Node {
   const void* fn;
   dim3 grid, block;
-  uint8_t arg[sizeof(PipeArgs)];  // a Launch<A> or the pipeline's arguments (the kernels a graph may hold)
+  uint8_t arg[32];  // a Launch<A> (every kernel a graph may hold takes one)
 };
 
 struct Graph {
@@ -307,81 +307,6 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
   }
 }
 
-// The type name a phase's body takes (as it appears in the kernel names).
-const char* phase_type(uint32_t kind) {
-  switch (kind) {
-    case PK_SCAN_REDUCE: case PK_SCAN_TOP: case PK_SCAN_DOWN: case PK_SCAN_ONE: return "7ScanSeg";
-    case PK_COMPACT: return "11CompactArgs";
-    case PK_INSERT: case PK_FIXMIN: case PK_RESOLVE: return "9DenseArgs";
-    case PK_SINGLE_LEAF: return "k_single_leaf";
-    case PK_SWEEP_A_M3: case PK_SWEEP_A_MD5: case PK_SWEEP_B_M3: case PK_SWEEP_B_MD5: return "9SweepArgs";
-    case PK_SCATTER_ROOTS: return "8RootArgs";
-    case PK_SHIFT_SRC: return "9ShiftArgs";
-    case PK_SORT_COUNT: case PK_SORT_MOVE: return "8SortArgs";
-    case PK_REGION_SIZES: return "10RegionArgs";
-    case PK_ASSEMBLE: return "8DiffArgs";
-    case PK_BASIC_DIFF: return "13BasicDiffArgs";
-    default: return nullptr;
-  }
-}
-
-// The persistent pipeline launch: its tables, counters and every phase's launch.
-void check_pipeline(const std::string& name, const PipeArgs& P) {
-  const uint64_t cells = (uint64_t)P.nmem * P.nph;
-  bool ok = need(name, "ph", P.ph, sizeof(PhaseRec) * (uint64_t)P.nph);
-  ok = need(name, "wait", P.wait, 4 * cells) && ok;
-  ok = need(name, "need", P.need, 4 * cells) && ok;
-  ok = need(name, "sweeps_end", P.sweeps_end, 4ull * P.nmem) && ok;
-  ok = need(name, "ctl", P.ctl, 8) && ok;
-  ok = need(name, "done", P.done, 4 * cells) && ok;
-  ok = need(name, "stamp", P.stamp, 24ull * P.nmem) && ok;
-  if (!ok || !P.nph) return;
-  if (reinterpret_cast<uintptr_t>(P.stamp) % 8) violation(name + ": time stamps are not 8-byte aligned");
-  std::vector<uint32_t> want(cells, 0);
-  uint32_t first = 0;
-  for (uint32_t p = 0; p < P.nph; p++) {
-    const PhaseRec& R = P.ph[p];
-    const std::string w = name + "[phase " + std::to_string(p) + "]";
-    const char* type = phase_type(R.kind);
-    if (!type) {
-      violation(w + ": unknown body");
-      continue;
-    }
-    static const char* const kNames[] = {"scan_reduce", "scan_top", "scan_down", "scan_one", "compact", "insert",
-                                         "fixmin", "resolve", "single_leaf", "sweep_a", "sweep_a", "sweep_b",
-                                         "sweep_b", "scatter_roots", "shift_src", "sort_count", "sort_move",
-                                         "region_sizes", "assemble", "basic_diff"};
-    S().launches.push_back(std::string("k_pipeline phase ph_") + kNames[R.kind]);
-    if (R.first != first) violation(w + ": tickets are not contiguous");
-    first += R.items;
-    LaunchRaw raw{P.base + R.args, reinterpret_cast<const uint32_t*>(P.base + R.prefix),
-                  R.count > 1 ? reinterpret_cast<const uint32_t*>(P.base + R.group) : nullptr, R.count};
-    uint8_t launch[32] = {};
-    std::memcpy(launch, &raw, sizeof(raw));
-    check_launch(w + " " + type, launch, R.items);
-    const uint32_t* owner = reinterpret_cast<const uint32_t*>(P.base + R.owner);
-    if (!need(w, "owner", owner, 4ull * R.count)) continue;
-    for (uint32_t m = 0; m < R.count; m++) {
-      if (owner[m] >= P.nmem) {
-        violation(w + ": owner past the step's members");
-        continue;
-      }
-      want[(uint64_t)owner[m] * P.nph + p] += raw.prefix[m + 1] - raw.prefix[m];
-    }
-  }
-  if (first != P.total) violation(name + ": ticket count differs from the phases' items");
-  for (uint32_t m = 0; m < P.nmem; m++) {
-    uint32_t last = ~0u;
-    for (uint32_t p = 0; p < P.nph; p++) {
-      const uint64_t c = (uint64_t)m * P.nph + p;
-      if (P.need[c] != want[c]) violation(name + ": a member's item count differs from its blocks");
-      if (!want[c]) continue;
-      if (P.wait[c] != last) violation(name + ": a phase does not wait for the member's previous phase");
-      last = p;
-    }
-  }
-}
-
 void check(const void* fn, dim3 grid, dim3 block, void** args) {
   auto& st = S();
   auto it = st.names.find(fn);
@@ -391,10 +316,6 @@ void check(const void* fn, dim3 grid, dim3 block, void** args) {
   if (block.x * block.y * block.z > 1024) violation(name + ": block too large");
   if (name.find("6Launch") != std::string::npos) {
     check_launch(name, static_cast<const uint8_t*>(args[0]), grid.x);
-  } else if (name.find("k_pipeline") != std::string::npos) {
-    PipeArgs P;
-    std::memcpy(&P, args[0], sizeof(P));
-    check_pipeline(name, P);
   } else if (name.find("k_fill_u64") != std::string::npos) {
     uint64_t* p;
     uint64_t count;
@@ -481,12 +402,11 @@ hipError_t hipLaunchKernel(const void* fn, dim3 grid, dim3 block, void** args, s
     nd.grid = grid;
     nd.block = block;
     auto it = st.names.find(fn);
-    const bool pipe = it != st.names.end() && it->second.find("k_pipeline") != std::string::npos;
-    if (it == st.names.end() || (it->second.find("6Launch") == std::string::npos && !pipe)) {
+    if (it == st.names.end() || it->second.find("6Launch") == std::string::npos) {
       violation("a kernel without a Launch argument was captured into a graph");
       return hipSuccess;
     }
-    std::memcpy(nd.arg, args[0], pipe ? sizeof(PipeArgs) : 32);
+    std::memcpy(nd.arg, args[0], sizeof(nd.arg));
     st.captured.push_back(nd);
     return hipSuccess;
   }
@@ -589,10 +509,7 @@ hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t attr, int) {
   }
   return hipSuccess;
 }
-hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t) {
-  *n = 4;
-  return hipSuccess;
-}
+
 hipError_t hipGetLastError() { return hipSuccess; }
 const char* hipGetErrorString(hipError_t) { return "fake HIP error"; }
 

@@ -86,18 +86,8 @@ def test_checker_flags_the_unallocated_table_reset(fake):
     L.fakehip_reset()
 
 
-@pytest.fixture(params=["pipeline", "launches"])
-def mode(request, monkeypatch):
-    """The phases after the leaf kernel as one persistent launch, or one launch each."""
-    if request.param == "launches":
-        monkeypatch.setenv("MTC_PIPELINE", "launches")
-    else:
-        monkeypatch.delenv("MTC_PIPELINE", raising=False)
-    return request.param
-
-
 @pytest.mark.parametrize("C,size", [(128, 1000 * 128 + 5), (64, 64 * 64), (16, 16 * 3 + 1), (100, 100 * 777)])
-def test_basic_steps_launch_in_bounds(fake, mode, C, size):
+def test_basic_steps_launch_in_bounds(fake, C, size):
     L = fake
     L.fakehip_reset()
     h = create(L, C, BASIC)
@@ -122,7 +112,7 @@ def test_basic_steps_launch_in_bounds(fake, mode, C, size):
     (100, 100 * 777, 1, (2, 0.125, 5)),
     (4096, 4096 * 300, 0, None),
 ])
-def test_tree_steps_launch_in_bounds(fake, mode, C, size, hk, tuning):
+def test_tree_steps_launch_in_bounds(fake, C, size, hk, tuning):
     L = fake
     L.fakehip_reset()
     h = create(L, C, TREE, hk)
@@ -139,7 +129,7 @@ def test_tree_steps_launch_in_bounds(fake, mode, C, size, hk, tuning):
     L.mtc_destroy(h)
 
 
-def test_batch_of_tree_and_basic_members_with_graph_replay(fake, mode):
+def test_batch_of_tree_and_basic_members_with_graph_replay(fake):
     L = fake
     L.fakehip_reset()
     b = ctypes.c_void_p()
