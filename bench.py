@@ -42,7 +42,11 @@ def parse():
     p.add_argument("--load-factor", type=float, default=0.125, help="digest-table entries per slot at a baseline")
     p.add_argument("--tiles-per-block", type=int, default=0, help="leaf-kernel tiles per block (0: the library default)")
     p.add_argument("--sweep", action="store_true",
-                   help="chunk sizes {64,128,512,4096} x {murmur3,md5} x change {0.1,2,20}%%, one JSON line each")
+                   help="only the sweep: chunk sizes {64,128,512,4096} x {murmur3,md5} x change {0.1,2,20}%%, "
+                        "one JSON line each")
+    p.add_argument("--no-sweep", action="store_true", help="skip the sweep in the default run")
+    p.add_argument("--sweep-steps", type=int, default=3, help="timed checkpoints per sweep point")
+    p.add_argument("--sweep-warmup", type=int, default=1, help="untimed checkpoints per sweep point (the baseline)")
     p.add_argument("--no-batched", action="store_true", help="skip the batched multi-buffer measurement")
     p.add_argument("--no-small-series", action="store_true", help="skip the 64 MiB ten-checkpoint series")
     p.add_argument("--only-batched", action="store_true", help="profiling aid: run the batched measurement alone")
@@ -134,6 +138,20 @@ def leaf_traffic(nbytes: int, C: int, hash_name: str, change: float):
     return int(fields["traffic_bytes_per_dispatch"])
 
 
+def leaf_profile(nbytes: int, C: int, hash_name: str, change: float):
+    """(summary file, average k_leaf1 duration in ms) from the committed
+    rocprofv3 --kernel-trace --stats summary of this workload, or None."""
+    name = f"checkpoint_kernels_{nbytes >> 30}gib_c{C}_{hash_name}" + ("" if change == 0.02 else f"_{change * 100:g}pct")
+    path = os.path.join(ROOT, "profiles", name + ".md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("| `mtc::k_leaf1<"):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            return f"profiles/{name}.md", round(float(cells[2]) / 1e3, 4)
+    return None
+
+
 def header_tuple(h):
     return (h.ref_id, h.chkpt_id, h.datalen, h.chunk_size, h.num_first_ocur, h.num_prior_chkpts, h.num_shift_dupl)
 
@@ -141,8 +159,8 @@ def header_tuple(h):
 def batched_bench(args):
     """Many independent buffers of uneven sizes: one batched step (graph-replayed
     batched launches) against one checkpoint call per buffer, same inputs; the
-    baseline step is untimed.  Headers and sizes of every member are compared each
-    step, and full diffs of a sample of members at the end."""
+    baseline step is untimed.  Every member's whole diff is compared between the
+    two forms at every step, on the device (outside the timed calls)."""
     import torch
 
     import mtcheck_amd as M
@@ -158,7 +176,7 @@ def batched_bench(args):
     P = [M.TreeDeduplicator(s.chunk_size) for s in members]
     batch = M.Batch(use_graph=True)
     tb, tp, lat_b = [], [], []
-    exact = True
+    exact, compared = True, 0
     for k in range(args.batch_steps):
         for s, b in zip(members, bufs):
             s.apply_device(b, k)
@@ -173,9 +191,10 @@ def batched_bench(args):
             tp.append(t2 - t1)
         for h, sz, (hs, _, szs) in zip(hdrs, sizes, solo):
             exact &= header_tuple(h) == header_tuple(hs) and sz == szs
-    sample = range(0, len(members), max(1, len(members) // 16))
-    for i in sample:
-        exact &= A[i].copy_last_diff() == P[i].copy_last_diff()
+        # every member's whole diff, batched against one call per buffer, on the device
+        for a, p in zip(A, P):
+            exact &= bool(torch.equal(a.copy_last_diff_device(), p.copy_last_diff_device()))
+        compared += len(A)
     vb = total * len(tb) / sum(tb) / 1e9
     vp = total * len(tp) / sum(tp) / 1e9
     res = {"workload": f"{len(members)} buffers, log-uniform 1 MiB-512 MiB ({total / 2**30:.1f} GiB), chunk size "
@@ -183,7 +202,7 @@ def batched_bench(args):
            "value": round(vb, 2), "unit": "GB/s", "per_buffer_value": round(vp, 2), "speedup": round(vb / vp, 3),
            "ms_per_step": round(sum(tb) / len(tb) * 1e3, 3), "per_buffer_ms_per_step": round(sum(tp) / len(tp) * 1e3, 3),
            "step_ms": [round(t * 1e3, 2) for t in tb], "per_buffer_step_ms": [round(t * 1e3, 2) for t in tp],
-           "bit_exact": bool(exact), "graph": True,
+           "bit_exact": bool(exact), "diffs_compared": compared, "graph": True,
            "tuning": {str(c): list(t) for c, t in sorted(tuned.items())}}
     for d in A + P:
         d.close()
@@ -367,58 +386,136 @@ def served_bench(args):
     worker.join()
     st = srv.stats()
     srv.close()
+    torch.cuda.empty_cache()
+    check = served_check(clients, bufs, plans, step, K)
     del bufs, plans
     torch.cuda.empty_cache()
     return {"workload": f"{len(clients)} clients, log-uniform 4 MiB-2 GiB ({total / 2**30:.1f} GiB), 128 B chunks, "
                         f"murmur3, change U[0.1%,10%], Poisson arrivals, {args.serve_seconds:g} s per load, "
                         f"{args.serve_lanes} lanes, budget 200 GiB",
             "levels": levels, "steps": st["steps"], "served": st["served"],
-            "mean_batch": round(st["served"] / max(1, st["steps"]), 2)}
+            "mean_batch": round(st["served"] / max(1, st["steps"]), 2), **check}
 
 
-def host_path_runs(versions, C: int, hk: int, warmup: int, steps: int, diff_max: int, tune) -> dict:
-    """The same checkpoints with each diff handed to pinned host memory: serially
-    (checkpoint, then a synchronous copy) and overlapped (the copy of checkpoint k
-    runs during checkpoint k+1; two pinned buffers, a buffer is reused once its copy
-    has finished).  PCIe-inclusive rates; the device-resident rate is `value`."""
+class Versions:
+    """Versions of a checkpoint series in two alternating device buffers: version k
+    is built in buffer k % 2 from version k - 2 by applying steps k - 1 and k, so a
+    run of any length holds two buffers.  Uploaded step plans are cached and shared
+    between runs of the same series."""
+
+    def __init__(self, series, device, plans=None):
+        import torch
+        from mtcheck_amd.workload import DevicePlan
+        self.series, self.device = series, device
+        self.plans = {} if plans is None else plans
+        self._plan = lambda j: self.plans.setdefault(j, DevicePlan(series, series.plan(j), device))
+        self.buf = [torch.empty(series.nbytes, dtype=torch.uint8, device=device) for _ in range(2)]
+        self.have = [-1, -1]
+
+    def get(self, k: int):
+        b = k % 2
+        if self.have[b] > k:
+            raise ValueError("versions are produced in increasing order")
+        for j in range(self.have[b] + 1, k + 1):
+            self.series.apply_device(self.buf[b], j, plan=self._plan(j))
+        self.have[b] = k
+        return self.buf[b]
+
+
+def served_check(clients, bufs, plans, step, K, rounds: int = 4, groups: int = 2) -> dict:
+    """Served diffs against serial per-client dedup, untimed: the clients in
+    `groups` halves, each on a fresh server (ids from 0) beside a serial
+    deduplicator per client.  Every round submits all clients of the half at once
+    (so their checkpoints run mixed in the server's batched steps), then checkpoints
+    the same buffers serially and compares each client's whole diff on the device."""
     import torch
 
     import mtcheck_amd as M
-    nbytes = versions[0].numel()
+    from mtcheck_amd.serve import Server
+    from mtcheck_amd.workload import stream_key
+    exact, checked = True, 0
+    for g in range(groups):
+        members = [i for i in range(len(clients)) if i % groups == g]
+        srv = Server(lanes=2, max_batch=64, memory_budget=200 << 30)
+        cids = {i: srv.add_client(clients[i].chunk_size, clients[i].nbytes) for i in members}
+        serial = {i: M.TreeDeduplicator(clients[i].chunk_size) for i in members}
+        for r in range(rounds):
+            if r:
+                for i in members:
+                    step[i] += 1
+                    plans[i][(step[i] - 1) % K].apply(bufs[i], stream_key(clients[i].seed, step[i]), False)
+                torch.cuda.synchronize()
+            tickets = [srv.submit(cids[i], bufs[i], make_baseline=(r == 0)) for i in members]
+            for t in tickets:
+                srv.result(t)
+            for i in members:
+                serial[i].checkpoint_device(bufs[i], make_baseline=(r == 0))
+                exact &= bool(torch.equal(srv.client(cids[i]).copy_last_diff_device(),
+                                          serial[i].copy_last_diff_device()))
+                checked += 1
+        for d in serial.values():
+            d.close()
+        srv.close()
+        torch.cuda.empty_cache()
+    return {"bit_exact_vs_serial": bool(exact), "checked_vs_serial": checked}
+
+
+def host_path_runs(series, C: int, hk: int, warmup: int, steps: int, diff_max: int, tune, plans) -> dict:
+    """The same checkpoints with each diff handed to pinned host memory: serially
+    (checkpoint, then a synchronous copy) and overlapped (the copy of checkpoint k
+    runs during checkpoint k+1; two pinned buffers, a buffer is reused once its copy
+    has finished).  PCIe-inclusive rates; the device-resident rate is `value`.
+    Versions are generated between the timed checkpoints; the first asynchronous
+    copy (which allocates the handle's second diff arena) is made in the warm-up."""
+    import torch
+
+    import mtcheck_amd as M
+    nbytes = series.nbytes
     cap = int(diff_max * 1.5) + (1 << 20)
     pinned = [torch.empty(cap, dtype=torch.uint8, pin_memory=True) for _ in range(2)]
     out = {}
     for mode in ("serial", "overlapped"):
+        vs = Versions(series, "cuda:0", plans)
         d = M.TreeDeduplicator(C, hash_kind=hk)
         if tune:
             d.set_tuning(*tune)
         for k in range(warmup):
-            d.checkpoint_device(versions[k], make_baseline=(k == 0))
+            d.checkpoint_device(vs.get(k), make_baseline=(k == 0))
+        if mode == "overlapped":
+            d.copy_last_diff_async(pinned[0])
+            d.wait_copies()
         torch.cuda.synchronize()
-        copy_ms = []
-        t0 = time.perf_counter()
+        copy_ms, elapsed = [], 0.0
         for j, k in enumerate(range(warmup, warmup + steps)):
-            d.checkpoint_device(versions[k])
+            buf = vs.get(k)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            d.checkpoint_device(buf)
             if mode == "serial":
                 d.copy_last_diff_device(pinned[0])
                 copy_ms.append(d.stats().ms_copy)
             else:
-                if j >= 2:
-                    d.wait_copies()  # pinned[j % 2] is free again
-                d.copy_last_diff_async(pinned[j % 2])
+                if j >= 1:
+                    d.wait_copies()  # the copy of checkpoint k-1 (into the other pinned buffer) is done
+                d.copy_last_diff_async(pinned[(j + 1) % 2])
+            elapsed += time.perf_counter() - t0
+        t0 = time.perf_counter()
         d.wait_copies()
-        elapsed = time.perf_counter() - t0
+        elapsed += time.perf_counter() - t0
         out[f"{mode}_gbs"] = round(nbytes * steps / elapsed / 1e9, 2)
         if copy_ms:
             out["d2h_ms_mean"] = round(sum(copy_ms) / len(copy_ms), 3)
         d.close()
+        del vs
+        torch.cuda.empty_cache()
     return out
 
 
 def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, warmup: int,
               load_factor: float = 0.125, tiles_per_block: int = 0, host_path: bool = False) -> dict:
-    """Incremental checkpoints of one buffer resident in HBM; every version is
-    generated before the timed region (version k = version k-1 + step k)."""
+    """Incremental checkpoints of one buffer resident in HBM.  Version k = version
+    k-1 + step k, generated in one of two alternating buffers between the timed
+    checkpoints; each timed checkpoint starts and ends on an idle device."""
     import numpy as np
     import torch
 
@@ -428,26 +525,20 @@ def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, wa
     n = (nbytes + C - 1) // C
     total = warmup + steps
     series = CheckpointSeries.with_change(nbytes, C, change)
-    dev = torch.device("cuda:0")
-    versions = []
-    for k in range(total):
-        v = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-        if k:
-            v.copy_(versions[-1])
-        series.apply_device(v, k)
-        versions.append(v)
-    torch.cuda.synchronize()
+    plans = {}
+    vs = Versions(series, torch.device("cuda:0"), plans)
     d = M.TreeDeduplicator(C, hash_kind=hk)
     if tiles_per_block or load_factor != 0.125:
         d.set_tuning(tiles_per_block or 1, load_factor, 8)
     for k in range(warmup):
-        d.checkpoint_device(versions[k], make_baseline=(k == 0))
+        d.checkpoint_device(vs.get(k), make_baseline=(k == 0))
     torch.cuda.synchronize()
     lat, leaf_ms, changed, balg, dsize = [], [], [], [], []
-    t0 = time.perf_counter()
     for k in range(warmup, total):
+        buf = vs.get(k)
+        torch.cuda.synchronize()
         s0 = time.perf_counter()
-        hdr, _, size = d.checkpoint_device(versions[k])
+        hdr, _, size = d.checkpoint_device(buf)
         lat.append(time.perf_counter() - s0)
         st = d.stats()
         leaf_ms.append(st.ms_leaf)
@@ -455,22 +546,31 @@ def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, wa
         dsize.append(size)
         meta = 32 + 4 * hdr.num_first_ocur + 8 * hdr.num_prior_chkpts + 8 * hdr.num_shift_dupl
         balg.append(nbytes + 32 * n + 2 * (size - meta) + meta)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    elapsed = sum(lat)
     # Leaf kernel: C B read + 16 B previous digest read + 1 B label per chunk,
     # 16 B digest + 4 B slot written per changed chunk.
     leaf_bytes = [nbytes + 17 * n + 20 * c for c in changed]
     leaf_avg_ms = float(np.mean(leaf_ms))
     achieved = float(np.mean(leaf_bytes)) / (leaf_avg_ms / 1e3)
     d.close()
+    del vs
+    torch.cuda.empty_cache()
     extra = {}
     if host_path:
         tune = (tiles_per_block or 1, load_factor, 8) if (tiles_per_block or load_factor != 0.125) else None
-        hp = host_path_runs(versions, C, hk, warmup, steps, max(dsize), tune)
+        hp = host_path_runs(series, C, hk, warmup, steps, max(dsize), tune, plans)
         hp["d2h_gbs"] = round(float(np.mean(dsize)) / (hp["d2h_ms_mean"] / 1e3) / 1e9, 2) if hp.get("d2h_ms_mean") else None
         extra["host_path"] = {"workload": "the same checkpoints, each diff copied to pinned host memory", **hp}
-    del versions
+    del plans
     torch.cuda.empty_cache()
+    prof = leaf_profile(nbytes, C, hash_name, change)
+    roof = {"kernel": "k_leaf1 (hash + fixed test + label + list of chunks to insert)", "bound": "hbm",
+            "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK, 4), "traffic": leaf_traffic(nbytes, C, hash_name, change),
+            "avg_ms": round(leaf_avg_ms, 4)}
+    if prof:
+        roof["profile"] = {"summary": prof[0], "avg_ms": prof[1],
+                           "frac": round(float(np.mean(leaf_bytes)) / (prof[1] / 1e3) / HBM_PEAK, 4)}
     return {**extra,
         "value": round(nbytes * steps / elapsed / 1e9, 3), "ms_per_step": round(elapsed / steps * 1e3, 4),
         "config": {"workload": f"plain single-buffer dedup, {nbytes / 2**30:g} GiB uint8, {C} B chunks, "
@@ -480,24 +580,29 @@ def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, wa
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "diff_mib_mean": round(float(np.mean(dsize)) / (1 << 20), 2),
-        "roofline": {"kernel": "k_leaf (hash + fixed test + label + list of chunks to insert)", "bound": "hbm",
-                     "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": leaf_traffic(nbytes, C, hash_name, change),
-                     "avg_ms": round(leaf_avg_ms, 4)},
+        "roofline": roof,
     }
 
 
-def sweep(args):
-    """Chunk sizes x hashes x change rates on one buffer; one JSON line each."""
+def sweep(args, emit=None):
+    """The 8 GiB sweep: chunk sizes {64,128,512,4096} x {murmur3, md5} x change
+    {0.1, 2, 20}%, each point a baseline, sweep_warmup - 1 untimed and sweep_steps
+    timed incremental checkpoints.  Returns one record per point (and passes each to
+    `emit` as it is measured)."""
     nbytes = int(args.size_gib * (1 << 30))
+    out = []
     for C in (64, 128, 512, 4096):
         for h in ("murmur3", "md5"):
             for ch in (0.001, 0.02, 0.2):
-                r = plain_run(nbytes, C, h, ch, args.steps, args.warmup)
-                print(json.dumps({"config": r["config"], "value": r["value"], "unit": "GB/s",
-                                  "ms_per_step": r["ms_per_step"], "hbm_frac_checkpoint": r["hbm_frac_checkpoint"],
-                                  "leaf_gbs": r["roofline"]["achieved"], "leaf_frac": r["roofline"]["frac"],
-                                  "leaf_ms": r["roofline"]["avg_ms"], "diff_mib_mean": r["diff_mib_mean"]}), flush=True)
+                r = plain_run(nbytes, C, h, ch, args.sweep_steps, args.sweep_warmup)
+                rec = {"chunk": C, "hash": h, "change": ch, "value": r["value"], "unit": "GB/s",
+                       "ms_per_step": r["ms_per_step"], "hbm_frac_checkpoint": r["hbm_frac_checkpoint"],
+                       "leaf_gbs": r["roofline"]["achieved"], "leaf_frac": r["roofline"]["frac"],
+                       "leaf_ms": r["roofline"]["avg_ms"], "diff_mib_mean": r["diff_mib_mean"]}
+                out.append(rec)
+                if emit:
+                    emit(rec)
+    return out
 
 
 def small_series():
@@ -547,7 +652,7 @@ def main():
         return
     import mtcheck_amd as M
     if args.sweep:
-        sweep(args)
+        sweep(args, emit=lambda rec: print(json.dumps(rec), flush=True))
         return
     r = plain_run(int(args.size_gib * (1 << 30)), args.chunk, args.hash, args.change, args.steps, args.warmup,
                   args.load_factor, args.tiles_per_block, host_path=not args.no_host_path)
@@ -569,6 +674,10 @@ def main():
         out["restart"] = restart_bench(args)
     if not args.no_served:
         out["served"] = served_bench(args)
+    if not args.no_sweep:
+        out["sweep"] = {"workload": f"{args.size_gib:g} GiB uint8, the headline generator, chunk x hash x change; "
+                                    f"per point a baseline and {args.sweep_steps} timed checkpoints",
+                        "points": sweep(args)}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args, hk)
     print(json.dumps(out), flush=True)

@@ -170,6 +170,16 @@ int mtc_digest_host(const void* host_data, uint64_t len, int hash_kind, uint8_t 
 int mtc_hash_chunks(const void* dev_data, uint64_t len, uint32_t chunk_size, int hash_kind, void* dev_digests,
                     void* stream);
 
+/* Reference-identical bytes where this library departs from them on purpose
+ * (from the next checkpoint on; off by default).  With on != 0 a first-occurrence
+ * region stores the reference's leaf range (num_leaf_descendents/leftmost_leaf,
+ * include/kokkos_merkle_tree.hpp:102-128) instead of the root's true leaf set,
+ * and a buffer equal to an earlier checkpoint drops shifted root 0
+ * (include/reference_impl.hpp:105-106).  The two differ only when the chunk count
+ * is not a power of two, or for that shifted root; there the reference's diff
+ * does not restore the buffer, and neither does mtc_restart. */
+int mtc_set_compat(mtc_dedup* h, int on);
+
 /* Debug views, as the serial reference leaves them (include/reference_impl.hpp
  * :121): the full tree (16 * (2n-1) bytes, zero where no digest was computed) and
  * every node's label.  Enable before a checkpoint with mtc_set_debug(h, 1). */

@@ -100,6 +100,8 @@ class TreeDeduplicator {
   TreeDeduplicator(const TreeDeduplicator&) = delete;
   TreeDeduplicator& operator=(const TreeDeduplicator&) = delete;
 
+  // mtc_set_compat: the reference's bytes where this library departs from them.
+  void set_compat(bool on) { check(mtc_set_compat(h_, on ? 1 : 0), h_); }
   size_t num_first_ocur() const { return stats().num_first_ocur; }  // include/tree_approach.hpp:53-55
   size_t num_shift_dupl() const { return stats().num_shift_dupl; }  // include/tree_approach.hpp:57-59
   uint32_t current_id() const { return stats().current_id; }

@@ -24,7 +24,7 @@ STATUS_NAMES = {
 EXPORTS = [
     "mtc_create", "mtc_destroy", "mtc_checkpoint", "mtc_diff_device", "mtc_copy_diff", "mtc_copy_diff_async",
     "mtc_copy_wait", "mtc_get_stats",
-    "mtc_restart", "mtc_restart_basic", "mtc_restart_noalloc", "mtc_restart_times", "mtc_digest_host", "mtc_hash_chunks", "mtc_set_debug", "mtc_copy_tree", "mtc_copy_labels",
+    "mtc_restart", "mtc_restart_basic", "mtc_restart_noalloc", "mtc_restart_times", "mtc_digest_host", "mtc_hash_chunks", "mtc_set_compat", "mtc_set_debug", "mtc_copy_tree", "mtc_copy_labels",
     "mtc_last_error", "mtc_status_string", "mtc_set_tuning", "mtc_batch_create", "mtc_batch_destroy",
     "mtc_checkpoint_batch", "mtc_batch_last_error", "mtc_server_create", "mtc_server_destroy", "mtc_server_add_client",
     "mtc_server_client", "mtc_server_remove_client", "mtc_server_submit", "mtc_server_result", "mtc_server_stats",

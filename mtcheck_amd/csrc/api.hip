@@ -273,6 +273,12 @@ int mtc_hash_chunks(const void* dev_data, uint64_t len, uint32_t chunk_size, int
   });
 }
 
+int mtc_set_compat(mtc_dedup* h, int on) {
+  if (!h) return MTC_ERR_INVALID_ARG;
+  h->compat = on ? 1 : 0;
+  return MTC_OK;
+}
+
 int mtc_set_debug(mtc_dedup* h, int on) {
   if (!h) return MTC_ERR_INVALID_ARG;
   h->debug = on ? 1 : 0;

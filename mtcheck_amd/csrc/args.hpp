@@ -136,6 +136,7 @@ struct SweepArgs {
   const uint8_t* tile_first;  // stage 0: tile holds a first-occurrence leaf
   const uint8_t* tile_busy;   // stage 0: tile holds a leaf that is not a fixed duplicate
   uint32_t tiles_per_wave;     // sweep A stage 0: consecutive tiles one wavefront walks (<= 64)
+  int compat;                  // node 0 is a root only when it is a first occurrence
   __device__ SweepArgs global() const {
     SweepArgs c = *this;
     MTC_GLOBAL(leaf_dg); MTC_GLOBAL(labels); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(in_lab); MTC_GLOBAL(in_dg); MTC_GLOBAL(in_slot); MTC_GLOBAL(out_lab); MTC_GLOBAL(out_dg); MTC_GLOBAL(out_slot); MTC_GLOBAL(tab); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(staging); MTC_GLOBAL(cnt_f); MTC_GLOBAL(cnt_s); MTC_GLOBAL(seg_base); MTC_GLOBAL(dbg_tree); MTC_GLOBAL(dbg_labels); MTC_GLOBAL(tile_first); MTC_GLOBAL(tile_busy);
@@ -208,6 +209,7 @@ struct RegionArgs {
   const uint32_t* F;
   Geo g;
   uint32_t* sizes;
+  int compat;     // the reference's leaf ranges
   PtableArgs pt;  // the (chkpt, count) table, built by block 0
   __device__ RegionArgs global() const {
     RegionArgs c = *this;
@@ -231,6 +233,7 @@ struct DiffArgs {
   Geo g;
   Ctl* ctl;
   int aligned;
+  int compat;                     // the reference's leaf ranges
   const unsigned long long* ctr;  // striped counters
   const uint32_t* changed;        // chunks inserted by the leaf pass
   Ctl* out;                       // the member's entry of the step's result array

@@ -1228,8 +1228,9 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
     }
     if (a.d_out == 0) {
       // Node 0: a first-occurrence root (:105-106); a shifted root too, which the
-      // reference omits and so cannot restart (documented departure).
-      const bool root = is_region(lc);
+      // reference omits and so cannot restart (documented departure), unless the
+      // reference's bytes are asked for.
+      const bool root = a.compat ? lc == L_FIRST : is_region(lc);
       if (root) a.staging[0] = ((lc == L_FIRST ? kFirstTag : (uint64_t)sc) << 32) | 0u;
       a.cnt_f[a.seg_base[0]] = (root && lc == L_FIRST) ? 1u : 0u;
       a.cnt_s[a.seg_base[0]] = (root && lc == L_SHIFT) ? 1u : 0u;
@@ -1259,7 +1260,7 @@ __device__ __forceinline__ void ph_single_leaf(const Launch<SweepArgs>& L, const
   if (threadIdx.x != 0) return;
   const SweepArgs a = L.args[w.m].global();
   const uint8_t l = a.labels[0];
-  const bool root = is_region(l);
+  const bool root = a.compat ? l == L_FIRST : is_region(l);
   if (root) a.staging[0] = (l == L_FIRST ? (kFirstTag << 32) : ((uint64_t)a.leaf_slot[0] << 32)) | 0u;
   a.cnt_f[0] = (root && l == L_FIRST) ? 1u : 0u;
   a.cnt_s[0] = (root && l == L_SHIFT) ? 1u : 0u;
@@ -1450,8 +1451,11 @@ __device__ __forceinline__ void ph_region_sizes(const Launch<RegionArgs>& L, con
   const RegionArgs a = L.args[w.m].global();
   if (w.blk == 0) build_ptable(a.pt, s_ws);
   const uint32_t F = *a.F;
-  for (uint32_t i = w.blk * kThreads + threadIdx.x; i < F; i += w.nblk * kThreads)
-    a.sizes[i] = leaf_count(a.g, a.first_roots[i]);
+  for (uint32_t i = w.blk * kThreads + threadIdx.x; i < F; i += w.nblk * kThreads) {
+    uint32_t st[2], ct[2];
+    const int r = region_runs(a.g, a.first_roots[i], a.compat, st, ct);
+    a.sizes[i] = r == 0 ? 0 : (r == 1 ? ct[0] : ct[0] + ct[1]);
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void k_region_sizes(Launch<RegionArgs> L) {
@@ -1527,7 +1531,7 @@ __device__ __forceinline__ void ph_assemble(const Launch<DiffArgs>& L, const Whe
         else hi = mid;
       }
       uint32_t st[2], ct[2];
-      const int r = leaf_runs(a.g, a.first_roots[lo], st, ct);
+      const int r = region_runs(a.g, a.first_roots[lo], a.compat, st, ct);
       s_src[threadIdx.x] = run_at(r, st, ct, (uint32_t)(j - a.region_off[lo]));
     }
     __syncthreads();
@@ -2219,6 +2223,7 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       sa.dbg_labels = h->debug ? h->dbg_labels.p : nullptr;
       sa.tile_first = h->tile_flags.p;
       sa.tile_busy = h->tile_flags.p + M.ntile0;
+      sa.compat = h->compat;
       max_stages = std::max(max_stages, h->stages.size());
     }
     // single-chunk members
@@ -2344,7 +2349,7 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
     std::vector<ScanSeg> ss;
     for (auto& M : ms) {
       mtc_dedup* h = M.h;
-      ra.push_back(RegionArgs{h->first_roots.p, &h->ctl_d->F, M.g, h->region_off.p,
+      ra.push_back(RegionArgs{h->first_roots.p, &h->ctl_d->F, M.g, h->region_off.p, h->compat,
                               PtableArgs{h->src_hist.p, M.nbins, h->baseline_id, h->ptable.p, h->ctl_d}});
       rb.push_back(grid_for(M.n, 16 * kThreads, 256));
       ss.push_back(ScanSeg{h->region_off.p, h->region_off.p, &h->ctl_d->F, M.n, h->region_off.p + M.n, M.ws_reg});
@@ -2372,6 +2377,7 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       d.g = M.g;
       d.ctl = h->ctl_d;
       d.aligned = ((h->chunk_size % 16) == 0 && (uintptr_t)M.data % 16 == 0) ? 1 : 0;
+      d.compat = h->compat;
       d.ctr = h->ctr.p;
       d.changed = h->blk_off.p + M.nblk_leaf;
       d.out = M.ctl_out;

@@ -542,6 +542,32 @@ __host__ __device__ inline int leaf_runs(const Geo& g, uint32_t node, uint32_t s
   return r;
 }
 
+// The reference's leaf range of a node (include/kokkos_merkle_tree.hpp:102-128):
+// one run from the leftmost bottom-level leaf under it, as long as the walk to the
+// rightmost one says, clamped to the last node.  Where a node's leaves lie on two
+// depths (a chunk count that is not a power of two) this is not the node's leaf
+// set; it is kept only to reproduce the reference's bytes on request.
+__host__ __device__ inline int ref_leaf_run(const Geo& g, uint32_t node, uint32_t st[2], uint32_t ct[2]) {
+  const uint64_t num_nodes = 2ull * g.n - 1;
+  uint64_t l = 2ull * node + 1, r = 2ull * node + 2;
+  while (l < num_nodes) {
+    l = 2 * l + 1;
+    r = 2 * r + 2;
+  }
+  l = (l - 1) / 2;
+  r = (r - 2) / 2;
+  if (r > num_nodes) r = num_nodes - 1;
+  st[0] = (uint32_t)(l - (g.n - 1));
+  ct[0] = (uint32_t)(r - l + 1);
+  return 1;
+}
+
+// Leaves of a region root as stored in the data section: its true leaf set, or
+// with `compat` the reference's single run.
+__host__ __device__ inline int region_runs(const Geo& g, uint32_t node, int compat, uint32_t st[2], uint32_t ct[2]) {
+  return compat ? ref_leaf_run(g, node, st, ct) : leaf_runs(g, node, st, ct);
+}
+
 __host__ __device__ inline uint32_t leaf_count(const Geo& g, uint32_t node) {
   uint32_t st[2], ct[2];
   int r = leaf_runs(g, node, st, ct);

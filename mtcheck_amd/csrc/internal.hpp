@@ -160,6 +160,7 @@ struct mtc_dedup {
   std::atomic<uint64_t> table_cap{0};
   uint64_t table_size = 0;  // entries (host mirror, refreshed after every checkpoint)
   int debug = 0;
+  int compat = 0;  // reproduce the reference's bytes where they depart (mtc_set_compat)
 
   mtc::DevBuf<mtc::Digest> leaf_dg;   // persistent leaf digests (the tree's leaves)
   mtc::DevBuf<uint8_t> labels;        // leaf labels of the current checkpoint

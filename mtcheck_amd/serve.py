@@ -30,6 +30,14 @@ class ClientHandle:
         _lib.check(lib().mtc_copy_diff(self._h, buf, size.value, None), self._h)
         return buf.raw
 
+    def copy_last_diff_device(self):
+        """The last diff copied into a new uint8 cuda tensor."""
+        import torch
+        s = self.stats()
+        out = torch.empty(s.diff_size, dtype=torch.uint8, device="cuda:0")
+        _lib.check(lib().mtc_copy_diff(self._h, ctypes.c_void_p(out.data_ptr()), out.numel(), None), self._h)
+        return out
+
     def stats(self):
         """The client's deduplicator statistics (read between its checkpoints)."""
         s = _lib.Stats()

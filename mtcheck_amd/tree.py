@@ -106,6 +106,11 @@ class TreeDeduplicator:
     def set_debug(self, on: bool = True) -> None:
         check(lib().mtc_set_debug(self._h, 1 if on else 0), self._h)
 
+    def set_compat(self, on: bool = True) -> None:
+        """Reference-identical bytes where the library departs from them (the
+        reference's leaf ranges; no shifted root 0), from the next checkpoint on."""
+        check(lib().mtc_set_compat(self._h, 1 if on else 0), self._h)
+
     # -- checkpoint overloads (src/tree_approach.cpp:1577-1776) -----------------
     def _run(self, data, make_baseline: bool, advance: bool, stream=None, copy: bool = True):
         ptr, n = _dev_ptr(data)
