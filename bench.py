@@ -11,6 +11,7 @@ Prints one JSON line (rank 0).  `value` is input GB/s (1e9 B) over the timed ste
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -66,6 +67,16 @@ def parse():
     p.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                    help="only run a plain device copy of GIB GiB three times (scales the HBM byte counters)")
     return p.parse_args()
+
+
+def note(msg: str) -> None:
+    """Progress on stderr: the leg about to run and the device memory free."""
+    try:
+        import torch
+        free = torch.cuda.mem_get_info()[0] / 2**30 if torch.cuda.is_available() else float("nan")
+    except Exception:
+        free = float("nan")
+    print(f"[bench] {msg} (free {free:.1f} GiB)", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(args, hk):
@@ -400,17 +411,25 @@ def served_bench(args):
 class Versions:
     """Versions of a checkpoint series in two alternating device buffers: version k
     is built in buffer k % 2 from version k - 2 by applying steps k - 1 and k, so a
-    run of any length holds two buffers.  Uploaded step plans are cached and shared
-    between runs of the same series."""
+    run of any length holds two buffers.  The step plans of versions 0..upto-1 are
+    drawn and uploaded up front (and shared between runs of the same series), so
+    producing a version between timed checkpoints is device work of a few
+    milliseconds rather than host work that leaves the GPU idle."""
 
-    def __init__(self, series, device, plans=None):
+    def __init__(self, series, device, plans=None, upto: int = 0):
         import torch
-        from mtcheck_amd.workload import DevicePlan
         self.series, self.device = series, device
         self.plans = {} if plans is None else plans
-        self._plan = lambda j: self.plans.setdefault(j, DevicePlan(series, series.plan(j), device))
         self.buf = [torch.empty(series.nbytes, dtype=torch.uint8, device=device) for _ in range(2)]
         self.have = [-1, -1]
+        for j in range(upto):
+            self._plan(j)
+
+    def _plan(self, j: int):
+        from mtcheck_amd.workload import DevicePlan
+        if j not in self.plans:
+            self.plans[j] = DevicePlan(self.series, self.series.plan(j), self.device)
+        return self.plans[j]
 
     def get(self, k: int):
         b = k % 2
@@ -420,6 +439,9 @@ class Versions:
             self.series.apply_device(self.buf[b], j, plan=self._plan(j))
         self.have[b] = k
         return self.buf[b]
+
+    def release(self) -> None:
+        self.buf = []
 
 
 def served_check(clients, bufs, plans, step, K, rounds: int = 4, groups: int = 2) -> dict:
@@ -506,7 +528,7 @@ def host_path_runs(series, C: int, hk: int, warmup: int, steps: int, diff_max: i
         if copy_ms:
             out["d2h_ms_mean"] = round(sum(copy_ms) / len(copy_ms), 3)
         d.close()
-        del vs
+        vs.release()
         torch.cuda.empty_cache()
     return out
 
@@ -526,7 +548,7 @@ def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, wa
     total = warmup + steps
     series = CheckpointSeries.with_change(nbytes, C, change)
     plans = {}
-    vs = Versions(series, torch.device("cuda:0"), plans)
+    vs = Versions(series, torch.device("cuda:0"), plans, upto=total)
     d = M.TreeDeduplicator(C, hash_kind=hk)
     if tiles_per_block or load_factor != 0.125:
         d.set_tuning(tiles_per_block or 1, load_factor, 8)
@@ -553,7 +575,7 @@ def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, wa
     leaf_avg_ms = float(np.mean(leaf_ms))
     achieved = float(np.mean(leaf_bytes)) / (leaf_avg_ms / 1e3)
     d.close()
-    del vs
+    vs.release()
     torch.cuda.empty_cache()
     extra = {}
     if host_path:
@@ -561,7 +583,8 @@ def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, wa
         hp = host_path_runs(series, C, hk, warmup, steps, max(dsize), tune, plans)
         hp["d2h_gbs"] = round(float(np.mean(dsize)) / (hp["d2h_ms_mean"] / 1e3) / 1e9, 2) if hp.get("d2h_ms_mean") else None
         extra["host_path"] = {"workload": "the same checkpoints, each diff copied to pinned host memory", **hp}
-    del plans
+    plans.clear()
+    gc.collect()
     torch.cuda.empty_cache()
     prof = leaf_profile(nbytes, C, hash_name, change)
     roof = {"kernel": "k_leaf1 (hash + fixed test + label + list of chunks to insert)", "bound": "hbm",
@@ -594,11 +617,16 @@ def sweep(args, emit=None):
     for C in (64, 128, 512, 4096):
         for h in ("murmur3", "md5"):
             for ch in (0.001, 0.02, 0.2):
-                r = plain_run(nbytes, C, h, ch, args.sweep_steps, args.sweep_warmup)
-                rec = {"chunk": C, "hash": h, "change": ch, "value": r["value"], "unit": "GB/s",
-                       "ms_per_step": r["ms_per_step"], "hbm_frac_checkpoint": r["hbm_frac_checkpoint"],
-                       "leaf_gbs": r["roofline"]["achieved"], "leaf_frac": r["roofline"]["frac"],
-                       "leaf_ms": r["roofline"]["avg_ms"], "diff_mib_mean": r["diff_mib_mean"]}
+                note(f"sweep point C={C} {h} change={ch}")
+                try:
+                    r = plain_run(nbytes, C, h, ch, args.sweep_steps, args.sweep_warmup)
+                except Exception as e:  # a point that cannot run is reported, not fatal
+                    rec = {"chunk": C, "hash": h, "change": ch, "error": str(e)[:200]}
+                else:
+                    rec = {"chunk": C, "hash": h, "change": ch, "value": r["value"], "unit": "GB/s",
+                           "ms_per_step": r["ms_per_step"], "hbm_frac_checkpoint": r["hbm_frac_checkpoint"],
+                           "leaf_gbs": r["roofline"]["achieved"], "leaf_frac": r["roofline"]["frac"],
+                           "leaf_ms": r["roofline"]["avg_ms"], "diff_mib_mean": r["diff_mib_mean"]}
                 out.append(rec)
                 if emit:
                     emit(rec)
@@ -654,6 +682,7 @@ def main():
     if args.sweep:
         sweep(args, emit=lambda rec: print(json.dumps(rec), flush=True))
         return
+    note("headline")
     r = plain_run(int(args.size_gib * (1 << 30)), args.chunk, args.hash, args.change, args.steps, args.warmup,
                   args.load_factor, args.tiles_per_block, host_path=not args.no_host_path)
     out = {
@@ -667,18 +696,23 @@ def main():
         out["host_path"] = r["host_path"]
     hk = M.MD5 if args.hash == "md5" else M.MURMUR3
     if not args.no_small_series:
+        note("small series")
         out["small_series"] = small_series()
     if not args.no_batched:
+        note("batched")
         out["batched"] = batched_bench(args)
     if not args.no_restart:
+        note("restart")
         out["restart"] = restart_bench(args)
     if not args.no_served:
+        note("served")
         out["served"] = served_bench(args)
     if not args.no_sweep:
         out["sweep"] = {"workload": f"{args.size_gib:g} GiB uint8, the headline generator, chunk x hash x change; "
                                     f"per point a baseline and {args.sweep_steps} timed checkpoints",
                         "points": sweep(args)}
     if not args.no_cpu_baseline:
+        note("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(args, hk)
     print(json.dumps(out), flush=True)
 
