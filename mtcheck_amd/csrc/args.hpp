@@ -65,9 +65,16 @@ struct LeafArgs {
   uint32_t* list;     // chunks to insert, packed per block
   uint32_t* blk_cnt;  // their count per block
   uint64_t tiles_per_block;
+  // 1: label every chunk.  0: the labels already hold L_FIXED for every chunk (the
+  // previous step reset the ones it listed) and no chunk is labelled here: k_insert
+  // labels the listed ones
+  int all_labels;
+  // Tree members: each listed chunk's digest is stored beside its list entry and
+  // k_insert moves it to leaf_dg.  Null: leaf_dg is written here.
+  Digest* list_dg;
   __device__ LeafArgs global() const {
     LeafArgs c = *this;
-    MTC_GLOBAL(data); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(labels); MTC_GLOBAL(list); MTC_GLOBAL(blk_cnt);
+    MTC_GLOBAL(data); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(labels); MTC_GLOBAL(list); MTC_GLOBAL(blk_cnt); MTC_GLOBAL(list_dg);
     return c;
   }
 };
@@ -78,10 +85,12 @@ struct CompactArgs {
   const uint32_t* blk_off;
   uint64_t tiles_per_block;
   uint32_t* dense;
-  uint32_t nblk;  // leaf blocks
+  uint32_t nblk;            // leaf blocks
+  const Digest* list_dg;    // digests beside the list entries (tree members), or null
+  Digest* dense_dg;         // their dense copy
   __device__ CompactArgs global() const {
     CompactArgs c = *this;
-    MTC_GLOBAL(list); MTC_GLOBAL(blk_cnt); MTC_GLOBAL(blk_off); MTC_GLOBAL(dense);
+    MTC_GLOBAL(list); MTC_GLOBAL(blk_cnt); MTC_GLOBAL(blk_off); MTC_GLOBAL(dense); MTC_GLOBAL(list_dg); MTC_GLOBAL(dense_dg);
     return c;
   }
 };
@@ -90,7 +99,7 @@ struct DenseArgs {
   const uint32_t* dense;  // chunks to insert, ascending within each wavefront's run
   const uint32_t* total;  // their number
   uint32_t n;
-  const Digest* leaf_dg;
+  Digest* leaf_dg;         // written here from dense_dg
   uint32_t* leaf_slot;
   uint8_t* labels;
   Slot* tab;
@@ -102,9 +111,11 @@ struct DenseArgs {
   uint32_t R0;          // levels of the first sweep stage
   uint8_t* tile_first;  // per stage-0 tile: holds a first-occurrence leaf
   uint8_t* tile_busy;   // per stage-0 tile: holds a leaf that is not a fixed duplicate
+  const Digest* dense_dg;  // the listed chunks' digests, in dense order
+  int leaf_labels;         // the leaf kernel labelled the chunks (otherwise they count as pending)
   __device__ DenseArgs global() const {
     DenseArgs c = *this;
-    MTC_GLOBAL(dense); MTC_GLOBAL(total); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(labels); MTC_GLOBAL(tab); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(tile_first); MTC_GLOBAL(tile_busy);
+    MTC_GLOBAL(dense); MTC_GLOBAL(total); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(labels); MTC_GLOBAL(tab); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(tile_first); MTC_GLOBAL(tile_busy); MTC_GLOBAL(dense_dg);
     return c;
   }
 };
@@ -135,7 +146,7 @@ struct SweepArgs {
   uint8_t* dbg_labels;
   const uint8_t* tile_first;  // stage 0: tile holds a first-occurrence leaf
   const uint8_t* tile_busy;   // stage 0: tile holds a leaf that is not a fixed duplicate
-  uint32_t tiles_per_wave;     // sweep A stage 0: consecutive tiles one wavefront walks (<= 64)
+  uint32_t tiles_per_wave;     // stage 0: consecutive tiles one wavefront walks (<= 64)
   int compat;                  // node 0 is a root only when it is a first occurrence
   __device__ SweepArgs global() const {
     SweepArgs c = *this;
@@ -237,10 +248,12 @@ struct DiffArgs {
   const unsigned long long* ctr;  // striped counters
   const uint32_t* changed;        // chunks inserted by the leaf pass
   Ctl* out;                       // the member's entry of the step's result array
+  const uint32_t* dense;          // the listed chunks (*changed of them)
+  uint8_t* reset_labels;          // when set: their labels go back to L_FIXED
   __device__ DiffArgs global() const {
     DiffArgs c = *this;
     MTC_GLOBAL(diff); MTC_GLOBAL(first_roots); MTC_GLOBAL(ptable); MTC_GLOBAL(shift_sorted); MTC_GLOBAL(region_off); MTC_GLOBAL(total_chunks); MTC_GLOBAL(data); MTC_GLOBAL(ctl);
-    MTC_GLOBAL(ctr); MTC_GLOBAL(changed); MTC_GLOBAL(out);
+    MTC_GLOBAL(ctr); MTC_GLOBAL(changed); MTC_GLOBAL(out); MTC_GLOBAL(dense); MTC_GLOBAL(reset_labels);
     return c;
   }
 };

@@ -40,6 +40,10 @@ constexpr uint32_t kSortBlocks = 1024;         // most blocks per member of the 
 #define MTC_TPW_A 8
 #endif
 constexpr uint32_t kTilesPerWaveA = MTC_TPW_A;  // stage-0 tiles per wavefront, sweep A (<= 64)
+#ifndef MTC_TPW_B
+#define MTC_TPW_B 8
+#endif
+constexpr uint32_t kTilesPerWaveB = MTC_TPW_B;  // stage-0 tiles per wavefront, sweep B (<= 64)
 
 // ---------------------------------------------------------------------------
 // Batched launches
@@ -383,10 +387,8 @@ __device__ __forceinline__ void leaf_epilogue(const LeafArgs& a, uint64_t t, con
   // A fixed leaf's digest was inserted by an earlier checkpoint of this baseline,
   // so its insert is a no-op except right after a reset, where every leaf inserts.
   const bool ins = valid && (a.baseline || !fixed);
-  if (valid) {
-    a.labels[c] = fixed ? L_FIXED : L_PENDING;
-    if (ins) a.leaf_dg[c] = d;
-  }
+  if (valid && a.all_labels) a.labels[c] = fixed ? L_FIXED : L_PENDING;
+  if (ins && !a.list_dg) a.leaf_dg[c] = d;
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
   const uint64_t m = __ballot(ins);
   if (lane == 0) s_wcnt[w] = (uint32_t)__popcll(m);
@@ -397,7 +399,11 @@ __device__ __forceinline__ void leaf_epilogue(const LeafArgs& a, uint64_t t, con
     base += (i < w) ? s_wcnt[i] : 0u;
     tot += s_wcnt[i];
   }
-  if (ins) region[base + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)c;
+  if (ins) {
+    const uint32_t k = base + __popcll(m & ((1ull << lane) - 1));
+    region[k] = (uint32_t)c;
+    if (a.list_dg) a.list_dg[(region - a.list) + k] = d;
+  }
   run += tot;
   __syncthreads();  // s_wcnt is reused by the next tile
 }
@@ -513,15 +519,19 @@ __global__ __launch_bounds__(kThreads) void k_leaf(Launch<LeafArgs> L) {
 // One 256-chunk tile per block, chunks of 16..128 bytes staged whole: the kernel
 // used when a block takes a single tile.  LDS holds exactly the tile (32 KiB at
 // 128-byte chunks: five blocks per CU), its 16-byte columns swizzled instead of
-// padded: column ^ ((row / (128 / CB)) & (CB / 16 - 1)) gives the eight rows that
-// share a 128-byte bank window distinct columns, so both the row-wise stores and
-// the per-lane reads stay free of bank conflicts.  Each
+// padded: column ^ ((row / (128 / SEG) ^ row / (256 / SEG)) & (SEG / 16 - 1))
+// gives the rows whose 16-byte accesses are served together distinct columns,
+// whether the banks span 128 bytes (8 lanes per pass) or 256 bytes (16 lanes per
+// pass), so both the row-wise stores and the per-lane reads are free of bank
+// conflicts.  (A plain column ^ row swizzle leaves 2-way conflicts on the reads
+// with 256-byte banks: SQ_LDS_BANK_CONFLICT was a quarter of SQ_LDS_IDX_ACTIVE.)
+// Each
 // wavefront reserves its run of the block's region with one atomic add (the
 // block's count is zeroed by the step's first fill), so no block-wide count is
 // needed; within a region the runs are in arrival order.
 template <int SEG>
 __device__ __forceinline__ uint32_t lean_col(uint32_t row, uint32_t col) {
-  return col ^ ((row / (128 / SEG)) & (SEG / 16 - 1));
+  return col ^ (((row / (128 / SEG)) ^ (row / (256 / SEG))) & (SEG / 16 - 1));
 }
 
 template <int HK, int CB>
@@ -593,16 +603,23 @@ __global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
   const bool valid = c < a.n;
   const bool fixed = valid && a.baseline != 2 && d == pv;
   const bool ins = valid && (a.baseline || !fixed);  // as in leaf_epilogue
-  if (valid) {
-    a.labels[c] = fixed ? L_FIXED : L_PENDING;
-    if (ins) a.leaf_dg[c] = d;
-  }
+  // Writes beside the read stream are costly (a label byte per chunk cost the
+  // kernel 10% of its rate at 128-byte chunks, scattered digests of changed chunks
+  // as much again): once the labels read L_FIXED, none is written here (k_insert
+  // labels the listed chunks), and each listed digest goes beside its list entry,
+  // packed, for k_insert to move to leaf_dg.
+  if (valid && a.all_labels) a.labels[c] = fixed ? L_FIXED : L_PENDING;
+  if (ins && !a.list_dg) a.leaf_dg[c] = d;
   const uint32_t lane = lane_id();
   const uint64_t m = __ballot(ins);
   uint32_t base = 0;
   if (lane == 0 && m) base = atomicAdd(a.blk_cnt + wh.blk, (uint32_t)__popcll(m));
   base = __shfl(base, 0);
-  if (ins) a.list[t * kThreads + base + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)c;
+  if (ins) {
+    const uint64_t k = t * kThreads + base + __popcll(m & ((1ull << lane) - 1));
+    a.list[k] = (uint32_t)c;
+    if (a.list_dg) a.list_dg[k] = d;
+  }
 }
 
 // Concatenate the per-block regions into the dense list of chunks to insert.
@@ -626,7 +643,8 @@ __device__ __forceinline__ void ph_compact(const Launch<CompactArgs>& L, const W
   }
   __syncthreads();
   const uint32_t total = s_pre[nb];
-  uint32_t* dst = a.dense + a.blk_off[b0];
+  const uint32_t off = a.blk_off[b0];
+  uint32_t* dst = a.dense + off;
   for (uint32_t e = threadIdx.x; e < total; e += kThreads) {
     uint32_t lo = 0, hi = nb;  // last region whose start <= e
     while (hi - lo > 1) {
@@ -634,7 +652,9 @@ __device__ __forceinline__ void ph_compact(const Launch<CompactArgs>& L, const W
       if (s_pre[mid] <= e) lo = mid;
       else hi = mid;
     }
-    dst[e] = a.list[(uint64_t)(b0 + lo) * a.tiles_per_block * kThreads + (e - s_pre[lo])];
+    const uint64_t src = (uint64_t)(b0 + lo) * a.tiles_per_block * kThreads + (e - s_pre[lo]);
+    dst[e] = a.list[src];
+    if (a.dense_dg) a.dense_dg[off + e] = a.list_dg[src];
   }
 }
 
@@ -674,7 +694,7 @@ __device__ __forceinline__ void ph_insert(const Launch<DenseArgs>& L, const Wher
     Digest d{0, 0};
     if (v) {
       c = a.dense[i];
-      d = a.leaf_dg[c];
+      d = a.dense_dg ? a.dense_dg[i] : a.leaf_dg[c];
     }
     const uint64_t plo = __shfl_up(d.lo, 1), phi = __shfl_up(d.hi, 1);
     const int pv = __shfl_up((int)v, 1);
@@ -691,24 +711,34 @@ __device__ __forceinline__ void ph_insert(const Launch<DenseArgs>& L, const Wher
     const uint64_t mine = __ballot(head) & ((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
     const uint32_t hslot = __shfl(slot, mine ? 63 - __clzll((long long)mine) : (int)lane);
     if (follower) slot = hslot;
-    if (v) a.leaf_slot[c] = slot;
+    if (v) {
+      a.leaf_slot[c] = slot;
+      if (a.dense_dg) a.leaf_dg[c] = d;
+      // Every leaf that is not a fixed duplicate is on this list, so sweep B walks
+      // only the stage-0 tiles flagged here.
+      if (a.R0) a.tile_busy[chunk_tile(a.g, a.R0, c)] = 1;
+    }
     // Decided here, so k_resolve skips them: a follower's leaf is above its head's,
     // so it never holds the minimum, and an entry of an earlier checkpoint keeps
     // its value.  A zero digest is left to k_resolve: after a reset it may belong
-    // to a fixed leaf, whose label stays.
+    // to a fixed leaf, whose label stays.  When the leaf kernel labelled nothing,
+    // every listed chunk is a changed one (pending) until labelled here.
+    const bool nz = (d.lo | d.hi) != 0;
     const bool older = head && !claimed && seen != TEMPTY && (uint32_t)(seen >> 32) < a.cur;
-    if ((follower || older) && (d.lo | d.hi) != 0) a.labels[c] = L_SHIFT;
+    int nl = (v && !a.leaf_labels) ? (int)L_PENDING : -1;
+    if ((follower || older) && nz) nl = L_SHIFT;
     // A claimer is the first occurrence unless a marked match below it takes the
     // entry in k_fixmin, which then relabels it.
-    if (claimed && (d.lo | d.hi) != 0) {
-      a.labels[c] = L_FIRST;
+    if (claimed && nz) {
+      nl = L_FIRST;
       if (a.R0) a.tile_first[chunk_tile(a.g, a.R0, c)] = 1;
     }
     // A head that matched an entry of this checkpoint: k_fixmin applies its min.
     if (head && !claimed && !older && slot <= a.mask + 1) {
-      const uint8_t l = a.labels[c];
-      a.labels[c] = l == L_FIXED ? L_MIN_FIXED : L_MIN_PENDING;
+      const uint8_t l = a.leaf_labels ? a.labels[c] : (uint8_t)L_PENDING;
+      nl = l == L_FIXED ? L_MIN_FIXED : L_MIN_PENDING;
     }
+    if (nl >= 0) a.labels[c] = (uint8_t)nl;
     claims += claimed;
   }
   const uint32_t wc = wave_sum(claims);
@@ -1238,16 +1268,35 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
   }
 }
 
-// One wavefront per tile.
+// One wavefront per tile, or at stage 0 tiles_per_wave consecutive tiles: their
+// busy flags are read with one load per lane and only the flagged tiles are
+// walked; every leaf below an unflagged tile is a fixed duplicate, so the tile is
+// fixed and its root counts stay zero.
 template <int HK>
 __device__ __forceinline__ void ph_sweep_b(const Launch<SweepArgs>& L, const Where& wh, uint8_t* lds) {
   (void)lds;
   const SweepArgs a = L.args[wh.m].global();
   const uint32_t lane = lane_id();
-  const uint64_t tile = (uint64_t)wh.blk * (kThreads / 64) + (threadIdx.x >> 6);
-  if (tile >= (1ull << a.d_out)) return;  // whole wavefront
+  const uint64_t ntiles = 1ull << a.d_out;
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
-  sweep_b_tile<HK>(a, tile, load_lab4(a, tile << R, T, lane), lane);
+  const uint32_t tpw = a.stage0 ? a.tiles_per_wave : 1u;
+  const uint64_t t0 = ((uint64_t)wh.blk * (kThreads / 64) + (threadIdx.x >> 6)) * tpw;
+  if (t0 >= ntiles) return;  // whole wavefront
+  if (tpw == 1) {
+    sweep_b_tile<HK>(a, t0, load_lab4(a, t0 << R, T, lane), lane);
+    return;
+  }
+  const uint64_t t1 = min(t0 + tpw, ntiles);
+  const uint64_t mine = t0 + lane;
+  const bool has = mine < t1 && a.tile_busy[mine];
+  if (mine < t1 && !has) a.out_lab[mine] = L_FIXED;
+  uint64_t todo = __ballot(has);
+  while (todo) {
+    const int i = __ffsll((long long)todo) - 1;
+    todo &= todo - 1;
+    const uint64_t t = t0 + i;
+    sweep_b_tile<HK>(a, t, load_lab4(a, t << R, T, lane), lane);
+  }
 }
 
 template <int HK>
@@ -1509,6 +1558,13 @@ __device__ __forceinline__ void ph_assemble(const Launch<DiffArgs>& L, const Whe
   static_assert(kThreads == kStripes, "one block folds the striped counters");
   const DiffArgs a = L.args[wh.m].global();
   diff_meta(a, wh);
+  // Nothing reads the leaf labels after sweep B: the listed chunks go back to
+  // L_FIXED, so the next step's leaf kernel labels only the chunks it lists.
+  if (a.reset_labels) {
+    const uint64_t listed = *a.changed;
+    for (uint64_t i = (uint64_t)wh.blk * kThreads + threadIdx.x; i < listed; i += (uint64_t)wh.nblk * kThreads)
+      a.reset_labels[a.dense[i]] = L_FIXED;
+  }
   if (wh.blk == 0) {
     __syncthreads();  // the header thread's control fields
     finish_member(a.ctr, a.changed, a.ctl, a.out, s_fold);
@@ -1843,6 +1899,10 @@ static void ensure_buffers(mtc_dedup* h, hipStream_t st) {
   }
   h->list.ensure(n);
   h->dense.ensure(n);
+  if (h->approach != MTC_APPROACH_BASIC) {  // listed digests beside the list, then dense
+    h->list_dg.ensure(n);
+    h->dense_dg.ensure(n);
+  }
   const uint64_t max_blk = (n + kThreads - 1) / kThreads;  // at most one block per tile
   h->blk_cnt.ensure(max_blk);
   h->blk_off.ensure(max_blk + 1);
@@ -1938,7 +1998,7 @@ uint64_t table_slots(uint64_t n, double load_factor) {
 // the table at the given load factor and the diff arena).
 uint64_t state_bytes(uint64_t n, uint32_t chunk_size, double load_factor) {
   const uint64_t table = 32 * (table_slots(n, load_factor) + 1);
-  const uint64_t per_chunk = 16 + 1 + 4 + 4 + 4 + 8 * 2 + 4 + 8 + 2 * (4 + 8) + 4 + 12 + 1;
+  const uint64_t per_chunk = 16 + 1 + 4 + 4 + 4 + 8 * 2 + 4 + 8 + 2 * (4 + 8) + 4 + 12 + 1 + 2 * 16;
   return table + per_chunk * n + (uint64_t)chunk_size * n + (8u << 20);
 }
 
@@ -2056,6 +2116,7 @@ struct Member {
   uint64_t tpb = 1, ntile0 = 1;
   int cb = 0;
   bool split = false;  // the table bound is checked after the leaf kernel
+  bool all_labels = true;  // the leaf kernel labels every chunk, not only the listed ones
   Geo g{};
   uint32_t* ws_blk = nullptr;  // scan partial sums of this member
   uint32_t* ws_f = nullptr;
@@ -2111,6 +2172,10 @@ static void prepare_member(Member& M, hipStream_t st) {
     plan_tree(h);
   }
   ensure_buffers(h, st);
+  // Until this step completes, the labels are not known to be reset.  A baseline
+  // lists every chunk, fixed ones included, so its leaf kernel labels them all.
+  M.all_labels = M.baseline || !h->labels_clean || h->debug || h->approach == MTC_APPROACH_BASIC;
+  h->labels_clean = false;
   M.split = h->approach == MTC_APPROACH_BASIC ? false : prepare_table(h, M.baseline, st);
   if (M.baseline) h->baseline_id = h->current_id;
   M.n = h->n;
@@ -2184,11 +2249,12 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
     std::vector<uint32_t> db;
     for (auto& M : ms) {
       mtc_dedup* h = M.h;
-      ca.push_back(CompactArgs{h->list.p, h->blk_cnt.p, h->blk_off.p, M.tpb, h->dense.p, M.nblk_leaf});
+      ca.push_back(CompactArgs{h->list.p, h->blk_cnt.p, h->blk_off.p, M.tpb, h->dense.p, M.nblk_leaf, h->list_dg.p,
+                               h->dense_dg.p});
       cb.push_back((M.nblk_leaf + kCompactPer - 1) / kCompactPer);
       da.push_back(DenseArgs{h->dense.p, h->blk_off.p + M.nblk_leaf, M.n, h->leaf_dg.p, h->leaf_slot.p, h->labels.p,
                              h->table.p, h->table_cap - 1, M.cur, h->ctl_d, h->ctr.p, M.g, M.R0, h->tile_flags.p,
-                             h->tile_flags.p + M.ntile0});
+                             h->tile_flags.p + M.ntile0, h->dense_dg.p, M.all_labels ? 1 : 0});
       db.push_back(grid_for(M.n, 8 * kThreads, 16384));
     }
     ops.add<CompactArgs>(20, img.add(ca, cb), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
@@ -2259,9 +2325,10 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
             a.out_lab = h->fr_lab.p + out_off;
             a.out_dg = h->fr_dg.p + out_off;
             a.out_slot = h->fr_slot.p + out_off;
-            // stage 0 of sweep A: several tiles per wavefront (their flags are read at
-            // once); four wavefronts per block
-            a.tiles_per_wave = (a.stage0 && pass == 0) ? kTilesPerWaveA : 1u;
+            // stage 0: several tiles per wavefront (their flags are read at once; sweep
+            // B walks every tile when it records the debug labels); four wavefronts
+            // per block
+            a.tiles_per_wave = !a.stage0 ? 1u : (pass == 0 ? kTilesPerWaveA : (a.dbg_labels ? 1u : kTilesPerWaveB));
             const uint64_t waves = ((1ull << a.d_out) + a.tiles_per_wave - 1) / a.tiles_per_wave;
             blocks = (uint32_t)((waves + 3) / 4);
           }
@@ -2381,6 +2448,8 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       d.ctr = h->ctr.p;
       d.changed = h->blk_off.p + M.nblk_leaf;
       d.out = M.ctl_out;
+      d.dense = h->dense.p;
+      d.reset_labels = h->debug ? nullptr : h->labels.p;  // the debug view copies them after the step
       da.push_back(d);
       gb.push_back(grid_for(M.n, 8 * kThreads, 8192));
     }
@@ -2398,7 +2467,7 @@ static void build_basic_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) 
     std::vector<uint32_t> cb;
     for (auto& M : ms) {
       mtc_dedup* h = M.h;
-      ca.push_back(CompactArgs{h->list.p, h->blk_cnt.p, h->blk_off.p, M.tpb, h->dense.p, M.nblk_leaf});
+      ca.push_back(CompactArgs{h->list.p, h->blk_cnt.p, h->blk_off.p, M.tpb, h->dense.p, M.nblk_leaf, nullptr, nullptr});
       cb.push_back((M.nblk_leaf + kCompactPer - 1) / kCompactPer);
     }
     ops.add<CompactArgs>(21, img.add(ca, cb), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
@@ -2472,7 +2541,8 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
         // basic approach: every chunk of the first checkpoint is changed (src/basic_approach.cpp:36-45)
         const int reset = !M.baseline ? 0 : (h->approach == MTC_APPROACH_BASIC && M.cur == 0) ? 2 : 1;
         la.push_back(LeafArgs{M.data, M.len, M.n, h->chunk_size, h->leaf_dg.p, h->labels.p, reset,
-                              h->list.p, h->blk_cnt.p, M.tpb});
+                              h->list.p, h->blk_cnt.p, M.tpb, M.all_labels ? 1 : 0,
+                              h->approach == MTC_APPROACH_BASIC ? nullptr : h->list_dg.p});
         lb.push_back(M.nblk_leaf);
       }
       const int hk = std::get<0>(kv.first), cb = std::get<1>(kv.first);
@@ -2676,6 +2746,8 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
     mtc_dedup* h = ms[i].h;
     const Ctl& c = ctx.ctl_h[i];
     h->has_state = true;
+    // k_assemble reset the listed chunks' labels unless the debug view needed them
+    h->labels_clean = h->approach != MTC_APPROACH_BASIC && !h->debug;
     h->copy_pending[ms[i].arena] = false;  // the step waited for that copy and has completed
     *h->ctl_h = c;
     h->table_size += c.claims;

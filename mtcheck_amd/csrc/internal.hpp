@@ -153,6 +153,9 @@ struct mtc_dedup {
   uint32_t current_id = 0;
   uint32_t baseline_id = 0;
   bool has_state = false;
+  // Every leaf label is L_FIXED (the last step reset the chunks it listed), so the
+  // next leaf kernel labels only the chunks it lists.
+  bool labels_clean = false;
   uint64_t data_len = 0;
   uint32_t n = 0;
   // power of two; slot index cap is the overflow slot.  Atomic: a serving lane may
@@ -167,6 +170,8 @@ struct mtc_dedup {
   mtc::DevBuf<uint32_t> leaf_slot;    // table slot of each non-fixed leaf
   mtc::DevBuf<uint32_t> list;         // chunks to insert, packed per leaf block
   mtc::DevBuf<uint32_t> dense;        // the same, concatenated
+  mtc::DevBuf<mtc::Digest> list_dg;   // tree handles: their digests beside the list entries
+  mtc::DevBuf<mtc::Digest> dense_dg;  // and concatenated
   mtc::DevBuf<uint32_t> blk_cnt, blk_off;
   mtc::DevBuf<uint8_t> tile_flags;    // per stage-0 tile: has-first, then busy
   mtc::DevBuf<mtc::Slot> table;

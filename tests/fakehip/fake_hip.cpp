@@ -56,6 +56,8 @@ struct State {
   std::map<const void*, std::pair<void*, uint64_t>> vars;  // host shadow -> (fake device address, size)
   std::vector<std::string> violations;
   std::vector<std::string> launches;  // kernel names in launch order
+  std::vector<int> leaf_all_labels;   // per leaf member launched: its all_labels flag
+  std::vector<int> label_resets;      // per k_assemble member launched: reset_labels is set
   hipStream_t capturing = nullptr;
   std::vector<Node> captured;
   uint64_t bytes_alive = 0;
@@ -179,7 +181,9 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       need(w, "labels", a.labels, a.n);
       need(w, "list", a.list, 4ull * a.n);
       need(w, "blk_cnt", a.blk_cnt, 4ull * nblk);
+      maybe(w, "list_dg", a.list_dg, 16ull * a.n);
       if ((uint64_t)nblk * a.tiles_per_block * 256 < a.n) violation(w + ": leaf blocks do not cover the chunks");
+      S().leaf_all_labels.push_back(a.all_labels);
     });
   } else if (has("11CompactArgs")) {
     check_members<CompactArgs>(name, L, grid, [](const std::string& w, const CompactArgs& a, uint32_t) {
@@ -187,6 +191,9 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       need(w, "blk_cnt", a.blk_cnt, 4ull * a.nblk);
       need(w, "blk_off", a.blk_off, 4ull * (a.nblk + 1));
       need(w, "dense", a.dense, 4);
+      if (!a.list_dg != !a.dense_dg) violation(w + ": list_dg and dense_dg come together");
+      maybe(w, "list_dg", a.list_dg, 16);
+      maybe(w, "dense_dg", a.dense_dg, 16);
     });
   } else if (has("9DenseArgs")) {
     check_members<DenseArgs>(name, L, grid, [](const std::string& w, const DenseArgs& a, uint32_t) {
@@ -195,6 +202,7 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       need(w, "leaf_dg", a.leaf_dg, 16ull * a.n);
       need(w, "leaf_slot", a.leaf_slot, 4ull * a.n);
       need(w, "labels", a.labels, a.n);
+      need(w, "dense_dg", a.dense_dg, 16ull * a.n);  // tree members' digests come only this way
       need(w, "tab", a.tab, 32ull * (a.mask + 2));
       need(w, "ctl", a.ctl, kCtl);
       need(w, "ctr", a.ctr, 8);
@@ -301,6 +309,9 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       need(w, "ctr", a.ctr, 8);
       need(w, "changed", a.changed, 4);
       need(w, "out", a.out, kCtl);
+      need(w, "dense", a.dense, 4ull * a.n);
+      maybe(w, "reset_labels", a.reset_labels, a.n);
+      S().label_resets.push_back(a.reset_labels ? 1 : 0);
     });
   } else {
     violation(name + ": batched launch of an argument type this checker does not know");
@@ -342,6 +353,14 @@ int fakehip_launch_count() {
   std::lock_guard<std::mutex> g(S().mu);
   return (int)S().launches.size();
 }
+int fakehip_leaf_all_labels(int i) {  // -1 past the end
+  std::lock_guard<std::mutex> g(S().mu);
+  return (i >= 0 && i < (int)S().leaf_all_labels.size()) ? S().leaf_all_labels[i] : -1;
+}
+int fakehip_label_resets(int i) {
+  std::lock_guard<std::mutex> g(S().mu);
+  return (i >= 0 && i < (int)S().label_resets.size()) ? S().label_resets[i] : -1;
+}
 const char* fakehip_launch_name(int i) {
   std::lock_guard<std::mutex> g(S().mu);
   return (i >= 0 && i < (int)S().launches.size()) ? S().launches[i].c_str() : "";
@@ -350,6 +369,8 @@ void fakehip_reset() {
   std::lock_guard<std::mutex> g(S().mu);
   S().violations.clear();
   S().launches.clear();
+  S().leaf_all_labels.clear();
+  S().label_resets.clear();
 }
 uint64_t fakehip_bytes_alive() {
   std::lock_guard<std::mutex> g(S().mu);

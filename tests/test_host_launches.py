@@ -129,6 +129,54 @@ def test_tree_steps_launch_in_bounds(fake, C, size, hk, tuning):
     L.mtc_destroy(h)
 
 
+def test_leaf_labels_written_for_listed_chunks_once_reset(fake, monkeypatch):
+    """The leaf kernel labels every chunk until a completed step has reset the
+    labels of the chunks it listed (k_assemble does, unless the debug view needs
+    them); after that it labels only the chunks it lists.  A debug step, a failed
+    step and a basic handle keep every chunk labelled."""
+    L = fake
+    L.fakehip_reset()
+    h = create(L, 128)
+    size = 128 * 3000
+    data = dev_alloc(L, size)
+
+    def step(baseline=False, expect=OK):
+        n0 = 0
+        while L.fakehip_leaf_all_labels(n0) >= 0:
+            n0 += 1
+        r0 = 0
+        while L.fakehip_label_resets(r0) >= 0:
+            r0 += 1
+        rc, _ = checkpoint(L, h, data, size, baseline=baseline)
+        assert rc == expect, L.mtc_last_error(h)
+        return L.fakehip_leaf_all_labels(n0), L.fakehip_label_resets(r0)
+
+    assert step(baseline=True) == (1, 1)   # fresh labels: all written, listed ones reset
+    assert step() == (0, 1)
+    assert step() == (0, 1)
+    L.mtc_set_debug(h, 1)
+    assert step() == (1, 0)                # the debug view copies the labels after the step
+    L.mtc_set_debug(h, 0)
+    assert step() == (1, 1)                # the debug step left them unreset
+    assert step() == (0, 1)
+    monkeypatch.setenv("MTC_TEST_FAIL_MEMBER", "0")
+    assert step(expect=STATE) == (0, 1)
+    monkeypatch.delenv("MTC_TEST_FAIL_MEMBER")
+    assert step() == (1, 1)                # a failed step: a baseline, every label written
+    assert violations(L) == []
+    L.mtc_destroy(h)
+    b = create(L, 128, BASIC)
+    rc, _ = checkpoint(L, b, data, size, baseline=True)
+    assert rc == OK
+    rc, _ = checkpoint(L, b, data, size)
+    assert rc == OK
+    n = 0
+    while L.fakehip_leaf_all_labels(n) >= 0:
+        n += 1
+    assert L.fakehip_leaf_all_labels(n - 1) == 1 and L.fakehip_leaf_all_labels(n - 2) == 1
+    L.mtc_destroy(b)
+
+
 def test_batch_of_tree_and_basic_members_with_graph_replay(fake):
     L = fake
     L.fakehip_reset()
