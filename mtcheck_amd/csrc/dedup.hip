@@ -44,6 +44,20 @@ constexpr uint32_t kTilesPerWaveA = MTC_TPW_A;  // stage-0 tiles per wavefront, 
 #define MTC_TPW_B 8
 #endif
 constexpr uint32_t kTilesPerWaveB = MTC_TPW_B;  // stage-0 tiles per wavefront, sweep B (<= 64)
+// Wavefronts per block of the sweeps.  Wavefronts of one block draw very
+// different numbers of busy tiles, and a block holds its slot (and sweep A's LDS
+// records) until its slowest wavefront ends: with four wavefronts per block only
+// about 2500 of the chip's 8192 wavefront slots were busy during sweep A's first
+// stage.  One wavefront per block: sweep A 178 -> 153 us, sweep B 210 -> 184 us
+// (8 GiB, 128-byte chunks, 2% change).
+#ifndef MTC_SWEEP_WPB_A
+#define MTC_SWEEP_WPB_A 1
+#endif
+#ifndef MTC_SWEEP_WPB_B
+#define MTC_SWEEP_WPB_B 1
+#endif
+constexpr uint32_t kSweepWpbA = MTC_SWEEP_WPB_A;
+constexpr uint32_t kSweepWpbB = MTC_SWEEP_WPB_B;
 
 // ---------------------------------------------------------------------------
 // Batched launches
@@ -999,15 +1013,15 @@ __device__ __forceinline__ void sweep_a_tile(const SweepArgs& a, uint64_t tile, 
 
 // Stage 0: a wavefront owns tiles_per_wave consecutive tiles, reads their
 // first-occurrence flags with one load per lane and walks only the flagged ones.
-template <int HK>
+template <int HK, int WPB>
 __device__ __forceinline__ void ph_sweep_a(const Launch<SweepArgs>& L, const Where& wh, uint8_t* lds) {
-  auto& rec_dg = *reinterpret_cast<Digest(*)[kThreads / 64][256]>(lds);
-  auto& rec_node = *reinterpret_cast<uint32_t(*)[kThreads / 64][256]>(lds + sizeof(Digest) * (kThreads / 64) * 256);
+  auto& rec_dg = *reinterpret_cast<Digest(*)[WPB][256]>(lds);
+  auto& rec_node = *reinterpret_cast<uint32_t(*)[WPB][256]>(lds + sizeof(Digest) * WPB * 256);
   const SweepArgs a = L.args[wh.m].global();
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
   const uint64_t ntiles = 1ull << a.d_out;
   const uint32_t tpw = a.stage0 ? a.tiles_per_wave : 1u;
-  const uint64_t t0 = ((uint64_t)wh.blk * (kThreads / 64) + w) * tpw;
+  const uint64_t t0 = ((uint64_t)wh.blk * WPB + w) * tpw;
   if (t0 >= ntiles) return;  // whole wavefront
   const uint64_t t1 = min(t0 + tpw, ntiles);
   if (!a.stage0) {
@@ -1025,10 +1039,10 @@ __device__ __forceinline__ void ph_sweep_a(const Launch<SweepArgs>& L, const Whe
   }
 }
 
-template <int HK>
-__global__ __launch_bounds__(kThreads) void k_sweep_a(Launch<SweepArgs> L) {
-  __shared__ __align__(16) uint8_t lds[(sizeof(Digest) + sizeof(uint32_t)) * (kThreads / 64) * 256];
-  ph_sweep_a<HK>(L, where(L), lds);
+template <int HK, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_sweep_a(Launch<SweepArgs> L) {
+  __shared__ __align__(16) uint8_t lds[(sizeof(Digest) + sizeof(uint32_t)) * WPB * 256];
+  ph_sweep_a<HK, WPB>(L, where(L), lds);
 }
 
 // Sweep B: shifted-duplicate forest and region roots (include/reference_impl.hpp
@@ -1272,7 +1286,7 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
 // busy flags are read with one load per lane and only the flagged tiles are
 // walked; every leaf below an unflagged tile is a fixed duplicate, so the tile is
 // fixed and its root counts stay zero.
-template <int HK>
+template <int HK, int WPB>
 __device__ __forceinline__ void ph_sweep_b(const Launch<SweepArgs>& L, const Where& wh, uint8_t* lds) {
   (void)lds;
   const SweepArgs a = L.args[wh.m].global();
@@ -1280,7 +1294,7 @@ __device__ __forceinline__ void ph_sweep_b(const Launch<SweepArgs>& L, const Whe
   const uint64_t ntiles = 1ull << a.d_out;
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
   const uint32_t tpw = a.stage0 ? a.tiles_per_wave : 1u;
-  const uint64_t t0 = ((uint64_t)wh.blk * (kThreads / 64) + (threadIdx.x >> 6)) * tpw;
+  const uint64_t t0 = ((uint64_t)wh.blk * WPB + (threadIdx.x >> 6)) * tpw;
   if (t0 >= ntiles) return;  // whole wavefront
   if (tpw == 1) {
     sweep_b_tile<HK>(a, t0, load_lab4(a, t0 << R, T, lane), lane);
@@ -1291,17 +1305,25 @@ __device__ __forceinline__ void ph_sweep_b(const Launch<SweepArgs>& L, const Whe
   const bool has = mine < t1 && a.tile_busy[mine];
   if (mine < t1 && !has) a.out_lab[mine] = L_FIXED;
   uint64_t todo = __ballot(has);
+  if (!todo) return;
+  // The next flagged tile's labels are loaded before this one is walked.
+  int i = __ffsll((long long)todo) - 1;
+  Lab4 lb = load_lab4(a, (t0 + i) << R, T, lane);
   while (todo) {
-    const int i = __ffsll((long long)todo) - 1;
-    todo &= todo - 1;
     const uint64_t t = t0 + i;
-    sweep_b_tile<HK>(a, t, load_lab4(a, t << R, T, lane), lane);
+    const Lab4 cur = lb;
+    todo &= todo - 1;
+    if (todo) {
+      i = __ffsll((long long)todo) - 1;
+      lb = load_lab4(a, (t0 + i) << R, T, lane);
+    }
+    sweep_b_tile<HK>(a, t, cur, lane);
   }
 }
 
-template <int HK>
-__global__ __launch_bounds__(kThreads) void k_sweep_b(Launch<SweepArgs> L) {
-  ph_sweep_b<HK>(L, where(L), nullptr);
+template <int HK, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_sweep_b(Launch<SweepArgs> L) {
+  ph_sweep_b<HK, WPB>(L, where(L), nullptr);
 }
 
 // A single chunk: the leaf is the root.
@@ -2330,7 +2352,8 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
             // per block
             a.tiles_per_wave = !a.stage0 ? 1u : (pass == 0 ? kTilesPerWaveA : (a.dbg_labels ? 1u : kTilesPerWaveB));
             const uint64_t waves = ((1ull << a.d_out) + a.tiles_per_wave - 1) / a.tiles_per_wave;
-            blocks = (uint32_t)((waves + 3) / 4);
+            const uint32_t wpb = pass == 0 ? kSweepWpbA : kSweepWpbB;
+            blocks = (uint32_t)((waves + wpb - 1) / wpb);
           }
           slot.args.push_back(a);
           slot.blocks.push_back(blocks);
@@ -2341,13 +2364,13 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
           const LaunchRec r = img.add(kv.second.args, kv.second.blocks);
           if (pass == 0)
             ops.add<SweepArgs>(tag, r, [hk](Launch<SweepArgs> L, uint32_t g, hipStream_t st) {
-              if (hk == HASH_MD5) k_sweep_a<HASH_MD5><<<g, kThreads, 0, st>>>(L);
-              else k_sweep_a<HASH_MURMUR3><<<g, kThreads, 0, st>>>(L);
+              if (hk == HASH_MD5) k_sweep_a<HASH_MD5, kSweepWpbA><<<g, 64 * kSweepWpbA, 0, st>>>(L);
+              else k_sweep_a<HASH_MURMUR3, kSweepWpbA><<<g, 64 * kSweepWpbA, 0, st>>>(L);
             });
           else
             ops.add<SweepArgs>(tag, r, [hk](Launch<SweepArgs> L, uint32_t g, hipStream_t st) {
-              if (hk == HASH_MD5) k_sweep_b<HASH_MD5><<<g, kThreads, 0, st>>>(L);
-              else k_sweep_b<HASH_MURMUR3><<<g, kThreads, 0, st>>>(L);
+              if (hk == HASH_MD5) k_sweep_b<HASH_MD5, kSweepWpbB><<<g, 64 * kSweepWpbB, 0, st>>>(L);
+              else k_sweep_b<HASH_MURMUR3, kSweepWpbB><<<g, 64 * kSweepWpbB, 0, st>>>(L);
             });
         }
       }
