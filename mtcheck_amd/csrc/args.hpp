@@ -47,9 +47,10 @@ struct ScanSeg {
   uint32_t count_max;
   uint32_t* total;        // optional: sum of all elements
   uint32_t* sums;         // ceil(count_max / kScanTile) words
+  unsigned long long* stamp;  // optional: the wall clock when the segment's first block starts
   __device__ ScanSeg global() const {
     ScanSeg c = *this;
-    MTC_GLOBAL(in); MTC_GLOBAL(out); MTC_GLOBAL(count_dev); MTC_GLOBAL(total); MTC_GLOBAL(sums);
+    MTC_GLOBAL(in); MTC_GLOBAL(out); MTC_GLOBAL(count_dev); MTC_GLOBAL(total); MTC_GLOBAL(sums); MTC_GLOBAL(stamp);
     return c;
   }
 };
@@ -72,9 +73,11 @@ struct LeafArgs {
   // Tree members: each listed chunk's digest is stored beside its list entry and
   // k_insert moves it to leaf_dg.  Null: leaf_dg is written here.
   Digest* list_dg;
+  unsigned long long* stamp;  // the wall clock when the member's first block starts
   __device__ LeafArgs global() const {
     LeafArgs c = *this;
     MTC_GLOBAL(data); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(labels); MTC_GLOBAL(list); MTC_GLOBAL(blk_cnt); MTC_GLOBAL(list_dg);
+    MTC_GLOBAL(stamp);
     return c;
   }
 };

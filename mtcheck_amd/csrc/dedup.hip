@@ -161,18 +161,24 @@ __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t
 }
 
 // ---------------------------------------------------------------------------
-// Batched exclusive scan of u32 arrays: per-tile sums, a per-array scan of the
-// sums, then the tiles with their offsets.  The element count may live on the
-// device (count_dev), bounded by count_max.
+// Batched exclusive scan of u32 arrays in two launches: per-tile sums, then the
+// tiles with their offsets (each block adds up the sums of the tiles before its
+// own).  The element count may live on the device (count_dev), bounded by
+// count_max.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ uint32_t scan_count(const ScanSeg& s) {
   return s.count_dev ? min(*s.count_dev, s.count_max) : s.count_max;
 }
 
+__device__ __forceinline__ void scan_stamp(const ScanSeg& s, const Where& w) {
+  if (s.stamp && w.blk == 0 && threadIdx.x == 0) *s.stamp = wall_clock64();
+}
+
 __device__ __forceinline__ void ph_scan_reduce(const Launch<ScanSeg>& L, const Where& w, uint8_t* lds) {
   uint32_t* ws = reinterpret_cast<uint32_t*>(lds);
   const ScanSeg s = L.args[w.m].global();
+  scan_stamp(s, w);
   const uint32_t cnt = scan_count(s), ntiles = (cnt + kScanTile - 1) / kScanTile;
   for (uint32_t tile = w.blk; tile < ntiles; tile += w.nblk) {
     const uint64_t base = (uint64_t)tile * kScanTile;
@@ -193,32 +199,21 @@ __global__ __launch_bounds__(kThreads) void k_scan_reduce(Launch<ScanSeg> L) {
   ph_scan_reduce(L, where(L), lds);
 }
 
-__device__ __forceinline__ void ph_scan_top(const Launch<ScanSeg>& L, const Where& w, uint8_t* lds) {
+// The second launch of the scan: each block sums the tile sums before its tiles
+// itself (reading each sum once, as its tiles advance), so the top-level scan
+// needs no launch of its own; the block of the last tile writes the total.
+__device__ __forceinline__ void ph_scan_down2(const Launch<ScanSeg>& L, const Where& w, uint8_t* lds) {
   uint32_t* ws = reinterpret_cast<uint32_t*>(lds);
   const ScanSeg s = L.args[w.m].global();
   const uint32_t cnt = scan_count(s), ntiles = (cnt + kScanTile - 1) / kScanTile;
-  uint32_t carry = 0;
-  for (uint32_t b0 = 0; b0 < ntiles; b0 += kThreads) {
-    const uint32_t i = b0 + threadIdx.x;
-    const uint32_t v = i < ntiles ? s.sums[i] : 0u;
-    uint32_t tot;
-    const uint32_t ex = block_scan_excl(v, &tot, ws);
-    if (i < ntiles) s.sums[i] = carry + ex;
-    carry += tot;
-  }
-  if (threadIdx.x == 0 && s.total) *s.total = carry;
-}
-
-__global__ __launch_bounds__(kThreads) void k_scan_top(Launch<ScanSeg> L) {
-  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * (kThreads / 64)];
-  ph_scan_top(L, where(L), lds);
-}
-
-__device__ __forceinline__ void ph_scan_down(const Launch<ScanSeg>& L, const Where& w, uint8_t* lds) {
-  uint32_t* ws = reinterpret_cast<uint32_t*>(lds);
-  const ScanSeg s = L.args[w.m].global();
-  const uint32_t cnt = scan_count(s), ntiles = (cnt + kScanTile - 1) / kScanTile;
+  if (ntiles == 0 && w.blk == 0 && threadIdx.x == 0 && s.total) *s.total = 0;
+  uint32_t pre = 0, done = 0;  // pre = sums[0] + ... + sums[done - 1]
   for (uint32_t tile = w.blk; tile < ntiles; tile += w.nblk) {
+    uint32_t acc = 0;
+    for (uint32_t j = done + threadIdx.x; j < tile; j += kThreads) acc += s.sums[j];
+    uint32_t more;
+    block_scan_excl(acc, &more, ws);
+    pre += more;
     const uint64_t base = (uint64_t)tile * kScanTile + 8ull * threadIdx.x;
     uint32_t v[8], run = 0;
 #pragma unroll
@@ -229,23 +224,27 @@ __device__ __forceinline__ void ph_scan_down(const Launch<ScanSeg>& L, const Whe
       run += x;
     }
     uint32_t tot;
-    const uint32_t off = s.sums[tile] + block_scan_excl(run, &tot, ws);
+    const uint32_t off = pre + block_scan_excl(run, &tot, ws);
 #pragma unroll
     for (uint32_t j = 0; j < 8; j++)
       if (base + j < cnt) s.out[base + j] = off + v[j];
+    if (tile + 1 == ntiles && threadIdx.x == 0 && s.total) *s.total = pre + tot;
+    pre += tot;  // this tile's sum
+    done = tile + 1;
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_scan_down(Launch<ScanSeg> L) {
+__global__ __launch_bounds__(kThreads) void k_scan_down2(Launch<ScanSeg> L) {
   __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * (kThreads / 64)];
-  ph_scan_down(L, where(L), lds);
+  ph_scan_down2(L, where(L), lds);
 }
 
 // A scan whose every segment fits one tile (count_max <= kScanTile): one block per
-// segment scans it and writes its total, in one launch instead of three.
+// segment scans it and writes its total, in one launch instead of two.
 __device__ __forceinline__ void ph_scan_one(const Launch<ScanSeg>& L, const Where& w, uint8_t* lds) {
   uint32_t* ws = reinterpret_cast<uint32_t*>(lds);
   const ScanSeg s = L.args[w.m].global();
+  scan_stamp(s, w);
   const uint32_t cnt = scan_count(s);
   const uint32_t base = 8u * threadIdx.x;
   uint32_t v[8], run = 0;
@@ -431,6 +430,7 @@ __global__ __launch_bounds__(kThreads) void k_leaf(Launch<LeafArgs> L) {
   __shared__ uint32_t s_wcnt[kThreads / 64];
   const Where wh = where(L);
   const LeafArgs a = L.args[wh.m].global();
+  if (a.stamp && wh.blk == 0 && threadIdx.x == 0) *a.stamp = wall_clock64();
   const uint64_t ntiles = (a.n + kThreads - 1) / kThreads;
   const uint64_t t0 = (uint64_t)wh.blk * a.tiles_per_block;
   const uint64_t t1 = std::min<uint64_t>(t0 + a.tiles_per_block, ntiles);
@@ -556,6 +556,7 @@ __global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
   __shared__ __align__(16) uint8_t lds[kThreads * SEG];
   const Where wh = where(L);
   const LeafArgs a = L.args[wh.m].global();
+  if (a.stamp && wh.blk == 0 && threadIdx.x == 0) *a.stamp = wall_clock64();
   const uint64_t t = wh.blk;
   const uint64_t full = a.len / CB;
   const uint32_t tid = threadIdx.x;
@@ -1282,20 +1283,9 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
   }
 }
 
-// One wavefront per tile, or at stage 0 tiles_per_wave consecutive tiles: their
-// busy flags are read with one load per lane and only the flagged tiles are
-// walked; every leaf below an unflagged tile is a fixed duplicate, so the tile is
-// fixed and its root counts stay zero.
-template <int HK, int WPB>
-__device__ __forceinline__ void ph_sweep_b(const Launch<SweepArgs>& L, const Where& wh, uint8_t* lds) {
-  (void)lds;
-  const SweepArgs a = L.args[wh.m].global();
-  const uint32_t lane = lane_id();
-  const uint64_t ntiles = 1ull << a.d_out;
-  const uint32_t R = a.d_in - a.d_out, T = 1u << R;
-  const uint32_t tpw = a.stage0 ? a.tiles_per_wave : 1u;
-  const uint64_t t0 = ((uint64_t)wh.blk * WPB + (threadIdx.x >> 6)) * tpw;
-  if (t0 >= ntiles) return;  // whole wavefront
+template <int HK>
+__device__ __forceinline__ void ph_sweep_b_tiles(const SweepArgs& a, uint64_t t0, uint32_t tpw, uint64_t ntiles,
+                                                 uint32_t R, uint32_t T, uint32_t lane) {
   if (tpw == 1) {
     sweep_b_tile<HK>(a, t0, load_lab4(a, t0 << R, T, lane), lane);
     return;
@@ -1321,6 +1311,24 @@ __device__ __forceinline__ void ph_sweep_b(const Launch<SweepArgs>& L, const Whe
   }
 }
 
+// One wavefront per tile, or at stage 0 tiles_per_wave consecutive tiles: their
+// busy flags are read with one load per lane and only the flagged tiles are
+// walked; every leaf below an unflagged tile is a fixed duplicate, so the tile is
+// fixed and its root counts stay zero.
+template <int HK, int WPB>
+__device__ __forceinline__ void ph_sweep_b(const Launch<SweepArgs>& L, const Where& wh, uint8_t* lds) {
+  (void)lds;
+  const SweepArgs a = L.args[wh.m].global();
+  const uint32_t lane = lane_id();
+  const uint64_t ntiles = 1ull << a.d_out;
+  const uint32_t R = a.d_in - a.d_out, T = 1u << R;
+  const uint32_t tpw = a.stage0 ? a.tiles_per_wave : 1u;
+  const uint64_t t0 = ((uint64_t)wh.blk * WPB + (threadIdx.x >> 6)) * tpw;
+  if (t0 >= ntiles) return;  // whole wavefront
+  ph_sweep_b_tiles<HK>(a, t0, tpw, ntiles, R, T, lane);
+  if (a.d_out == 0 && lane == 0) a.ctl->t_forest = wall_clock64();  // the member's last sweep stage
+}
+
 template <int HK, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_sweep_b(Launch<SweepArgs> L) {
   ph_sweep_b<HK, WPB>(L, where(L), nullptr);
@@ -1335,6 +1343,7 @@ __device__ __forceinline__ void ph_single_leaf(const Launch<SweepArgs>& L, const
   if (root) a.staging[0] = (l == L_FIRST ? (kFirstTag << 32) : ((uint64_t)a.leaf_slot[0] << 32)) | 0u;
   a.cnt_f[0] = (root && l == L_FIRST) ? 1u : 0u;
   a.cnt_s[0] = (root && l == L_SHIFT) ? 1u : 0u;
+  a.ctl->t_forest = wall_clock64();
 }
 
 __global__ void k_single_leaf(Launch<SweepArgs> L) { ph_single_leaf(L, where(L)); }
@@ -2244,18 +2253,18 @@ static void add_scan(Ops& ops, ArenaImage& img, const std::vector<ScanSeg>& segs
     return;
   }
   for (const auto& s : segs) tiles.push_back(std::max<uint32_t>(1, std::min<uint32_t>(512, (s.count_max + kScanTile - 1) / kScanTile)));
-  const LaunchRec rt = img.add(segs, tiles), r1 = img.add(segs, ones);
+  const LaunchRec rt = img.add(segs, tiles);
   ops.add<ScanSeg>(10, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_reduce<<<g, kThreads, 0, st>>>(L); });
-  ops.add<ScanSeg>(11, r1, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_top<<<g, kThreads, 0, st>>>(L); });
-  ops.add<ScanSeg>(12, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_down<<<g, kThreads, 0, st>>>(L); });
+  ops.add<ScanSeg>(14, rt, [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_down2<<<g, kThreads, 0, st>>>(L); });
 }
 
 // The scan of the leaf blocks' counts (the offsets of each block's list of
 // chunks to insert, and their total).
 static void add_leaf_scan(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
   std::vector<ScanSeg> ss;
-  for (auto& M : ms)
-    ss.push_back(ScanSeg{M.h->blk_cnt.p, M.h->blk_off.p, nullptr, M.nblk_leaf, M.h->blk_off.p + M.nblk_leaf, M.ws_blk});
+  for (auto& M : ms)  // its first block's start closes the leaf phase
+    ss.push_back(ScanSeg{M.h->blk_cnt.p, M.h->blk_off.p, nullptr, M.nblk_leaf, M.h->blk_off.p + M.nblk_leaf, M.ws_blk,
+                         &M.h->ctl_d->t_leaf});
   add_scan(ops, img, ss);
 }
 
@@ -2565,7 +2574,7 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
         const int reset = !M.baseline ? 0 : (h->approach == MTC_APPROACH_BASIC && M.cur == 0) ? 2 : 1;
         la.push_back(LeafArgs{M.data, M.len, M.n, h->chunk_size, h->leaf_dg.p, h->labels.p, reset,
                               h->list.p, h->blk_cnt.p, M.tpb, M.all_labels ? 1 : 0,
-                              h->approach == MTC_APPROACH_BASIC ? nullptr : h->list_dg.p});
+                              h->approach == MTC_APPROACH_BASIC ? nullptr : h->list_dg.p, &h->ctl_d->t_begin});
         lb.push_back(M.nblk_leaf);
       }
       const int hk = std::get<0>(kv.first), cb = std::get<1>(kv.first);
@@ -2654,9 +2663,10 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
   const bool graph = use_graph && !debug;
   ctx.ctl_all.ensure(B);
   for (size_t i = 0; i < B; i++) ms[i].ctl_out = ctx.ctl_all.p + i;
-  bool ev2 = false;  // ev[2] marks the end of the forests (eager launches only)
-  // Uploads an arena image and runs its launches, replayed from a graph or eagerly
-  // (the end of the leaf kernels is ev[4]).
+  // Uploads an arena image and runs its launches, replayed from a graph or eagerly.
+  // Phase boundaries are wall-clock stamps the kernels write into each member's
+  // control block: an event recorded between two launches stalled the queue for
+  // about 6 us each.
   auto run_ops = [&](Ops& ops, ArenaImage& img, bool as_graph, DevBuf<uint8_t>& arena_d, uint8_t*& arena_h,
                      uint64_t& arena_h_cap) {
     const uint64_t bytes = img.bytes.size();
@@ -2690,12 +2700,6 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
     for (size_t i = 0; i < ops.fn.size(); i++) {
       ops.fn[i](base, st);
       MTC_HIP(hipGetLastError());
-      const uint32_t t = ops.tag[i];
-      if (t >= 100 && (i + 1 == ops.fn.size() || ops.tag[i + 1] < 100)) MTC_HIP(hipEventRecord(ctx.ev[4], st));
-      if (t == 44 || t == 45 || t == 30) {
-        MTC_HIP(hipEventRecord(ctx.ev[2], st));
-        ev2 = true;
-      }
     }
   };
   if (split || graph) {
@@ -2747,11 +2751,15 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
   MTC_HIP(hipEventRecord(ctx.ev[3], st));
   MTC_HIP(hipStreamSynchronize(st));
   ht.mark("sync");
-  float ms_setup = 0, ms_total = 0, ms_leaf = 0, ms_forest = 0;
+  float ms_setup = 0, ms_total = 0;
   MTC_HIP(hipEventElapsedTime(&ms_setup, ctx.ev[0], ctx.ev[1]));
   MTC_HIP(hipEventElapsedTime(&ms_total, ctx.ev[1], ctx.ev[3]));
-  MTC_HIP(hipEventElapsedTime(&ms_leaf, ctx.ev[1], ctx.ev[4]));
-  if (ev2) MTC_HIP(hipEventElapsedTime(&ms_forest, ctx.ev[1], ctx.ev[2]));
+  if (ctx.clock_khz == 0) {
+    int dev = 0, khz = 0;
+    MTC_HIP(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+    ctx.clock_khz = khz;
+  }
   // MTC_TEST_FAIL_MEMBER=i makes member i report an inconsistent state (tests of
   // the failure path; nothing else reads it).
   const char* inject = std::getenv("MTC_TEST_FAIL_MEMBER");
@@ -2778,12 +2786,16 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
     h->diff_cur = ms[i].arena;
     h->last_header = mtc_header{h->baseline_id, ms[i].cur, ms[i].len, h->chunk_size, c.F, c.P, c.S};
     // Timers (src/tree_approach.cpp:1640,1657,1672): set-up (buffers, table reset);
-    // labels and forests (fills, leaf kernels, table passes, both sweeps); the diff
-    // (root compaction to assembly).  A graph-replayed step reports its whole time
-    // as the second.
+    // labels and forests (leaf kernel, table passes, both sweeps); the diff (root
+    // compaction to assembly).  The basic approach has no forests: its whole step
+    // counts as the second.  The leaf time runs from the leaf kernel's first block
+    // to the leaf scan's first block.
+    const double tick_ms = 1.0 / ctx.clock_khz;
+    const float ms_leaf = c.t_leaf > c.t_begin ? (float)((c.t_leaf - c.t_begin) * tick_ms) : 0.0f;
+    const float ms_forest = c.t_forest > c.t_begin ? std::min(ms_total, (float)((c.t_forest - c.t_begin) * tick_ms)) : ms_total;
     h->last_ms[0] = ms_setup;
-    h->last_ms[1] = ev2 ? ms_forest : ms_total;
-    h->last_ms[2] = ev2 ? ms_total - ms_forest : 0.0f;
+    h->last_ms[1] = ms_forest;
+    h->last_ms[2] = ms_total - ms_forest;
     h->last_ms[3] = 0;
     h->last_ms[4] = ms_leaf;
   }

@@ -86,8 +86,13 @@ struct Ctl {
   unsigned long long data_chunks; // chunks in the data section
   unsigned long long diff_size;   // bytes of the diff
   unsigned long long changed;     // leaves that are not fixed duplicates
-  uint32_t pad[52];
+  // Device wall clock (hipDeviceAttributeWallClockRate) when the leaf kernel's
+  // first block starts, when the leaf scan's first block starts and when sweep B
+  // ends: the phase timers, without event records inside the step.
+  unsigned long long t_begin, t_leaf, t_forest;
+  uint32_t pad[46];
 };
+static_assert(sizeof(Ctl) == 256, "the control block is one 256-byte record");
 
 struct Workspace {
   DevBuf<uint32_t> u32a, u32b, u32c;  // scan partials
@@ -116,6 +121,7 @@ struct StepCtx {
   uint32_t* cnt_h = nullptr;  // pinned: chunks to insert per member (steps that pause after the leaf kernel)
   uint32_t cnt_h_cap = 0;
   hipEvent_t ev[5] = {};
+  int clock_khz = 0;  // device wall clock rate (the control block's stamps)
   hipGraphExec_t exec = nullptr;
   std::vector<uint64_t> sig;
   StepCtx() = default;

@@ -173,6 +173,7 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       maybe(w, "count_dev", a.count_dev, 4);
       maybe(w, "total", a.total, 4);
       need(w, "sums", a.sums, 4 * tiles(a.count_max));
+      maybe(w, "stamp", a.stamp, 8);
     });
   } else if (has("8LeafArgs")) {
     check_members<LeafArgs>(name, L, grid, [](const std::string& w, const LeafArgs& a, uint32_t nblk) {
@@ -182,6 +183,7 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       need(w, "list", a.list, 4ull * a.n);
       need(w, "blk_cnt", a.blk_cnt, 4ull * nblk);
       maybe(w, "list_dg", a.list_dg, 16ull * a.n);
+      maybe(w, "stamp", a.stamp, 8);
       if ((uint64_t)nblk * a.tiles_per_block * 256 < a.n) violation(w + ": leaf blocks do not cover the chunks");
       S().leaf_all_labels.push_back(a.all_labels);
     });

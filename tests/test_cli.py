@@ -54,6 +54,9 @@ def test_file_commands_round_trip(tmp_path):
         assert same, (i, first_mismatch(got, expect[i]))
         log = (tmp_path / f"chk{i}.dat.chunk_size.{C}.csv").read_text().splitlines()
         assert log[0].startswith("Approach,Chkpt ID") and log[1].startswith(f"Tree,{i},{C},{nbytes},")
+        row = log[1].split(",")
+        # Comparison Time (labels and forests) and Gather Time (the diff) both measured
+        assert float(row[8]) > 0 and float(row[9]) > 0, row
         size = (tmp_path / f"chk{i}.dat.chunk_size.{C}.size.csv").read_text().strip().split(",")
         assert size[:3] == ["Tree", str(i), str(C)] and size[5] == "32"
     for rid in (0, k - 1):
