@@ -754,6 +754,14 @@ __device__ __forceinline__ void ph_insert(const Launch<DenseArgs>& L, const Wher
       nl = l == L_FIXED ? L_MIN_FIXED : L_MIN_PENDING;
     }
     if (nl >= 0) a.labels[c] = (uint8_t)nl;
+    // Only entries marked for a min or still pending (a zero digest, a failed
+    // probe, or a leaf kernel's label kept) need k_fixmin and k_resolve.
+    const bool later = v && (nl < 0 || nl == L_PENDING || nl == L_MIN_PENDING || nl == L_MIN_FIXED);
+    const uint64_t pm = __ballot(later);
+    uint32_t pbase = 0;
+    if (lane == 0 && pm) pbase = atomicAdd(a.npend, (uint32_t)__popcll(pm));
+    pbase = __shfl(pbase, 0);
+    if (later) a.pend[pbase + __popcll(pm & ((1ull << lane) - 1))] = c;
     claims += claimed;
   }
   const uint32_t wc = wave_sum(claims);
@@ -773,9 +781,9 @@ __global__ __launch_bounds__(kThreads) void k_insert(Launch<DenseArgs> L) {
 __device__ __forceinline__ void ph_fixmin(const Launch<DenseArgs>& L, const Where& w, uint8_t* lds) {
   (void)lds;
   const DenseArgs a = L.args[w.m].global();
-  const uint64_t total = *a.total;
+  const uint64_t total = *a.npend;
   for (uint64_t i = (uint64_t)w.blk * kThreads + threadIdx.x; i < total; i += (uint64_t)w.nblk * kThreads) {
-    const uint32_t c = a.dense[i];
+    const uint32_t c = a.pend[i];
     const uint8_t l = a.labels[c];
     if (l != L_MIN_PENDING && l != L_MIN_FIXED) continue;
     const uint64_t v = ((uint64_t)a.cur << 32) | (uint32_t)(a.n - 1 + c);
@@ -801,9 +809,9 @@ __global__ __launch_bounds__(kThreads) void k_fixmin(Launch<DenseArgs> L) {
 __device__ __forceinline__ void ph_resolve(const Launch<DenseArgs>& L, const Where& w, uint8_t* lds) {
   (void)lds;
   const DenseArgs a = L.args[w.m].global();
-  const uint64_t total = *a.total;
+  const uint64_t total = *a.npend;
   for (uint64_t i = (uint64_t)w.blk * kThreads + threadIdx.x; i < total; i += (uint64_t)w.nblk * kThreads) {
-    const uint32_t c = a.dense[i];
+    const uint32_t c = a.pend[i];
     if (a.labels[c] != L_PENDING) continue;  // a fixed leaf right after a reset
     const uint32_t s = a.leaf_slot[c];
     uint8_t l = L_SHIFT;
@@ -1352,16 +1360,21 @@ __global__ void k_single_leaf(Launch<SweepArgs> L) { ph_single_leaf(L, where(L))
 // Root compaction: one thread per (depth, block) segment, in segment order.
 // ---------------------------------------------------------------------------
 
+// The per-depth segment bases (at most 33) are staged in LDS: each thread finds
+// its depth with a walk over them, which as dependent global loads cost most of
+// the kernel.
 __device__ __forceinline__ void ph_scatter_roots(const Launch<RootArgs>& L, const Where& w, uint8_t* lds) {
-  (void)lds;
+  uint32_t* s_base = reinterpret_cast<uint32_t*>(lds);
   const RootArgs a = L.args[w.m].global();
+  if (threadIdx.x <= a.ndepth) s_base[threadIdx.x] = a.seg_base[threadIdx.x];
+  __syncthreads();
   const uint32_t seg = w.blk * kThreads + threadIdx.x;
   if (seg >= a.nseg) return;
   const uint32_t nf = a.cnt_f[seg], ns = a.cnt_s[seg];
   if (nf + ns == 0) return;
   uint32_t c = 0;
-  while (c + 1 < a.ndepth && a.seg_base[c + 1] <= seg) c++;
-  const uint32_t nb = a.seg_base[c + 1] - a.seg_base[c];  // blocks of the emitting stage (a power of two)
+  while (c + 1 < a.ndepth && s_base[c + 1] <= seg) c++;
+  const uint32_t nb = s_base[c + 1] - s_base[c];  // blocks of the emitting stage (a power of two)
   const uint32_t span = c - (31u - (uint32_t)__clz((int)nb));
   const uint64_t sb = ((1ull << c) - 1) + ((uint64_t)(seg - a.seg_base[c]) << span);
   uint32_t of = a.off_f[seg], os = a.off_s[seg];
@@ -1373,7 +1386,8 @@ __device__ __forceinline__ void ph_scatter_roots(const Launch<RootArgs>& L, cons
 }
 
 __global__ __launch_bounds__(kThreads) void k_scatter_roots(Launch<RootArgs> L) {
-  ph_scatter_roots(L, where(L), nullptr);
+  __shared__ uint32_t s_base[64];
+  ph_scatter_roots(L, where(L), reinterpret_cast<uint8_t*>(s_base));
 }
 
 // ---------------------------------------------------------------------------
@@ -1933,6 +1947,7 @@ static void ensure_buffers(mtc_dedup* h, hipStream_t st) {
   if (h->approach != MTC_APPROACH_BASIC) {  // listed digests beside the list, then dense
     h->list_dg.ensure(n);
     h->dense_dg.ensure(n);
+    h->pend.ensure(n);
   }
   const uint64_t max_blk = (n + kThreads - 1) / kThreads;  // at most one block per tile
   h->blk_cnt.ensure(max_blk);
@@ -2029,7 +2044,7 @@ uint64_t table_slots(uint64_t n, double load_factor) {
 // the table at the given load factor and the diff arena).
 uint64_t state_bytes(uint64_t n, uint32_t chunk_size, double load_factor) {
   const uint64_t table = 32 * (table_slots(n, load_factor) + 1);
-  const uint64_t per_chunk = 16 + 1 + 4 + 4 + 4 + 8 * 2 + 4 + 8 + 2 * (4 + 8) + 4 + 12 + 1 + 2 * 16;
+  const uint64_t per_chunk = 16 + 1 + 4 + 4 + 4 + 8 * 2 + 4 + 8 + 2 * (4 + 8) + 4 + 12 + 1 + 2 * 16 + 4;
   return table + per_chunk * n + (uint64_t)chunk_size * n + (8u << 20);
 }
 
@@ -2277,7 +2292,7 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
     std::vector<CompactArgs> ca;
     std::vector<uint32_t> cb;
     std::vector<DenseArgs> da;
-    std::vector<uint32_t> db;
+    std::vector<uint32_t> db, pb;
     for (auto& M : ms) {
       mtc_dedup* h = M.h;
       ca.push_back(CompactArgs{h->list.p, h->blk_cnt.p, h->blk_off.p, M.tpb, h->dense.p, M.nblk_leaf, h->list_dg.p,
@@ -2285,14 +2300,16 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       cb.push_back((M.nblk_leaf + kCompactPer - 1) / kCompactPer);
       da.push_back(DenseArgs{h->dense.p, h->blk_off.p + M.nblk_leaf, M.n, h->leaf_dg.p, h->leaf_slot.p, h->labels.p,
                              h->table.p, h->table_cap - 1, M.cur, h->ctl_d, h->ctr.p, M.g, M.R0, h->tile_flags.p,
-                             h->tile_flags.p + M.ntile0, h->dense_dg.p, M.all_labels ? 1 : 0});
+                             h->tile_flags.p + M.ntile0, h->dense_dg.p, M.all_labels ? 1 : 0, h->pend.p,
+                             &h->ctl_d->pending});
       db.push_back(grid_for(M.n, 8 * kThreads, 16384));
+      pb.push_back(grid_for(M.n, 8 * kThreads, 1024));  // the pending list is usually short
     }
     ops.add<CompactArgs>(20, img.add(ca, cb), [](Launch<CompactArgs> L, uint32_t g, hipStream_t st) { k_compact<<<g, kThreads, 0, st>>>(L); });
-    const LaunchRec rd = img.add(da, db);
+    const LaunchRec rd = img.add(da, db), rp = img.add(da, pb);
     ops.add<DenseArgs>(22, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_insert<<<g, kThreads, 0, st>>>(L); });
-    ops.add<DenseArgs>(24, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_fixmin<<<g, kThreads, 0, st>>>(L); });
-    ops.add<DenseArgs>(23, rd, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_resolve<<<g, kThreads, 0, st>>>(L); });
+    ops.add<DenseArgs>(24, rp, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_fixmin<<<g, kThreads, 0, st>>>(L); });
+    ops.add<DenseArgs>(23, rp, [](Launch<DenseArgs> L, uint32_t g, hipStream_t st) { k_resolve<<<g, kThreads, 0, st>>>(L); });
   }
   // 4. sweeps: every stage of sweep A (and its inserts) before any stage of sweep B
   {
