@@ -865,6 +865,12 @@ __device__ __forceinline__ uint32_t wave_rank(uint32_t cnt, uint32_t* total) {
   return r;
 }
 
+// Wave-wide total of small per-lane counts (0..7): three ballots, no per-lane prefix.
+__device__ __forceinline__ uint32_t wave_total3(uint32_t cnt) {
+  return (uint32_t)__popcll(__ballot(cnt & 1)) + 2u * (uint32_t)__popcll(__ballot(cnt & 2)) +
+         4u * (uint32_t)__popcll(__ballot(cnt & 4));
+}
+
 __device__ __forceinline__ Digest shfl_down_dg(const Digest& d, int s) {
   return Digest{__shfl_down(d.lo, s), __shfl_down(d.hi, s)};
 }
@@ -926,7 +932,6 @@ __device__ __forceinline__ void sweep_a_tile(const SweepArgs& a, uint64_t tile, 
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
   const uint64_t base = tile << R;
   uint8_t l0[4];
-  Digest g0[4];
   bool any = false;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
@@ -936,15 +941,18 @@ __device__ __forceinline__ void sweep_a_tile(const SweepArgs& a, uint64_t tile, 
     const uint64_t p = base + lp;
     if (a.stage0) {
       if (p >= a.g.m) continue;
-      const uint64_t c = bottom_chunk(a.g, p);
-      l0[i] = a.labels[c];
-      if (l0[i] == L_FIRST) g0[i] = a.leaf_dg[c];
+      l0[i] = a.labels[bottom_chunk(a.g, p)];
     } else {
       l0[i] = a.in_lab[p];
-      if (l0[i] == L_FIRST) g0[i] = a.in_dg[p];
     }
     any |= (l0[i] == L_FIRST);
   }
+  // Digests of positions 4 lane + i, loaded where a pair of first occurrences is
+  // hashed (holding all four per lane cost wavefront slots in registers).
+  auto dg_at = [&](int i) -> Digest {
+    const uint64_t p = base + 4 * lane + i;
+    return a.stage0 ? a.leaf_dg[bottom_chunk(a.g, p)] : a.in_dg[p];
+  };
   // Depth D-1 leaves among this tile's level-1 positions (stage 0 only).
   uint8_t lf[2] = {L_NONE, L_NONE};
   Digest gf[2];
@@ -977,7 +985,10 @@ __device__ __forceinline__ void sweep_a_tile(const SweepArgs& a, uint64_t tile, 
       l1[k] = lf[k];
       g1[k] = gf[k];
     } else {
-      s.combine(a.d_in - 1, q, l0[2 * k], g0[2 * k], l0[2 * k + 1], g0[2 * k + 1], l1[k], g1[k]);
+      const bool both = l0[2 * k] == L_FIRST && l0[2 * k + 1] == L_FIRST;
+      const Digest da = both ? dg_at(2 * k) : Digest{0, 0};
+      const Digest db = both ? dg_at(2 * k + 1) : Digest{0, 0};
+      s.combine(a.d_in - 1, q, l0[2 * k], da, l0[2 * k + 1], db, l1[k], g1[k]);
     }
   }
   s.flush();
@@ -1103,9 +1114,10 @@ struct SweepB {
 
   // Stage this level's roots (children at depth c) for tile `tile`.
   __device__ void flush(uint32_t c, uint64_t tile) {
-    uint32_t tot, ftot;
+    if (!__ballot(ne != 0)) return;  // no root at this level (nf is zero too)
+    uint32_t tot;
     const uint32_t rank = wave_rank(ne, &tot);
-    wave_rank(nf, &ftot);
+    const uint32_t ftot = wave_total3(nf);
     uint64_t* dst = a.staging + heap_id(c, tile << (c - a.d_out)) + rank;
     if (ne > 0) dst[0] = e0;
     if (ne > 1) dst[1] = e1;
@@ -1131,11 +1143,13 @@ __device__ void fixed_tile(const SweepArgs& a, uint64_t tile, uint32_t R, uint32
       }
 }
 
-// Labels of a tile's positions, four per lane: one 32-bit load when whole and aligned.
+// Labels of a tile's positions, four per lane (packed in one word, L_NONE where
+// the position does not exist): one 32-bit load when whole and aligned.
 struct Lab4 {
-  uint8_t l[4];
+  uint32_t w;     // label of position 4 lane + i in bits 8i..8i+7
   uint32_t nval;  // valid positions of this lane
   uint64_t c0;    // index of the first one in labels (stage 0) or in_lab
+  __device__ uint8_t l(int i) const { return (uint8_t)(w >> (8 * i)); }
 };
 
 __device__ __forceinline__ Lab4 load_lab4(const SweepArgs& a, uint64_t base, uint32_t T, uint32_t lane) {
@@ -1145,16 +1159,13 @@ __device__ __forceinline__ Lab4 load_lab4(const SweepArgs& a, uint64_t base, uin
   const uint8_t* lsrc = a.stage0 ? a.labels + r.c0 : a.in_lab + r.c0;
   r.nval = lp0 < T ? min(4u, T - lp0) : 0u;
   if (a.stage0) r.nval = (base + lp0 >= a.g.m) ? 0u : (uint32_t)min<uint64_t>(r.nval, a.g.m - (base + lp0));
-#pragma unroll
-  for (int i = 0; i < 4; i++) r.l[i] = L_NONE;
   if (r.nval == 4 && ((uintptr_t)lsrc & 3) == 0) {
-    const uint32_t w4 = *reinterpret_cast<const uint32_t*>(lsrc);
-#pragma unroll
-    for (int i = 0; i < 4; i++) r.l[i] = (uint8_t)(w4 >> (8 * i));
+    r.w = *reinterpret_cast<const uint32_t*>(lsrc);
   } else {
+    r.w = 0x01010101u * L_NONE;  // L_NONE in every byte
 #pragma unroll
     for (int i = 0; i < 4; i++)
-      if ((uint32_t)i < r.nval) r.l[i] = lsrc[i];
+      if ((uint32_t)i < r.nval) r.w = (r.w & ~(0xFFu << (8 * i))) | ((uint32_t)lsrc[i] << (8 * i));
   }
   return r;
 }
@@ -1169,9 +1180,7 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
   const uint32_t nval = lb.nval;
   uint8_t l0[4];
 #pragma unroll
-  for (int i = 0; i < 4; i++) l0[i] = lb.l[i];
-  Digest g0[4];
-  uint32_t s0[4];
+  for (int i = 0; i < 4; i++) l0[i] = lb.l(i);
   bool busy = false;
 #pragma unroll
   for (int i = 0; i < 4; i++) busy |= ((uint32_t)i < nval && l0[i] != L_FIXED);
@@ -1205,11 +1214,31 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
 #pragma unroll
     for (int k = 0; k < 2; k++) pair |= (l0[2 * k] == l0[2 * k + 1] && is_region(l0[2 * k]));
     if (!__ballot(pair)) {
-      SweepB<HK> s{a};
+      // The region leaves are the tile's roots, ascending (what SweepB::emit and
+      // flush would stage, written directly).
+      uint32_t rm = 0, fm = 0;
 #pragma unroll
-      for (int i = 0; i < 4; i++)
-        if (is_region(l0[i])) s.emit(l0[i], (uint32_t)kLeafTag, heap_id(a.d_in, base + lp0 + i));
-      s.flush(a.d_in, tile);
+      for (int i = 0; i < 4; i++) {
+        rm |= (uint32_t)is_region(l0[i]) << i;
+        fm |= (uint32_t)(l0[i] == L_FIRST) << i;
+      }
+      if (__ballot(rm != 0)) {
+        uint32_t tot;
+        const uint32_t rank = wave_rank((uint32_t)__popc(rm), &tot);
+        const uint32_t ftot = wave_total3((uint32_t)__popc(fm));
+        uint64_t* dst = a.staging + heap_id(a.d_in, base) + rank;
+        const uint32_t node0 = (uint32_t)heap_id(a.d_in, base + lp0);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          if ((rm >> i) & 1) {
+            const uint64_t tag = ((fm >> i) & 1) ? kFirstTag : kLeafTag;
+            dst[__popc(rm & ((1u << i) - 1))] = (tag << 32) | (node0 + (uint32_t)i);
+          }
+        if (lane == 0) {
+          a.cnt_f[a.seg_base[a.d_in] + tile] = ftot;
+          a.cnt_s[a.seg_base[a.d_in] + tile] = tot - ftot;
+        }
+      }
       if (lane == 0) {
         a.out_lab[tile] = L_DONE;
         if (a.d_out == 0) {
@@ -1220,18 +1249,16 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
       return;
     }
   }
-  // Digests and table slots of the shifted positions (a leaf's slot is read when
-  // its root entry is consumed).
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-    if ((uint32_t)i < nval && l0[i] == L_SHIFT) {
-      g0[i] = a.stage0 ? a.leaf_dg[c0 + i] : a.in_dg[c0 + i];
-      s0[i] = a.stage0 ? (uint32_t)kLeafTag : a.in_slot[c0 + i];
-    }
+  // Level 1.  The digests and table slots of shifted positions are loaded only
+  // where a pair needs them (a shifted pair is hashed; a shifted root carries its
+  // slot; a leaf's slot is read when its root entry is consumed): holding all four
+  // per lane cost the kernel half its wavefront slots in registers.
   SweepB<HK> s{a};
   uint8_t l1[2] = {L_NONE, L_NONE};
   Digest g1[2];
   uint32_t s1[2] = {0, 0};
+  auto slot_at = [&](int i) -> uint32_t { return a.stage0 ? (uint32_t)kLeafTag : a.in_slot[c0 + i]; };
+  auto dg_at = [&](int i) -> Digest { return a.stage0 ? a.leaf_dg[c0 + i] : a.in_dg[c0 + i]; };
 #pragma unroll
   for (int k = 0; k < 2; k++) {
     const uint32_t j = 2 * lane + k;
@@ -1241,10 +1268,15 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
       l1[k] = lf[k];
       g1[k] = gf[k];
       s1[k] = sf[k];
-    } else {
-      s.combine(a.d_in - 1, q, l0[2 * k], g0[2 * k], s0[2 * k], l0[2 * k + 1], g0[2 * k + 1], s0[2 * k + 1], l1[k],
-                g1[k], s1[k]);
+      continue;
     }
+    const uint8_t la = l0[2 * k], lb2 = l0[2 * k + 1];
+    const bool both_shift = la == L_SHIFT && lb2 == L_SHIFT;
+    const Digest da = both_shift ? dg_at(2 * k) : Digest{0, 0};
+    const Digest db = both_shift ? dg_at(2 * k + 1) : Digest{0, 0};
+    const uint32_t sa = (la == L_SHIFT) ? slot_at(2 * k) : 0u;
+    const uint32_t sb = (lb2 == L_SHIFT) ? slot_at(2 * k + 1) : 0u;
+    s.combine(a.d_in - 1, q, la, da, sa, lb2, db, sb, l1[k], g1[k], s1[k]);
   }
   s.flush(a.d_in, tile);
   uint8_t lc = l1[0];
@@ -1259,8 +1291,13 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
   for (uint32_t r = 3; r <= R; r++) {
     const int sh = 1 << (r - 3);
     const uint8_t lp = (uint8_t)__shfl_down((int)lc, sh);
-    const Digest gp = shfl_down_dg(gc, sh);
-    const uint32_t sp = __shfl_down(sc, sh);
+    // Digests and slots matter only for shifted pairs (a fresh run has none).
+    Digest gp{0, 0};
+    uint32_t sp = 0;
+    if (__ballot(lc == L_SHIFT)) {
+      gp = shfl_down_dg(gc, sh);
+      sp = __shfl_down(sc, sh);
+    }
     const uint32_t j = lane >> (r - 2);
     if ((lane & ((2u << (r - 3)) - 1)) == 0 && j < (T >> r)) {
       uint8_t lo;
