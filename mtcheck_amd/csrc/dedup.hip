@@ -1071,16 +1071,12 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_a(Launch<SweepArgs> L) {
 template <int HK>
 struct SweepB {
   const SweepArgs& a;
-  uint64_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;  // up to four roots per lane and level, in order
+  uint64_t* ent;  // this wavefront's LDS: up to four roots per lane and level, entry k of lane l at [64 k + l]
   uint32_t ne = 0, nf = 0;
 
   __device__ void emit(uint8_t l, uint32_t slot, uint64_t node) {
     if (!is_region(l)) return;
-    const uint64_t v = ((l == L_FIRST ? kFirstTag : (uint64_t)slot) << 32) | (uint32_t)node;
-    e0 = ne == 0 ? v : e0;  // selects instead of an indexed array keep the entries in registers
-    e1 = ne == 1 ? v : e1;
-    e2 = ne == 2 ? v : e2;
-    e3 = ne == 3 ? v : e3;
+    ent[64 * ne + lane_id()] = ((l == L_FIRST ? kFirstTag : (uint64_t)slot) << 32) | (uint32_t)node;
     ne++;
     nf += (l == L_FIRST);
   }
@@ -1119,10 +1115,8 @@ struct SweepB {
     const uint32_t rank = wave_rank(ne, &tot);
     const uint32_t ftot = wave_total3(nf);
     uint64_t* dst = a.staging + heap_id(c, tile << (c - a.d_out)) + rank;
-    if (ne > 0) dst[0] = e0;
-    if (ne > 1) dst[1] = e1;
-    if (ne > 2) dst[2] = e2;
-    if (ne > 3) dst[3] = e3;
+    const uint32_t lane = lane_id();
+    for (uint32_t k = 0; k < ne; k++) dst[k] = ent[64 * k + lane];
     if (lane_id() == 0 && tot) {  // counts are zero-filled before the sweep
       a.cnt_f[a.seg_base[c] + tile] = ftot;
       a.cnt_s[a.seg_base[c] + tile] = tot - ftot;
@@ -1172,7 +1166,8 @@ __device__ __forceinline__ Lab4 load_lab4(const SweepArgs& a, uint64_t base, uin
 
 // One tile of sweep B (the whole wavefront), its labels already loaded.
 template <int HK>
-__device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, const Lab4& lb, uint32_t lane) {
+__device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, const Lab4& lb, uint32_t lane,
+                                             uint64_t* ent) {
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
   const uint64_t base = tile << R;
   const uint32_t lp0 = 4 * lane;
@@ -1184,21 +1179,14 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
   bool busy = false;
 #pragma unroll
   for (int i = 0; i < 4; i++) busy |= ((uint32_t)i < nval && l0[i] != L_FIXED);
-  uint8_t lf[2] = {L_NONE, L_NONE};
-  Digest gf[2];
-  uint32_t sf[2];
+  uint8_t lf[2] = {L_NONE, L_NONE};  // depth D-1 leaves among the level-1 positions
   if (a.stage0) {
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       const uint32_t j = 2 * lane + k;
       const uint64_t q = (base >> 1) + j;
       if (j >= T / 2 || q < a.g.half) continue;
-      const uint64_t c = q - a.g.half;
-      lf[k] = a.labels[c];
-      if (lf[k] == L_SHIFT) {
-        gf[k] = a.leaf_dg[c];
-        sf[k] = (uint32_t)kLeafTag;
-      }
+      lf[k] = a.labels[q - a.g.half];
       busy |= (lf[k] != L_FIXED);
     }
   }
@@ -1253,7 +1241,7 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
   // where a pair needs them (a shifted pair is hashed; a shifted root carries its
   // slot; a leaf's slot is read when its root entry is consumed): holding all four
   // per lane cost the kernel half its wavefront slots in registers.
-  SweepB<HK> s{a};
+  SweepB<HK> s{a, ent};
   uint8_t l1[2] = {L_NONE, L_NONE};
   Digest g1[2];
   uint32_t s1[2] = {0, 0};
@@ -1266,8 +1254,10 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
     const uint64_t q = (base >> 1) + j;
     if (a.stage0 && q >= a.g.half) {
       l1[k] = lf[k];
-      g1[k] = gf[k];
-      s1[k] = sf[k];
+      if (lf[k] == L_SHIFT) {
+        g1[k] = a.leaf_dg[q - a.g.half];
+        s1[k] = (uint32_t)kLeafTag;
+      }
       continue;
     }
     const uint8_t la = l0[2 * k], lb2 = l0[2 * k + 1];
@@ -1330,9 +1320,9 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
 
 template <int HK>
 __device__ __forceinline__ void ph_sweep_b_tiles(const SweepArgs& a, uint64_t t0, uint32_t tpw, uint64_t ntiles,
-                                                 uint32_t R, uint32_t T, uint32_t lane) {
+                                                 uint32_t R, uint32_t T, uint32_t lane, uint64_t* ent) {
   if (tpw == 1) {
-    sweep_b_tile<HK>(a, t0, load_lab4(a, t0 << R, T, lane), lane);
+    sweep_b_tile<HK>(a, t0, load_lab4(a, t0 << R, T, lane), lane, ent);
     return;
   }
   const uint64_t t1 = min(t0 + tpw, ntiles);
@@ -1352,7 +1342,7 @@ __device__ __forceinline__ void ph_sweep_b_tiles(const SweepArgs& a, uint64_t t0
       i = __ffsll((long long)todo) - 1;
       lb = load_lab4(a, (t0 + i) << R, T, lane);
     }
-    sweep_b_tile<HK>(a, t, cur, lane);
+    sweep_b_tile<HK>(a, t, cur, lane, ent);
   }
 }
 
@@ -1362,7 +1352,7 @@ __device__ __forceinline__ void ph_sweep_b_tiles(const SweepArgs& a, uint64_t t0
 // fixed and its root counts stay zero.
 template <int HK, int WPB>
 __device__ __forceinline__ void ph_sweep_b(const Launch<SweepArgs>& L, const Where& wh, uint8_t* lds) {
-  (void)lds;
+  uint64_t* ent = reinterpret_cast<uint64_t*>(lds) + 256 * (threadIdx.x >> 6);  // this wavefront's root entries
   const SweepArgs a = L.args[wh.m].global();
   const uint32_t lane = lane_id();
   const uint64_t ntiles = 1ull << a.d_out;
@@ -1370,13 +1360,16 @@ __device__ __forceinline__ void ph_sweep_b(const Launch<SweepArgs>& L, const Whe
   const uint32_t tpw = a.stage0 ? a.tiles_per_wave : 1u;
   const uint64_t t0 = ((uint64_t)wh.blk * WPB + (threadIdx.x >> 6)) * tpw;
   if (t0 >= ntiles) return;  // whole wavefront
-  ph_sweep_b_tiles<HK>(a, t0, tpw, ntiles, R, T, lane);
+  ph_sweep_b_tiles<HK>(a, t0, tpw, ntiles, R, T, lane, ent);
   if (a.d_out == 0 && lane == 0) a.ctl->t_forest = wall_clock64();  // the member's last sweep stage
 }
 
+// Six wavefronts per SIMD (at most 80 VGPRs; four spill): 146 against 154 us at
+// five for the first stage at 8 GiB, 128-byte chunks, 2% change.
 template <int HK, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_sweep_b(Launch<SweepArgs> L) {
-  ph_sweep_b<HK, WPB>(L, where(L), nullptr);
+__global__ __launch_bounds__(64 * WPB, 6) void k_sweep_b(Launch<SweepArgs> L) {
+  __shared__ __align__(16) uint64_t ent[WPB * 256];
+  ph_sweep_b<HK, WPB>(L, where(L), reinterpret_cast<uint8_t*>(ent));
 }
 
 // A single chunk: the leaf is the root.
