@@ -194,6 +194,12 @@ int mtc_copy_labels(const mtc_dedup* h, void* host_dst, uint64_t capacity);
  * take effect at the next baseline.  Results do not depend on them. */
 int mtc_set_tuning(mtc_dedup* h, uint32_t leaf_tiles_per_block, float load_factor, uint32_t fuse_levels);
 
+/* First-stage tiles of 256 positions each wavefront of sweep A (first occurrences)
+ * and sweep B (shifted duplicates and roots) walks, 1..64, or 0 for the library's
+ * default (8 and 8); no counterpart in the reference.  Applies from the next
+ * checkpoint.  Results do not depend on it. */
+int mtc_set_sweep_tiles(mtc_dedup* h, uint32_t tiles_a, uint32_t tiles_b);
+
 /* Batched checkpoints.  The reference checkpoints one buffer per deduplicator
  * object per call (src/tree_approach.cpp:1577-1776; one object per MPI rank,
  * src/dedup_files_mpi.cpp:72-81).  A batch runs one checkpoint of each of `count`

@@ -25,7 +25,7 @@ EXPORTS = [
     "mtc_create", "mtc_destroy", "mtc_checkpoint", "mtc_diff_device", "mtc_copy_diff", "mtc_copy_diff_async",
     "mtc_copy_wait", "mtc_get_stats",
     "mtc_restart", "mtc_restart_basic", "mtc_restart_noalloc", "mtc_restart_times", "mtc_digest_host", "mtc_hash_chunks", "mtc_set_compat", "mtc_set_debug", "mtc_copy_tree", "mtc_copy_labels",
-    "mtc_last_error", "mtc_status_string", "mtc_set_tuning", "mtc_batch_create", "mtc_batch_destroy",
+    "mtc_last_error", "mtc_status_string", "mtc_set_tuning", "mtc_set_sweep_tiles", "mtc_batch_create", "mtc_batch_destroy",
     "mtc_checkpoint_batch", "mtc_batch_last_error", "mtc_server_create", "mtc_server_destroy", "mtc_server_add_client",
     "mtc_server_client", "mtc_server_remove_client", "mtc_server_submit", "mtc_server_result", "mtc_server_stats",
     "mtc_server_last_error",
@@ -85,6 +85,7 @@ def lib():
         L.mtc_status_string.argtypes = [i32]
         L.mtc_status_string.restype = ctypes.c_char_p
         L.mtc_set_tuning.argtypes = [vp, u32, ctypes.c_float, u32]
+        L.mtc_set_sweep_tiles.argtypes = [vp, u32, u32]
         L.mtc_batch_create.argtypes = [ctypes.POINTER(vp), i32, i32]
         L.mtc_batch_destroy.argtypes = [vp]
         L.mtc_checkpoint_batch.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(u64),

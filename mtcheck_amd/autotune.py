@@ -1,8 +1,10 @@
-"""Schedule autotuner: per chunk size it picks the sweep fuse depth (and, for chunk
-sizes the leaf kernel does not stage through LDS, the leaf kernel's tiles per
-block) by timing incremental checkpoints of a synthetic sample buffer on the
-device.  The table load factor stays at its default.  The choice never changes
-results (diffs are identical for every schedule), only speed."""
+"""Schedule autotuner: per chunk size it times incremental checkpoints of a
+synthetic sample buffer on the device under each schedule of a search space and
+keeps the fastest, unless the library's default is within MARGIN of it.  A
+schedule is (leaf tiles per block, table load factor, tree levels fused per sweep
+stage, sweep A tiles per wavefront, sweep B tiles per wavefront).  The choice
+never changes results (diffs are identical for every schedule), only speed and,
+through the load factor, the table's memory."""
 from __future__ import annotations
 
 import itertools
@@ -11,30 +13,34 @@ import time
 from .tree import MURMUR3, TreeDeduplicator
 from .workload import CheckpointSeries
 
-DEFAULT = (1, 0.125, 8)
-# The load factor stays at the default: a fuller table is rehashed on the next
-# checkpoint, and a sparser one costs memory for little speed.
-CANDIDATES = list(itertools.product((1, 2, 4, 8), (0.125,), (6, 7, 8)))
+DEFAULT = (1, 0.125, 8, 8, 8)
+LOAD_FACTORS = (0.125, 0.25)   # a 0.25 table holds half the slots (half the memory)
+FUSE = (6, 7, 8)
+SWEEP_TILES = ((8, 8), (4, 8), (8, 16), (4, 4))
+LEAF_TILES = (1, 2, 4, 8)
 MARGIN = 0.03  # a schedule replaces the default only when it is this much faster
 # Chunk sizes the leaf kernel stages through LDS.  For these, one tile per block
 # selects the single-tile leaf kernel; at full size one to eight tiles per block
 # run within 2% of each other (64-byte chunks, 8 GiB: 1.83-1.91 ms), below the
-# margin, so a 256 MiB sample would pick among them by noise.  Only the fuse
-# depth is tuned for them.
+# margin, so the leaf tiles are searched only for the other chunk sizes.
 STAGED = {16, 32, 64, 128, 256, 512, 1024, 2048, 4096}
 
 
 def candidates_for(chunk_size: int):
-    if chunk_size in STAGED:
-        return [t for t in CANDIDATES if t[0] == 1]
-    return CANDIDATES
+    leaf = (1,) if chunk_size in STAGED else LEAF_TILES
+    return [(t, lf, f, sa, sb) for t, lf, f, (sa, sb) in itertools.product(leaf, LOAD_FACTORS, FUSE, SWEEP_TILES)]
 
 
-def time_schedule(series: CheckpointSeries, bufs, hash_kind: int, tuning, reps: int = 2) -> float:
-    """Mean seconds of the incremental checkpoints of the pre-generated versions."""
+def apply(d: TreeDeduplicator, sched) -> None:
+    d.set_tuning(*sched[:3])
+    d.set_sweep_tiles(*sched[3:])
+
+
+def time_schedule(series: CheckpointSeries, bufs, hash_kind: int, sched, reps: int = 2) -> float:
+    """Best mean seconds per incremental checkpoint of the pre-generated versions."""
     import torch
     d = TreeDeduplicator(series.chunk_size, hash_kind=hash_kind)
-    d.set_tuning(*tuning)
+    apply(d, sched)
     d.checkpoint_device(bufs[0], make_baseline=True)
     best = float("inf")
     for _ in range(reps):
@@ -50,8 +56,9 @@ def time_schedule(series: CheckpointSeries, bufs, hash_kind: int, tuning, reps: 
 
 
 def autotune(chunk_sizes, hash_kind: int = MURMUR3, sample_bytes: int = 256 << 20, change: float = 0.02,
-             steps: int = 3, candidates=None) -> dict:
-    """{chunk size: (leaf_tiles_per_block, load_factor, fuse_levels)}."""
+             steps: int = 3, candidates=None, report: dict | None = None) -> dict:
+    """{chunk size: schedule}.  `report`, when given, receives per chunk size the
+    searched space, the default's and the pick's time and the gain."""
     import torch
     out = {}
     for C in sorted(set(int(c) for c in chunk_sizes)):
@@ -63,10 +70,18 @@ def autotune(chunk_sizes, hash_kind: int = MURMUR3, sample_bytes: int = 256 << 2
                 v.copy_(bufs[-1])
             series.apply_device(v, k)
             bufs.append(v)
-        timings = {t: time_schedule(series, bufs, hash_kind, t) for t in (candidates or candidates_for(C))}
+        space = candidates or candidates_for(C)
+        timings = {t: time_schedule(series, bufs, hash_kind, t) for t in space}
         timings[DEFAULT] = min(timings.get(DEFAULT, float("inf")), time_schedule(series, bufs, hash_kind, DEFAULT))
         best = min(timings, key=timings.get)
         out[C] = best if timings[best] < (1 - MARGIN) * timings[DEFAULT] else DEFAULT
+        if report is not None:
+            report[C] = {"searched": len(timings), "sample_bytes": sample_bytes,
+                         "space": {"leaf_tiles": sorted({t[0] for t in space}), "load_factor": list(LOAD_FACTORS),
+                                   "fuse": list(FUSE), "sweep_tiles": [list(s) for s in SWEEP_TILES]},
+                         "default_ms": round(timings[DEFAULT] * 1e3, 4), "best": list(best),
+                         "best_ms": round(timings[best] * 1e3, 4), "pick": list(out[C]),
+                         "gain": round(timings[DEFAULT] / timings[out[C]], 4)}
         del bufs
         torch.cuda.empty_cache()
     return out

@@ -323,6 +323,15 @@ int mtc_set_tuning(mtc_dedup* h, uint32_t leaf_tiles_per_block, float load_facto
   });
 }
 
+int mtc_set_sweep_tiles(mtc_dedup* h, uint32_t tiles_a, uint32_t tiles_b) {
+  if (!h) return MTC_ERR_INVALID_ARG;
+  return guarded(h, [&] {
+    if (tiles_a > 64 || tiles_b > 64) throw Status{MTC_ERR_INVALID_ARG, "sweep tiles per wavefront out of range (0..64)"};
+    h->tune.sweep_tiles_a = tiles_a;
+    h->tune.sweep_tiles_b = tiles_b;
+  });
+}
+
 int mtc_batch_create(mtc_batch** out, int device, int use_graph) {
   if (!out) return MTC_ERR_INVALID_ARG;
   *out = nullptr;

@@ -2406,7 +2406,9 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
             // stage 0: several tiles per wavefront (their flags are read at once; sweep
             // B walks every tile when it records the debug labels); four wavefronts
             // per block
-            a.tiles_per_wave = !a.stage0 ? 1u : (pass == 0 ? kTilesPerWaveA : (a.dbg_labels ? 1u : kTilesPerWaveB));
+            const uint32_t ta = h->tune.sweep_tiles_a ? h->tune.sweep_tiles_a : kTilesPerWaveA;
+            const uint32_t tb = h->tune.sweep_tiles_b ? h->tune.sweep_tiles_b : kTilesPerWaveB;
+            a.tiles_per_wave = !a.stage0 ? 1u : (pass == 0 ? ta : (a.dbg_labels ? 1u : tb));
             const uint64_t waves = ((1ull << a.d_out) + a.tiles_per_wave - 1) / a.tiles_per_wave;
             const uint32_t wpb = pass == 0 ? kSweepWpbA : kSweepWpbB;
             blocks = (uint32_t)((waves + wpb - 1) / wpb);

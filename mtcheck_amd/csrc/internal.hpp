@@ -147,6 +147,8 @@ struct Tune {
   uint64_t leaf_tiles_per_block = 1;  // 256-chunk tiles each leaf block streams
   double load_factor = 0.125;         // table entries per slot at a baseline (capacity = n / load_factor)
   uint32_t fuse = 8;                  // tree levels per sweep stage (1..8)
+  uint32_t sweep_tiles_a = 0;         // first-stage tiles per wavefront of sweep A / sweep B (0: the
+  uint32_t sweep_tiles_b = 0;         // library's default)
 };
 
 }  // namespace mtc

@@ -103,6 +103,10 @@ class TreeDeduplicator:
         """Schedule parameters; load factor and fuse depth apply from the next baseline."""
         check(lib().mtc_set_tuning(self._h, leaf_tiles_per_block, load_factor, fuse_levels), self._h)
 
+    def set_sweep_tiles(self, tiles_a: int = 0, tiles_b: int = 0) -> None:
+        """First-stage tiles per wavefront of sweep A and sweep B (0: the default)."""
+        check(lib().mtc_set_sweep_tiles(self._h, tiles_a, tiles_b), self._h)
+
     def set_debug(self, on: bool = True) -> None:
         check(lib().mtc_set_debug(self._h, 1 if on else 0), self._h)
 

@@ -244,6 +244,8 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       need(w, "cnt_f", a.cnt_f, 4);
       need(w, "cnt_s", a.cnt_s, 4);
       need(w, "seg_base", a.seg_base, 4ull * (a.g.D + 2));
+      // a stage-0 wavefront reads its tiles' flags with one lane per tile
+      if (a.stage0 && (a.tiles_per_wave == 0 || a.tiles_per_wave > 64)) violation(w + ": tiles per wavefront out of 1..64");
       maybe(w, "dbg_tree", a.dbg_tree, 16 * (2 * n - 1));
       maybe(w, "dbg_labels", a.dbg_labels, 2 * n - 1);
       need(w, "tile_first", a.tile_first, 1);

@@ -27,7 +27,9 @@ def run_series(use_graph, steps=5, rebase_at=3, tuning=None):
     mates = [M.TreeDeduplicator(C, hash_kind=hk) for _, C, hk in MEMBERS]
     if tuning:
         for d in mates:
-            d.set_tuning(*tuning)
+            d.set_tuning(*tuning[:3])
+            if len(tuning) > 3:
+                d.set_sweep_tiles(*tuning[3:])
     batch = M.Batch(use_graph=use_graph)
     out = []
     for k in range(steps):
@@ -50,8 +52,8 @@ def test_batch_equals_single(use_graph):
 
 
 def test_batch_tuning_does_not_change_results():
-    a, _ = run_series(True, steps=3, tuning=(4, 0.4, 6))
-    b, _ = run_series(False, steps=3, tuning=(2, 0.15, 8))
+    a, _ = run_series(True, steps=3, tuning=(4, 0.4, 6, 1, 64))
+    b, _ = run_series(False, steps=3, tuning=(2, 0.15, 8, 3, 5))
     assert a == b
 
 

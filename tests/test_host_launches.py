@@ -110,6 +110,8 @@ def test_basic_steps_launch_in_bounds(fake, C, size):
     (128, 128, 0, None),
     (64, 64 * 5000, 0, (4, 0.25, 3)),
     (100, 100 * 777, 1, (2, 0.125, 5)),
+    (128, 128 * 70000 + 9, 0, (1, 0.25, 8, 64, 3)),
+    (128, 128 * 40000, 1, (1, 0.125, 7, 1, 1)),
     (4096, 4096 * 300, 0, None),
 ])
 def test_tree_steps_launch_in_bounds(fake, C, size, hk, tuning):
@@ -118,6 +120,9 @@ def test_tree_steps_launch_in_bounds(fake, C, size, hk, tuning):
     h = create(L, C, TREE, hk)
     if tuning:
         assert L.mtc_set_tuning(h, tuning[0], ctypes.c_float(tuning[1]), tuning[2]) == OK
+        if len(tuning) > 3:
+            assert L.mtc_set_sweep_tiles(h, tuning[3], tuning[4]) == OK
+    assert L.mtc_set_sweep_tiles(h, 65, 0) == INVALID_ARG and L.mtc_set_sweep_tiles(h, 0, 65) == INVALID_ARG
     data = dev_alloc(L, size)
     for k in range(4):
         if k == 2:
