@@ -116,11 +116,12 @@ struct DenseArgs {
   uint8_t* tile_busy;   // per stage-0 tile: holds a leaf that is not a fixed duplicate
   const Digest* dense_dg;  // the listed chunks' digests, in dense order
   int leaf_labels;         // the leaf kernel labelled the chunks (otherwise they count as pending)
-  uint32_t* pend;          // k_insert lists the chunks k_fixmin and k_resolve visit ...
-  uint32_t* npend;         // ... and counts them here
+  uint32_t* pend;          // k_insert lists the chunks k_fixmin and k_resolve visit, each group of
+                           // 64 dense entries at the front of its own 64 words ...
+  uint8_t* pcnt;           // ... and counts them per group (ceil(n / 64) bytes)
   __device__ DenseArgs global() const {
     DenseArgs c = *this;
-    MTC_GLOBAL(dense); MTC_GLOBAL(total); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(labels); MTC_GLOBAL(tab); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(tile_first); MTC_GLOBAL(tile_busy); MTC_GLOBAL(dense_dg); MTC_GLOBAL(pend); MTC_GLOBAL(npend);
+    MTC_GLOBAL(dense); MTC_GLOBAL(total); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(labels); MTC_GLOBAL(tab); MTC_GLOBAL(ctl); MTC_GLOBAL(ctr); MTC_GLOBAL(tile_first); MTC_GLOBAL(tile_busy); MTC_GLOBAL(dense_dg); MTC_GLOBAL(pend); MTC_GLOBAL(pcnt);
     return c;
   }
 };

@@ -90,8 +90,7 @@ struct Ctl {
   // first block starts, when the leaf scan's first block starts and when sweep B
   // ends: the phase timers, without event records inside the step.
   unsigned long long t_begin, t_leaf, t_forest;
-  uint32_t pending;  // listed chunks k_fixmin and k_resolve still have to visit
-  uint32_t pad[45];
+  uint32_t pad[46];
 };
 static_assert(sizeof(Ctl) == 256, "the control block is one 256-byte record");
 
@@ -182,6 +181,7 @@ struct mtc_dedup {
   mtc::DevBuf<mtc::Digest> list_dg;   // tree handles: their digests beside the list entries
   mtc::DevBuf<mtc::Digest> dense_dg;  // and concatenated
   mtc::DevBuf<uint32_t> pend;         // listed chunks left for k_fixmin / k_resolve
+  mtc::DevBuf<uint8_t> pcnt;          // their number per group of 64 dense entries
   mtc::DevBuf<uint32_t> blk_cnt, blk_off;
   mtc::DevBuf<uint8_t> tile_flags;    // per stage-0 tile: has-first, then busy
   mtc::DevBuf<mtc::Slot> table;

@@ -206,7 +206,7 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       need(w, "labels", a.labels, a.n);
       need(w, "dense_dg", a.dense_dg, 16ull * a.n);  // tree members' digests come only this way
       need(w, "pend", a.pend, 4ull * a.n);
-      need(w, "npend", a.npend, 4);
+      need(w, "pcnt", a.pcnt, (a.n + 63) / 64);
       need(w, "tab", a.tab, 32ull * (a.mask + 2));
       need(w, "ctl", a.ctl, kCtl);
       need(w, "ctr", a.ctr, 8);
