@@ -161,6 +161,14 @@ struct SweepArgs {
   }
 };
 
+// The last sweep stages of a member, each small enough for one block of
+// kTailWaves wavefronts, run by one block in order (a barrier between stages).
+constexpr uint32_t kTailStages = 8, kTailWaves = 8;
+struct SweepTail {
+  SweepArgs st[kTailStages];
+  uint32_t nst;
+};
+
 struct RootArgs {
   const uint64_t* staging;
   const uint32_t *cnt_f, *cnt_s, *off_f, *off_s, *seg_base;

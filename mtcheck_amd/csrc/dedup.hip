@@ -56,6 +56,9 @@ constexpr uint32_t kTilesPerWaveB = MTC_TPW_B;  // stage-0 tiles per wavefront, 
 #ifndef MTC_SWEEP_WPB_B
 #define MTC_SWEEP_WPB_B 1
 #endif
+#ifndef MTC_SWEEP_B_WAVES
+#define MTC_SWEEP_B_WAVES 5  // sweep B's wavefronts per SIMD (register budget)
+#endif
 constexpr uint32_t kSweepWpbA = MTC_SWEEP_WPB_A;
 constexpr uint32_t kSweepWpbB = MTC_SWEEP_WPB_B;
 
@@ -1066,14 +1069,13 @@ __device__ __forceinline__ void sweep_a_tile(const SweepArgs& a, uint64_t tile, 
 // Stage 0: a wavefront owns tiles_per_wave consecutive tiles, reads their
 // first-occurrence flags with one load per lane and walks only the flagged ones.
 template <int HK, int WPB>
-__device__ __forceinline__ void ph_sweep_a(const Launch<SweepArgs>& L, const Where& wh, uint8_t* lds) {
+__device__ __forceinline__ void sweep_a_stage(const SweepArgs& a, uint32_t blk, uint8_t* lds) {
   auto& rec_dg = *reinterpret_cast<Digest(*)[WPB][256]>(lds);
   auto& rec_node = *reinterpret_cast<uint32_t(*)[WPB][256]>(lds + sizeof(Digest) * WPB * 256);
-  const SweepArgs a = L.args[wh.m].global();
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
   const uint64_t ntiles = 1ull << a.d_out;
   const uint32_t tpw = a.stage0 ? a.tiles_per_wave : 1u;
-  const uint64_t t0 = ((uint64_t)wh.blk * WPB + w) * tpw;
+  const uint64_t t0 = ((uint64_t)blk * WPB + w) * tpw;
   if (t0 >= ntiles) return;  // whole wavefront
   const uint64_t t1 = min(t0 + tpw, ntiles);
   if (!a.stage0) {
@@ -1094,16 +1096,52 @@ __device__ __forceinline__ void ph_sweep_a(const Launch<SweepArgs>& L, const Whe
 template <int HK, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_sweep_a(Launch<SweepArgs> L) {
   __shared__ __align__(16) uint8_t lds[(sizeof(Digest) + sizeof(uint32_t)) * WPB * 256];
-  ph_sweep_a<HK, WPB>(L, where(L), lds);
+  const Where wh = where(L);
+  sweep_a_stage<HK, WPB>(L.args[wh.m].global(), wh.blk, lds);
+}
+
+// A member's last stages in one block: each stage's frontier and table inserts are
+// visible to the next stage's wavefronts of the same block after the barrier.  Each
+// launch costs about 5 us of dispatch on its own, more than these stages' work.
+template <int HK>
+__global__ __launch_bounds__(64 * kTailWaves) void k_sweep_a_tail(Launch<SweepTail> L) {
+  __shared__ __align__(16) uint8_t lds[(sizeof(Digest) + sizeof(uint32_t)) * kTailWaves * 256];
+  const Where wh = where(L);
+  const uint32_t nst = L.args[wh.m].nst;
+  for (uint32_t i = 0; i < nst; i++) {
+    sweep_a_stage<HK, kTailWaves>(L.args[wh.m].st[i].global(), 0, lds);
+    __syncthreads();
+  }
 }
 
 // Sweep B: shifted-duplicate forest and region roots (include/reference_impl.hpp
-// :83-106).  Read-only on the table.  Roots are staged per (depth, tile) in
-// ascending node id; a shifted root carries the table slot of its digest.
+// :83-106).  Read-only on the table.  Roots are staged per (depth, group) in
+// ascending node id, a group being the consecutive tiles one wavefront walks (one
+// tile after the first stage); a shifted root carries the table slot of its digest.
+// Where a level's `tot` roots (`ftot` of them first occurrences) go in the staging
+// array: after the roots the group has staged at depth c so far.  The per-level
+// tallies live in the wavefront's LDS (volatile, so every lane reads lane 0's last
+// update; one wavefront's LDS accesses execute in order), and the group's root
+// counts are written from them once its tiles are done.
+__device__ __forceinline__ uint64_t group_stage(const SweepArgs& a, uint32_t c, uint64_t gtile,
+                                                volatile uint32_t* run, uint32_t tot, uint32_t ftot) {
+  const uint32_t i = c - a.d_out - 1;
+  const uint32_t r0 = __builtin_amdgcn_readfirstlane(run[2 * i]);
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  if (lane_id() == 0) {
+    run[2 * i] = r0 + tot;
+    run[2 * i + 1] = run[2 * i + 1] + ftot;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  return heap_id(c, gtile << (c - a.d_out)) + r0;
+}
+
 template <int HK>
 struct SweepB {
   const SweepArgs& a;
   uint64_t* ent;  // this wavefront's LDS: up to four roots per lane and level, entry k of lane l at [64 k + l]
+  volatile uint32_t* run;  // this wavefront's LDS: roots staged so far per level (all, first), for the group
+  uint64_t gtile;          // the group's first tile
   uint32_t ne = 0, nf = 0;
 
   __device__ void emit(uint8_t l, uint32_t slot, uint64_t node) {
@@ -1140,19 +1178,15 @@ struct SweepB {
     if (a.dbg_labels) a.dbg_labels[node] = lo;
   }
 
-  // Stage this level's roots (children at depth c) for tile `tile`.
-  __device__ void flush(uint32_t c, uint64_t tile) {
+  // Stage this level's roots (children at depth c) after the group's earlier ones.
+  __device__ void flush(uint32_t c) {
     if (!__ballot(ne != 0)) return;  // no root at this level (nf is zero too)
     uint32_t tot;
     const uint32_t rank = wave_rank(ne, &tot);
     const uint32_t ftot = wave_total3(nf);
-    uint64_t* dst = a.staging + heap_id(c, tile << (c - a.d_out)) + rank;
     const uint32_t lane = lane_id();
+    uint64_t* dst = a.staging + group_stage(a, c, gtile, run, tot, ftot) + rank;
     for (uint32_t k = 0; k < ne; k++) dst[k] = ent[64 * k + lane];
-    if (lane_id() == 0 && tot) {  // counts are zero-filled before the sweep
-      a.cnt_f[a.seg_base[c] + tile] = ftot;
-      a.cnt_s[a.seg_base[c] + tile] = tot - ftot;
-    }
     ne = nf = 0;
   }
 };
@@ -1199,7 +1233,7 @@ __device__ __forceinline__ Lab4 load_lab4(const SweepArgs& a, uint64_t base, uin
 // One tile of sweep B (the whole wavefront), its labels already loaded.
 template <int HK>
 __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, const Lab4& lb, uint32_t lane,
-                                             uint64_t* ent) {
+                                             uint64_t* ent, volatile uint32_t* run, uint64_t gtile) {
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
   const uint64_t base = tile << R;
   const uint32_t lp0 = 4 * lane;
@@ -1246,7 +1280,7 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
         uint32_t tot;
         const uint32_t rank = wave_rank((uint32_t)__popc(rm), &tot);
         const uint32_t ftot = wave_total3((uint32_t)__popc(fm));
-        uint64_t* dst = a.staging + heap_id(a.d_in, base) + rank;
+        uint64_t* dst = a.staging + group_stage(a, a.d_in, gtile, run, tot, ftot) + rank;
         const uint32_t node0 = (uint32_t)heap_id(a.d_in, base + lp0);
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -1254,10 +1288,6 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
             const uint64_t tag = ((fm >> i) & 1) ? kFirstTag : kLeafTag;
             dst[__popc(rm & ((1u << i) - 1))] = (tag << 32) | (node0 + (uint32_t)i);
           }
-        if (lane == 0) {
-          a.cnt_f[a.seg_base[a.d_in] + tile] = ftot;
-          a.cnt_s[a.seg_base[a.d_in] + tile] = tot - ftot;
-        }
       }
       if (lane == 0) {
         a.out_lab[tile] = L_DONE;
@@ -1273,7 +1303,7 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
   // where a pair needs them (a shifted pair is hashed; a shifted root carries its
   // slot; a leaf's slot is read when its root entry is consumed): holding all four
   // per lane cost the kernel half its wavefront slots in registers.
-  SweepB<HK> s{a, ent};
+  SweepB<HK> s{a, ent, run, gtile};
   uint8_t l1[2] = {L_NONE, L_NONE};
   Digest g1[2];
   uint32_t s1[2] = {0, 0};
@@ -1300,7 +1330,7 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
     const uint32_t sb = (lb2 == L_SHIFT) ? slot_at(2 * k + 1) : 0u;
     s.combine(a.d_in - 1, q, la, da, sa, lb2, db, sb, l1[k], g1[k], s1[k]);
   }
-  s.flush(a.d_in, tile);
+  s.flush(a.d_in);
   uint8_t lc = l1[0];
   Digest gc = g1[0];
   uint32_t sc = s1[0];
@@ -1308,7 +1338,7 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
     lc = L_NONE;
     if (lane < T / 4)
       s.combine(a.d_in - 2, (base >> 2) + lane, l1[0], g1[0], s1[0], l1[1], g1[1], s1[1], lc, gc, sc);
-    s.flush(a.d_in - 1, tile);
+    s.flush(a.d_in - 1);
   }
   for (uint32_t r = 3; r <= R; r++) {
     const int sh = 1 << (r - 3);
@@ -1330,7 +1360,7 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
       gc = go;
       sc = so;
     }
-    s.flush(a.d_in - r + 1, tile);
+    s.flush(a.d_in - r + 1);
   }
   if (lane == 0) {
     a.out_lab[tile] = lc;
@@ -1352,9 +1382,10 @@ __device__ __forceinline__ void sweep_b_tile(const SweepArgs& a, uint64_t tile, 
 
 template <int HK>
 __device__ __forceinline__ void ph_sweep_b_tiles(const SweepArgs& a, uint64_t t0, uint32_t tpw, uint64_t ntiles,
-                                                 uint32_t R, uint32_t T, uint32_t lane, uint64_t* ent) {
+                                                 uint32_t R, uint32_t T, uint32_t lane, uint64_t* ent,
+                                                 volatile uint32_t* run) {
   if (tpw == 1) {
-    sweep_b_tile<HK>(a, t0, load_lab4(a, t0 << R, T, lane), lane, ent);
+    sweep_b_tile<HK>(a, t0, load_lab4(a, t0 << R, T, lane), lane, ent, run, t0);
     return;
   }
   const uint64_t t1 = min(t0 + tpw, ntiles);
@@ -1374,34 +1405,51 @@ __device__ __forceinline__ void ph_sweep_b_tiles(const SweepArgs& a, uint64_t t0
       i = __ffsll((long long)todo) - 1;
       lb = load_lab4(a, (t0 + i) << R, T, lane);
     }
-    sweep_b_tile<HK>(a, t, cur, lane, ent);
+    sweep_b_tile<HK>(a, t, cur, lane, ent, run, t0);
   }
 }
 
-// One wavefront per tile, or at stage 0 tiles_per_wave consecutive tiles: their
-// busy flags are read with one load per lane and only the flagged tiles are
-// walked; every leaf below an unflagged tile is a fixed duplicate, so the tile is
-// fixed and its root counts stay zero.
+// One wavefront per tile, or at stage 0 tiles_per_wave consecutive tiles (one
+// root segment per depth): their busy flags are read with one load per lane and
+// only the flagged tiles are walked; every leaf below an unflagged tile is a fixed
+// duplicate, so the tile is fixed and adds no roots.
 template <int HK, int WPB>
-__device__ __forceinline__ void ph_sweep_b(const Launch<SweepArgs>& L, const Where& wh, uint8_t* lds) {
-  uint64_t* ent = reinterpret_cast<uint64_t*>(lds) + 256 * (threadIdx.x >> 6);  // this wavefront's root entries
-  const SweepArgs a = L.args[wh.m].global();
+__device__ __forceinline__ void sweep_b_stage(const SweepArgs& a, uint32_t blk, uint64_t* ent_all,
+                                              uint32_t* run_all) {
+  const uint32_t w = threadIdx.x >> 6;
+  uint64_t* ent = ent_all + 256 * w;  // this wavefront's root entries
+  volatile uint32_t* run = run_all + 2 * kFuse * w;
   const uint32_t lane = lane_id();
   const uint64_t ntiles = 1ull << a.d_out;
   const uint32_t R = a.d_in - a.d_out, T = 1u << R;
   const uint32_t tpw = a.stage0 ? a.tiles_per_wave : 1u;
-  const uint64_t t0 = ((uint64_t)wh.blk * WPB + (threadIdx.x >> 6)) * tpw;
+  const uint64_t grp = (uint64_t)blk * WPB + w, t0 = grp * tpw;
   if (t0 >= ntiles) return;  // whole wavefront
-  ph_sweep_b_tiles<HK>(a, t0, tpw, ntiles, R, T, lane, ent);
+  if (lane < 2 * R) run[lane] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  ph_sweep_b_tiles<HK>(a, t0, tpw, ntiles, R, T, lane, ent, run);
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  // The group's root counts per depth (zero-filled before the sweep).
+  if (lane < R) {
+    const uint32_t tot = run[2 * lane], ftot = run[2 * lane + 1];
+    if (tot) {
+      const uint32_t seg = a.seg_base[a.d_out + 1 + lane] + (uint32_t)grp;
+      a.cnt_f[seg] = ftot;
+      a.cnt_s[seg] = tot - ftot;
+    }
+  }
   if (a.d_out == 0 && lane == 0) a.ctl->t_forest = wall_clock64();  // the member's last sweep stage
 }
 
-// Six wavefronts per SIMD (at most 80 VGPRs; four spill): 146 against 154 us at
-// five for the first stage at 8 GiB, 128-byte chunks, 2% change.
+// Five wavefronts per SIMD (96 VGPRs, two spill): sweep B took 150 us per
+// checkpoint against 169 us at six (80 VGPRs, eighteen spill) at 8 GiB, 128-byte
+// chunks, 2% change.
 template <int HK, int WPB>
-__global__ __launch_bounds__(64 * WPB, 6) void k_sweep_b(Launch<SweepArgs> L) {
+__global__ __launch_bounds__(64 * WPB, MTC_SWEEP_B_WAVES) void k_sweep_b(Launch<SweepArgs> L) {
   __shared__ __align__(16) uint64_t ent[WPB * 256];
-  ph_sweep_b<HK, WPB>(L, where(L), reinterpret_cast<uint8_t*>(ent));
+  __shared__ uint32_t run[WPB * 2 * kFuse];
+  const Where wh = where(L);
+  sweep_b_stage<HK, WPB>(L.args[wh.m].global(), wh.blk, ent, run);
 }
 
 // A single chunk: the leaf is the root.
@@ -1419,36 +1467,83 @@ __device__ __forceinline__ void ph_single_leaf(const Launch<SweepArgs>& L, const
 __global__ void k_single_leaf(Launch<SweepArgs> L) { ph_single_leaf(L, where(L)); }
 
 // ---------------------------------------------------------------------------
-// Root compaction: one thread per (depth, block) segment, in segment order.
+// Root compaction, in segment order (depth, then group).  A wavefront takes 64
+// segments, one per lane, and copies their roots 64 at a time: lane e of a round
+// finds its segment by bisection over the exclusive prefix of the root counts, and
+// its rank among the segment's first-occurrence (or shifted) roots from a ballot.
 // ---------------------------------------------------------------------------
 
-// The per-depth segment bases (at most 33) are staged in LDS: each thread finds
-// its depth with a walk over them, which as dependent global loads cost most of
-// the kernel.
+// Lanes lo..hi-1 (0 <= lo, hi <= 64).
+__device__ __forceinline__ uint64_t lanes_between(uint32_t lo, uint32_t hi) {
+  if (hi <= lo) return 0;
+  const uint64_t upto = hi >= 64 ? ~0ull : ((1ull << hi) - 1);
+  return upto & ~((1ull << lo) - 1);
+}
+
+// The per-depth segment bases and geometry (at most 2 x 33 + 1 words) are staged in
+// LDS: each lane finds its depth with a walk over them.
 __device__ __forceinline__ void ph_scatter_roots(const Launch<RootArgs>& L, const Where& w, uint8_t* lds) {
   uint32_t* s_base = reinterpret_cast<uint32_t*>(lds);
   const RootArgs a = L.args[w.m].global();
-  if (threadIdx.x <= a.ndepth) s_base[threadIdx.x] = a.seg_base[threadIdx.x];
+  const uint32_t* s_geo = s_base + a.ndepth + 1;
+  if (threadIdx.x <= 2 * a.ndepth) s_base[threadIdx.x] = a.seg_base[threadIdx.x];
   __syncthreads();
+  const uint32_t lane = lane_id();
   const uint32_t seg = w.blk * kThreads + threadIdx.x;
-  if (seg >= a.nseg) return;
-  const uint32_t nf = a.cnt_f[seg], ns = a.cnt_s[seg];
-  if (nf + ns == 0) return;
-  uint32_t c = 0;
-  while (c + 1 < a.ndepth && s_base[c + 1] <= seg) c++;
-  const uint32_t nb = s_base[c + 1] - s_base[c];  // blocks of the emitting stage (a power of two)
-  const uint32_t span = c - (31u - (uint32_t)__clz((int)nb));
-  const uint64_t sb = ((1ull << c) - 1) + ((uint64_t)(seg - a.seg_base[c]) << span);
-  uint32_t of = a.off_f[seg], os = a.off_s[seg];
-  for (uint32_t j = 0; j < nf + ns; j++) {
-    const uint64_t e = a.staging[sb + j];
-    if ((e >> 32) == kFirstTag) a.first_roots[of++] = (uint32_t)e;
-    else a.shift_tmp[os++] = e;
+  uint32_t nf = 0, ns = 0;
+  if (seg < a.nseg) {
+    nf = a.cnt_f[seg];
+    ns = a.cnt_s[seg];
+  }
+  const uint32_t nr = nf + ns;
+  if (!__ballot(nr != 0)) return;
+  uint64_t sb = 0;
+  uint32_t of = 0, os = 0;
+  if (nr) {
+    uint32_t c = 0;
+    while (c + 1 < a.ndepth && s_base[c + 1] <= seg) c++;
+    const uint32_t geo = s_geo[c];
+    sb = ((1ull << c) - 1) + (((uint64_t)(seg - s_base[c]) * (geo & 0xFFFFu)) << (geo >> 16));
+    of = a.off_f[seg];
+    os = a.off_s[seg];
+  }
+  uint32_t x = nr;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  const uint32_t total = __shfl(x, 63), excl = x - nr;
+  const uint32_t sb_lo = (uint32_t)sb, sb_hi = (uint32_t)(sb >> 32);
+  for (uint32_t b = 0; b < total; b += 64) {
+    const uint32_t e = b + lane;
+    uint32_t j = 0;  // the last segment whose roots start at or before e
+#pragma unroll
+    for (uint32_t step = 32; step; step >>= 1)
+      if (__shfl(excl, (int)(j + step)) <= e) j += step;
+    const uint32_t ej = __shfl(excl, (int)j);
+    const uint64_t sbj = ((uint64_t)(uint32_t)__shfl(sb_hi, (int)j) << 32) | (uint32_t)__shfl(sb_lo, (int)j);
+    const uint32_t ofj = __shfl(of, (int)j), osj = __shfl(os, (int)j);
+    const bool valid = e < total;
+    const uint64_t ent = valid ? a.staging[sbj + (e - ej)] : 0ull;
+    const bool isf = valid && (ent >> 32) == kFirstTag;
+    const uint64_t bf = __ballot(isf), bv = __ballot(valid);
+    const uint64_t before = lanes_between(ej > b ? ej - b : 0u, lane);  // this segment's lanes below
+    if (valid) {
+      if (isf) a.first_roots[ofj + __popcll(bf & before)] = (uint32_t)ent;
+      else a.shift_tmp[osj + __popcll(bv & ~bf & before)] = ent;
+    }
+    // Each segment's lane advances its offsets past the roots this round placed.
+    const uint32_t lo = excl > b ? min(excl - b, 64u) : 0u;
+    const uint32_t hi = excl + nr > b ? min(excl + nr - b, 64u) : 0u;
+    const uint64_t mine = lanes_between(lo, hi);
+    of += __popcll(bf & mine);
+    os += __popcll(bv & ~bf & mine);
   }
 }
 
 __global__ __launch_bounds__(kThreads) void k_scatter_roots(Launch<RootArgs> L) {
-  __shared__ uint32_t s_base[64];
+  __shared__ uint32_t s_base[128];
   ph_scatter_roots(L, where(L), reinterpret_cast<uint8_t*>(s_base));
 }
 
@@ -1957,24 +2052,42 @@ static void launch_leaf1(int cb, Launch<LeafArgs> L, uint32_t grid, hipStream_t 
   }
 }
 
-// Plan sweep stages and root segments for n chunks (fuse: levels per stage).
-static void plan_tree(mtc_dedup* h) {
+// Plan sweep stages and root segments for n chunks (fuse: levels per stage).  The
+// roots of depth c are counted per segment: a group of `group` consecutive tiles of
+// the first stage (the tiles one wavefront of sweep B walks), one tile in later
+// stages.  Per depth: the first segment (seg_base_h) and the segment geometry
+// (seg_geo_h: tiles per segment | log2 positions per tile at depth c << 16).
+static void plan_tree(mtc_dedup* h, uint32_t fuse_req, uint32_t group) {
   const Geo g = make_geo(h->n);
-  const uint32_t fuse = std::max<uint32_t>(1, std::min<uint32_t>(kFuse, h->tune.fuse));
+  const uint32_t fuse = std::max<uint32_t>(1, std::min<uint32_t>(kFuse, fuse_req));
+  h->plan_fuse = fuse;
+  h->plan_group = group;
   h->stages.clear();
-  std::vector<uint32_t> nb(g.D + 1, 1);  // blocks of the stage emitting roots at depth c
+  std::vector<uint32_t> nb(g.D + 1, 1);  // segments of depth c
+  h->seg_geo_h.assign(g.D + 1, 1u);
   if (h->n > 1) {
     uint32_t d = g.D;
     while (d > 0) {
       const uint32_t dout = d > fuse ? d - fuse : 0;
+      const uint32_t gs = h->stages.empty() ? group : 1u;
       h->stages.push_back(Stage{d, dout});
-      for (uint32_t c = dout + 1; c <= d; c++) nb[c] = 1u << dout;
+      for (uint32_t c = dout + 1; c <= d; c++) {
+        nb[c] = (uint32_t)(((1ull << dout) + gs - 1) / gs);
+        h->seg_geo_h[c] = gs | ((c - dout) << 16);
+      }
       d = dout;
     }
   }
   h->seg_base_h.assign(g.D + 2, 0);
   for (uint32_t c = 0; c <= g.D; c++) h->seg_base_h[c + 1] = h->seg_base_h[c] + nb[c];
   h->plan_dirty = true;
+}
+
+// Tiles per wavefront of sweep B's first stage, which is also the segment group:
+// every tile, one per wavefront, when the debug labels are recorded.
+static uint32_t sweep_b_group(const mtc_dedup* h) {
+  if (h->debug) return 1;
+  return h->tune.sweep_tiles_b ? h->tune.sweep_tiles_b : kTilesPerWaveB;
 }
 
 static uint32_t sort_blocks(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(kSortBlocks, (n + 16 * kThreads - 1) / (16 * kThreads))); }
@@ -1997,10 +2110,11 @@ static void ensure_buffers(mtc_dedup* h, hipStream_t st) {
   h->seg_cnt_s.ensure(nseg);
   h->seg_off_f.ensure(nseg + 1);
   h->seg_off_s.ensure(nseg + 1);
-  h->seg_base.ensure(h->seg_base_h.size());
+  h->seg_base.ensure(h->seg_base_h.size() + h->seg_geo_h.size());
   if (h->plan_dirty) {
-    MTC_HIP(hipMemcpyAsync(h->seg_base.p, h->seg_base_h.data(), h->seg_base_h.size() * sizeof(uint32_t),
-                           hipMemcpyHostToDevice, st));
+    std::vector<uint32_t> up(h->seg_base_h);
+    up.insert(up.end(), h->seg_geo_h.begin(), h->seg_geo_h.end());
+    MTC_HIP(hipMemcpyAsync(h->seg_base.p, up.data(), up.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     MTC_HIP(hipStreamSynchronize(st));  // the host vector is pageable
     h->plan_dirty = false;
   }
@@ -2235,7 +2349,8 @@ struct Member {
 };
 
 // Recorded launches of a step: run eagerly or captured into a graph.  A tag names
-// each launch's kernel (sweep B: 44 and 45, one chunk: 30, leaf kernels: 100 up).
+// each launch's kernel (sweep B: 44 and 45, sweep A's tails: 48 and 49, one chunk: 30, leaf
+// kernels: 100 up).
 struct Ops {
   std::vector<std::function<void(uint8_t*, hipStream_t)>> fn;
   std::vector<uint64_t> sig;
@@ -2278,7 +2393,9 @@ static void prepare_member(Member& M, hipStream_t st) {
     h->has_state = false;
     h->n = (uint32_t)n64;
     h->data_len = M.len;
-    plan_tree(h);
+    plan_tree(h, h->tune.fuse, sweep_b_group(h));
+  } else if (h->plan_group != sweep_b_group(h)) {
+    plan_tree(h, h->plan_fuse, sweep_b_group(h));  // same stages, new root segments
   }
   ensure_buffers(h, st);
   // Until this step completes, the labels are not known to be reset.  A baseline
@@ -2412,7 +2529,45 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       if (any)
         ops.add<SweepArgs>(30, img.add(base, sb), [](Launch<SweepArgs> L, uint32_t g, hipStream_t st) { k_single_leaf<<<g, 64, 0, st>>>(L); });
     }
+    // Stage s of member i in this pass (its frontiers alternate between two halves).
+    auto stage_args = [&](size_t i, size_t s, int pass) {
+      mtc_dedup* h = ms[i].h;
+      SweepArgs a = base[i];
+      a.d_in = h->stages[s].d_in;
+      a.d_out = h->stages[s].d_out;
+      a.stage0 = (s == 0);
+      const uint64_t half_cap = h->stages.size() > 1 ? (1ull << h->stages[0].d_out) : 1;
+      const uint64_t in_off = (s % 2 == 1) ? 0 : half_cap, out_off = (s % 2 == 0) ? 0 : half_cap;
+      a.in_lab = h->fr_lab.p + in_off;
+      a.in_dg = h->fr_dg.p + in_off;
+      a.in_slot = h->fr_slot.p + in_off;
+      a.out_lab = h->fr_lab.p + out_off;
+      a.out_dg = h->fr_dg.p + out_off;
+      a.out_slot = h->fr_slot.p + out_off;
+      // stage 0: several tiles per wavefront, their flags read at once; sweep B's
+      // tiles per wavefront are the root segment group the plan was made for
+      const uint32_t ta = h->tune.sweep_tiles_a ? h->tune.sweep_tiles_a : kTilesPerWaveA;
+      a.tiles_per_wave = !a.stage0 ? 1u : (pass == 0 ? ta : h->plan_group);
+      return a;
+    };
+    auto stage_waves = [](const SweepArgs& a) {
+      return ((1ull << a.d_out) + a.tiles_per_wave - 1) / a.tiles_per_wave;
+    };
     for (int pass = 0; pass < 2; pass++) {
+      // Sweep A: a member's stages from tail[i] on each fit one block of kTailWaves
+      // wavefronts, so they run in one launch after its other stages.  (Sweep B's
+      // last stages in one kernel of their own took 13.0 us against 10.7 us for two
+      // launches of the kernel that had just run stage 0 at 8 GiB: its code has to
+      // be fetched again, and the walk of one of its tiles is mostly instructions.)
+      std::vector<size_t> tail(B);
+      for (size_t i = 0; i < B; i++) {
+        const size_t ns = ms[i].n > 1 ? ms[i].h->stages.size() : 0;
+        tail[i] = ns;
+        while (pass == 0 && tail[i] > 0 && ns - (tail[i] - 1) <= kTailStages &&
+               stage_waves(stage_args(i, tail[i] - 1, pass)) <= kTailWaves)
+          tail[i]--;
+        if (ns - tail[i] < 2) tail[i] = ns;  // a single stage keeps its own launch
+      }
       for (size_t s = 0; s < max_stages; s++) {
         struct Group {
           std::vector<SweepArgs> args;
@@ -2424,27 +2579,10 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
           Group& slot = byhk[h->hash_kind];
           SweepArgs a = base[i];
           uint32_t blocks = 0;
-          if (s < h->stages.size()) {
-            a.d_in = h->stages[s].d_in;
-            a.d_out = h->stages[s].d_out;
-            a.stage0 = (s == 0);
-            const uint64_t half_cap = h->stages.size() > 1 ? (1ull << h->stages[0].d_out) : 1;
-            const uint64_t in_off = (s % 2 == 1) ? 0 : half_cap, out_off = (s % 2 == 0) ? 0 : half_cap;
-            a.in_lab = h->fr_lab.p + in_off;
-            a.in_dg = h->fr_dg.p + in_off;
-            a.in_slot = h->fr_slot.p + in_off;
-            a.out_lab = h->fr_lab.p + out_off;
-            a.out_dg = h->fr_dg.p + out_off;
-            a.out_slot = h->fr_slot.p + out_off;
-            // stage 0: several tiles per wavefront (their flags are read at once; sweep
-            // B walks every tile when it records the debug labels); four wavefronts
-            // per block
-            const uint32_t ta = h->tune.sweep_tiles_a ? h->tune.sweep_tiles_a : kTilesPerWaveA;
-            const uint32_t tb = h->tune.sweep_tiles_b ? h->tune.sweep_tiles_b : kTilesPerWaveB;
-            a.tiles_per_wave = !a.stage0 ? 1u : (pass == 0 ? ta : (a.dbg_labels ? 1u : tb));
-            const uint64_t waves = ((1ull << a.d_out) + a.tiles_per_wave - 1) / a.tiles_per_wave;
+          if (s < tail[i]) {
+            a = stage_args(i, s, pass);
             const uint32_t wpb = pass == 0 ? kSweepWpbA : kSweepWpbB;
-            blocks = (uint32_t)((waves + wpb - 1) / wpb);
+            blocks = (uint32_t)((stage_waves(a) + wpb - 1) / wpb);
           }
           slot.args.push_back(a);
           slot.blocks.push_back(blocks);
@@ -2465,9 +2603,27 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
             });
         }
       }
+      if (pass == 1) continue;
+      std::map<int, std::pair<std::vector<SweepTail>, std::vector<uint32_t>>> tails;
+      for (size_t i = 0; i < B; i++) {
+        auto& t = tails[ms[i].h->hash_kind];
+        SweepTail st{};
+        const size_t ns = ms[i].n > 1 ? ms[i].h->stages.size() : 0;
+        for (size_t s = tail[i]; s < ns; s++) st.st[st.nst++] = stage_args(i, s, pass);
+        t.first.push_back(st);
+        t.second.push_back(st.nst ? 1u : 0u);
+      }
+      for (auto& kv : tails) {
+        const int hk = kv.first;
+        const LaunchRec r = img.add(kv.second.first, kv.second.second);
+        ops.add<SweepTail>(48 + (uint32_t)hk, r, [hk](Launch<SweepTail> L, uint32_t g, hipStream_t st) {
+          if (hk == HASH_MD5) k_sweep_a_tail<HASH_MD5><<<g, 64 * kTailWaves, 0, st>>>(L);
+          else k_sweep_a_tail<HASH_MURMUR3><<<g, 64 * kTailWaves, 0, st>>>(L);
+        });
+      }
     }
   }
-  // 5. root totals and compaction (segments are (depth, block) in node-id order)
+  // 5. root totals and compaction (segments are (depth, group) in node-id order)
   {
     std::vector<ScanSeg> ss;
     for (auto& M : ms) {

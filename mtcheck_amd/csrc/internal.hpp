@@ -216,6 +216,8 @@ struct mtc_dedup {
   mtc::Workspace ws;
   std::vector<mtc::Stage> stages;
   std::vector<uint32_t> seg_base_h;   // first segment of each depth
+  std::vector<uint32_t> seg_geo_h;    // per depth: tiles per segment | log2 positions per tile << 16
+  uint32_t plan_fuse = 0, plan_group = 0;  // the levels per stage and segment group planned for
   bool plan_dirty = false;            // seg_base_h not yet on the device
   mtc::StepCtx step;                  // launch state of single checkpoints
   mtc::Tune tune;

@@ -158,6 +158,37 @@ void check_members(const std::string& name, const LaunchRaw& L, uint32_t grid, F
 
 const uint64_t kCtl = sizeof(Ctl);
 
+void check_sweep(const std::string& w, const SweepArgs& a) {
+  const uint64_t n = a.g.n;
+  need(w, "leaf_dg", a.leaf_dg, 16 * n);
+  need(w, "labels", a.labels, n);
+  need(w, "leaf_slot", a.leaf_slot, 4 * n);
+  const uint64_t tiles_out = 1ull << a.d_out;
+  need(w, "out_lab", a.out_lab, tiles_out);
+  need(w, "out_dg", a.out_dg, 16 * tiles_out);
+  need(w, "out_slot", a.out_slot, 4 * tiles_out);
+  if (!a.stage0) {
+    const uint64_t in = 1ull << a.d_in;
+    need(w, "in_lab", a.in_lab, in);
+    need(w, "in_dg", a.in_dg, 16 * in);
+    need(w, "in_slot", a.in_slot, 4 * in);
+  }
+  need(w, "tab", a.tab, 32ull * (a.mask + 2));
+  need(w, "ctl", a.ctl, kCtl);
+  need(w, "ctr", a.ctr, 8);
+  need(w, "staging", a.staging, 8 * (2 * n - 1));
+  need(w, "cnt_f", a.cnt_f, 4);
+  need(w, "cnt_s", a.cnt_s, 4);
+  need(w, "seg_base", a.seg_base, 4ull * (2 * a.g.D + 3));  // bases, then geometry
+  // a stage-0 wavefront reads its tiles' flags with one lane per tile
+  if (a.stage0 && (a.tiles_per_wave == 0 || a.tiles_per_wave > 64)) violation(w + ": tiles per wavefront out of 1..64");
+  maybe(w, "dbg_tree", a.dbg_tree, 16 * (2 * n - 1));
+  maybe(w, "dbg_labels", a.dbg_labels, 2 * n - 1);
+  need(w, "tile_first", a.tile_first, 1);
+  need(w, "tile_busy", a.tile_busy, 1);
+  if (a.d_in <= a.d_out || a.d_in - a.d_out > 8) violation(w + ": stage depths out of range");
+}
+
 void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
   LaunchRaw L;
   std::memcpy(&L, arg0, sizeof(L));
@@ -221,37 +252,20 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       need(w, "cnt_f", a.cnt_f, 4);
       need(w, "cnt_s", a.cnt_s, 4);
     });
-  } else if (has("9SweepArgs")) {
-    check_members<SweepArgs>(name, L, grid, [](const std::string& w, const SweepArgs& a, uint32_t) {
-      const uint64_t n = a.g.n;
-      need(w, "leaf_dg", a.leaf_dg, 16 * n);
-      need(w, "labels", a.labels, n);
-      need(w, "leaf_slot", a.leaf_slot, 4 * n);
-      const uint64_t tiles_out = 1ull << a.d_out;
-      need(w, "out_lab", a.out_lab, tiles_out);
-      need(w, "out_dg", a.out_dg, 16 * tiles_out);
-      need(w, "out_slot", a.out_slot, 4 * tiles_out);
-      if (!a.stage0) {
-        const uint64_t in = 1ull << a.d_in;
-        need(w, "in_lab", a.in_lab, in);
-        need(w, "in_dg", a.in_dg, 16 * in);
-        need(w, "in_slot", a.in_slot, 4 * in);
+  } else if (has("9SweepTail")) {
+    check_members<SweepTail>(name, L, grid, [](const std::string& w, const SweepTail& t, uint32_t nblk) {
+      if (nblk != 1) violation(w + ": a sweep tail runs in one block");
+      if (t.nst < 2 || t.nst > kTailStages) violation(w + ": sweep tail stage count out of 2..kTailStages");
+      for (uint32_t i = 0; i < t.nst && i < kTailStages; i++) {
+        const SweepArgs& a = t.st[i];
+        check_sweep(w + "[stage " + std::to_string(i) + "]", a);
+        const uint32_t tpw = a.stage0 ? a.tiles_per_wave : 1u;
+        if (tpw && ((1ull << a.d_out) + tpw - 1) / tpw > kTailWaves) violation(w + ": a tail stage needs more wavefronts than its block has");
+        if (i && a.d_in != t.st[i - 1].d_out) violation(w + ": tail stages do not follow each other");
       }
-      need(w, "tab", a.tab, 32ull * (a.mask + 2));
-      need(w, "ctl", a.ctl, kCtl);
-      need(w, "ctr", a.ctr, 8);
-      need(w, "staging", a.staging, 8 * (2 * n - 1));
-      need(w, "cnt_f", a.cnt_f, 4);
-      need(w, "cnt_s", a.cnt_s, 4);
-      need(w, "seg_base", a.seg_base, 4ull * (a.g.D + 2));
-      // a stage-0 wavefront reads its tiles' flags with one lane per tile
-      if (a.stage0 && (a.tiles_per_wave == 0 || a.tiles_per_wave > 64)) violation(w + ": tiles per wavefront out of 1..64");
-      maybe(w, "dbg_tree", a.dbg_tree, 16 * (2 * n - 1));
-      maybe(w, "dbg_labels", a.dbg_labels, 2 * n - 1);
-      need(w, "tile_first", a.tile_first, 1);
-      need(w, "tile_busy", a.tile_busy, 1);
-      if (a.d_in <= a.d_out || a.d_in - a.d_out > 8) violation(w + ": stage depths out of range");
     });
+  } else if (has("9SweepArgs")) {
+    check_members<SweepArgs>(name, L, grid, [](const std::string& w, const SweepArgs& a, uint32_t) { check_sweep(w, a); });
   } else if (has("8RootArgs")) {
     check_members<RootArgs>(name, L, grid, [](const std::string& w, const RootArgs& a, uint32_t) {
       need(w, "staging", a.staging, 8);
@@ -259,7 +273,7 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       need(w, "cnt_s", a.cnt_s, 4ull * a.nseg);
       need(w, "off_f", a.off_f, 4ull * a.nseg);
       need(w, "off_s", a.off_s, 4ull * a.nseg);
-      need(w, "seg_base", a.seg_base, 4ull * (a.ndepth + 1));
+      need(w, "seg_base", a.seg_base, 4ull * (2 * a.ndepth + 1));  // bases, then geometry
       need(w, "first_roots", a.first_roots, 4);
       need(w, "shift_tmp", a.shift_tmp, 8);
     });

@@ -129,7 +129,10 @@ def test_tree_steps_launch_in_bounds(fake, C, size, hk, tuning):
             L.mtc_set_debug(h, 1)
         rc, hdr = checkpoint(L, h, data, size, baseline=(k == 0))
         assert rc == OK, L.mtc_last_error(h)
-    assert any("assemble" in n for n in launches(L))
+    names = launches(L)
+    assert any("assemble" in n for n in names)
+    if size // C >= 300:  # two sweep stages or more: the last small ones run as one block
+        assert any("9SweepTail" in n for n in names)
     assert violations(L) == []
     L.mtc_destroy(h)
 
