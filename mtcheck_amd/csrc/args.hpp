@@ -82,9 +82,20 @@ struct LeafArgs {
   }
 };
 
+// What a direct leaf launch does before its tile: the arena copy (16-byte words,
+// from pinned host memory) and the step's fills.
+constexpr uint32_t kPrologueFills = 12;
+struct Prologue {
+  const uint4* src;
+  uint4* dst;
+  uint64_t n16;
+  FillSeg fill[kPrologueFills];
+  uint32_t nfill;
+};
+
 struct CompactArgs {
   const uint32_t* list;
-  const uint32_t* blk_cnt;
+  uint32_t* blk_cnt;  // read, then cleared for the next step's leaf kernel
   const uint32_t* blk_off;
   uint64_t tiles_per_block;
   uint32_t* dense;

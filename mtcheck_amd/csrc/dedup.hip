@@ -121,6 +121,7 @@ __device__ void finish_member(const unsigned long long* ctr, const uint32_t* cha
   if (threadIdx.x == 0) {
     ctl->claims = s[CTR_CLAIMS];
     ctl->changed = *changed;
+    ctl->t_end = wall_clock64();
   }
   __syncthreads();
   const uint32_t* src = reinterpret_cast<const uint32_t*>(ctl);
@@ -552,15 +553,12 @@ __device__ __forceinline__ uint32_t lean_col(uint32_t row, uint32_t col) {
 }
 
 template <int HK, int CB>
-__global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
+__device__ __forceinline__ void leaf1_tile(const LeafArgs& a, uint32_t blk, uint8_t* lds) {
   static_assert(CB >= 16 && (CB & (CB - 1)) == 0, "power-of-two chunk size");
   constexpr int SEG = CB < 128 ? CB : 128;  // bytes of each chunk per pass
   constexpr int PASSES = CB / SEG, LOADS = SEG / 16;
-  __shared__ __align__(16) uint8_t lds[kThreads * SEG];
-  const Where wh = where(L);
-  const LeafArgs a = L.args[wh.m].global();
-  if (a.stamp && wh.blk == 0 && threadIdx.x == 0) *a.stamp = wall_clock64();
-  const uint64_t t = wh.blk;
+  if (a.stamp && blk == 0 && threadIdx.x == 0) *a.stamp = wall_clock64();
+  const uint64_t t = blk;
   const uint64_t full = a.len / CB;
   const uint32_t tid = threadIdx.x;
   const Digest pv = leaf_prev(a, t);
@@ -631,13 +629,44 @@ __global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
   const uint32_t lane = lane_id();
   const uint64_t m = __ballot(ins);
   uint32_t base = 0;
-  if (lane == 0 && m) base = atomicAdd(a.blk_cnt + wh.blk, (uint32_t)__popcll(m));
+  if (lane == 0 && m) base = atomicAdd(a.blk_cnt + blk, (uint32_t)__popcll(m));
   base = __shfl(base, 0);
   if (ins) {
     const uint64_t k = t * kThreads + base + __popcll(m & ((1ull << lane) - 1));
     a.list[k] = (uint32_t)c;
     if (a.list_dg) a.list_dg[k] = d;
   }
+}
+
+template <int HK, int CB>
+__global__ __launch_bounds__(kThreads, 5) void k_leaf1(Launch<LeafArgs> L) {
+  __shared__ __align__(16) uint8_t lds[kThreads * (CB < 128 ? CB : 128)];
+  const Where wh = where(L);
+  leaf1_tile<HK, CB>(L.args[wh.m].global(), wh.blk, lds);
+}
+
+// The same kernel for a step of one member, started before anything else of the
+// step: its arguments come by value, and the threads of its first blocks copy the
+// step's argument arena from pinned host memory to the device (what the later
+// kernels read) and clear the step's counters before their tiles.  That saves the
+// upload copy, the clearing kernel and the idle time in front of each.  The block
+// counts k_leaf1 adds to are zero already: k_compact clears them after reading.
+// (Spread over every block, the same work made the kernel 3% slower: each block
+// read the fill list before its loads.)
+constexpr uint32_t kPrologueBlocks = 64;
+template <int HK, int CB>
+__global__ __launch_bounds__(kThreads, 5) void k_leaf1_direct(LeafArgs a, Prologue p) {
+  __shared__ __align__(16) uint8_t lds[kThreads * (CB < 128 ? CB : 128)];
+  if (blockIdx.x < kPrologueBlocks) {
+    const uint64_t gt = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    const uint64_t gs = (uint64_t)min(gridDim.x, kPrologueBlocks) * kThreads;
+    for (uint64_t i = gt; i < p.n16; i += gs) p.dst[i] = p.src[i];
+    for (uint32_t f = 0; f < p.nfill; f++) {
+      const FillSeg s = p.fill[f].global();
+      for (uint64_t i = gt; i < s.count; i += gs) s.p[i] = s.v;
+    }
+  }
+  leaf1_tile<HK, CB>(a.global(), blockIdx.x, lds);
 }
 
 // Concatenate the per-block regions into the dense list of chunks to insert.
@@ -660,6 +689,8 @@ __device__ __forceinline__ void ph_compact(const Launch<CompactArgs>& L, const W
     s_pre[nb] = run;
   }
   __syncthreads();
+  // The last reader of the counts clears them (the leaf scan read them before).
+  if (threadIdx.x < nb) a.blk_cnt[b0 + threadIdx.x] = 0;
   const uint32_t total = s_pre[nb];
   const uint32_t off = a.blk_off[b0];
   uint32_t* dst = a.dense + off;
@@ -2052,6 +2083,21 @@ static void launch_leaf1(int cb, Launch<LeafArgs> L, uint32_t grid, hipStream_t 
   }
 }
 
+template <int HK>
+static void launch_leaf1_direct(int cb, const LeafArgs& a, const Prologue& p, uint32_t grid, hipStream_t st) {
+  switch (cb) {
+    case 16: k_leaf1_direct<HK, 16><<<grid, kThreads, 0, st>>>(a, p); break;
+    case 32: k_leaf1_direct<HK, 32><<<grid, kThreads, 0, st>>>(a, p); break;
+    case 64: k_leaf1_direct<HK, 64><<<grid, kThreads, 0, st>>>(a, p); break;
+    case 128: k_leaf1_direct<HK, 128><<<grid, kThreads, 0, st>>>(a, p); break;
+    case 256: k_leaf1_direct<HK, 256><<<grid, kThreads, 0, st>>>(a, p); break;
+    case 512: k_leaf1_direct<HK, 512><<<grid, kThreads, 0, st>>>(a, p); break;
+    case 1024: k_leaf1_direct<HK, 1024><<<grid, kThreads, 0, st>>>(a, p); break;
+    case 2048: k_leaf1_direct<HK, 2048><<<grid, kThreads, 0, st>>>(a, p); break;
+    default: k_leaf1_direct<HK, 4096><<<grid, kThreads, 0, st>>>(a, p); break;
+  }
+}
+
 // Plan sweep stages and root segments for n chunks (fuse: levels per stage).  The
 // roots of depth c are counted per segment: a group of `group` consecutive tiles of
 // the first stage (the tiles one wavefront of sweep B walks), one tile in later
@@ -2340,6 +2386,7 @@ struct Member {
   int cb = 0;
   bool split = false;  // the table bound is checked after the leaf kernel
   bool all_labels = true;  // the leaf kernel labels every chunk, not only the listed ones
+  uint64_t blk_zero = 0;   // leading block counts known to be zero when the step starts
   Geo g{};
   uint32_t* ws_blk = nullptr;  // scan partial sums of this member
   uint32_t* ws_f = nullptr;
@@ -2416,6 +2463,10 @@ static void prepare_member(Member& M, hipStream_t st) {
   const uint64_t ntiles = (M.n + kThreads - 1) / kThreads;
   M.tpb = std::max<uint64_t>(1, h->tune.leaf_tiles_per_block);
   M.nblk_leaf = (uint32_t)((ntiles + M.tpb - 1) / M.tpb);
+  // Block counts that k_compact cleared at the end of the last completed step (the
+  // same buffer); unknown again until this step completes.
+  M.blk_zero = h->blk_zero_p == h->blk_cnt.p ? h->blk_zero : 0;
+  h->blk_zero = 0;
   M.nseg = h->seg_base_h.back();
   M.nbins = M.cur - h->baseline_id + 1;
   uint32_t bits = 0;
@@ -2770,8 +2821,24 @@ static void build_basic_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) 
 // flags, histograms, debug views) and the leaf kernels, plus the scan of the leaf
 // blocks' counts when `leaf_scan` (steps that read the count of chunks to insert
 // before part 2).  Part 2: the rest.  Part 0: both.
-static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int phase, bool leaf_scan) {
+// Launch of a step's single member's leaf kernel with its arguments by value (see
+// k_leaf1_direct), when build_ops leaves it out of the recorded launches.
+struct DirectLeaf {
+  bool ok = false;
+  int hk = 0, cb = 0;
+  uint32_t grid = 0;
+  LeafArgs la{};
+  Prologue pro{};
+};
+
+static bool lean_leaf(const Member& M) { return M.cb >= 16 && M.tpb == 1 && M.h->approach != MTC_APPROACH_BASIC; }
+
+static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int phase, bool leaf_scan,
+                      DirectLeaf* direct = nullptr) {
   const size_t B = ms.size();
+  // One member whose leaf kernel adds to block counts known to be zero: its leaf
+  // kernel goes first, by value, and does the fills itself.
+  const bool dl = direct && phase == 0 && B == 1 && lean_leaf(ms[0]) && ms[0].blk_zero >= ms[0].nblk_leaf;
   if (phase != 2) {
     {
       std::vector<FillSeg> fs;
@@ -2783,25 +2850,35 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
       };
       for (auto& M : ms) {
         mtc_dedup* h = M.h;
-        fill(h->ctl_d, sizeof(Ctl) / 4, 0);
+        if (dl) {  // the leaf kernel's first block stamps t_begin meanwhile
+          constexpr uint64_t tb = offsetof(Ctl, t_begin) / 4, words = sizeof(Ctl) / 4;
+          fill(h->ctl_d, tb, 0);
+          fill(reinterpret_cast<uint32_t*>(h->ctl_d) + tb + 2, words - tb - 2, 0);
+        } else {
+          fill(h->ctl_d, sizeof(Ctl) / 4, 0);
+        }
         fill(h->ctr.p, 2ull * CTR_KINDS * kStripes, 0);
         fill(h->tile_flags.p, (2 * M.ntile0 + 3) / 4, 0);
         fill(h->seg_cnt_f.p, M.nseg, 0);
         fill(h->seg_cnt_s.p, M.nseg, 0);
         fill(h->src_hist.p, M.nbins, 0);
-        fill(h->blk_cnt.p, M.nblk_leaf, 0);  // counts that k_leaf1 accumulates
+        if (!dl) fill(h->blk_cnt.p, M.nblk_leaf, 0);  // counts that k_leaf1 accumulates
         if (h->debug) {
           fill(h->dbg_tree.p, 4 * (2ull * M.n - 1), 0);
           fill(h->dbg_labels.p, (2ull * M.n - 1 + 3) / 4, 0);
         }
       }
-      ops.add<FillSeg>(1, img.add(fs, fb), [](Launch<FillSeg> L, uint32_t g, hipStream_t st) { k_fill<<<g, kThreads, 0, st>>>(L); });
+      if (dl) {
+        if (fs.size() > kPrologueFills) throw HipError{hipErrorInvalidValue, "too many fills for a direct leaf launch"};
+        for (size_t i = 0; i < fs.size(); i++) direct->pro.fill[i] = fs[i];
+        direct->pro.nfill = (uint32_t)fs.size();
+      } else {
+        ops.add<FillSeg>(1, img.add(fs, fb), [](Launch<FillSeg> L, uint32_t g, hipStream_t st) { k_fill<<<g, kThreads, 0, st>>>(L); });
+      }
     }
     // leaf kernel, one launch per (hash, staged chunk size, single tile) group
     std::map<std::tuple<int, int, bool>, std::vector<size_t>> groups;
-    for (size_t i = 0; i < B; i++)
-      groups[{ms[i].h->hash_kind, ms[i].cb,
-              ms[i].cb >= 16 && ms[i].tpb == 1 && ms[i].h->approach != MTC_APPROACH_BASIC}].push_back(i);
+    for (size_t i = 0; i < B; i++) groups[{ms[i].h->hash_kind, ms[i].cb, lean_leaf(ms[i])}].push_back(i);
     for (auto& kv : groups) {
       std::vector<LeafArgs> la;
       std::vector<uint32_t> lb;
@@ -2817,6 +2894,14 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
       }
       const int hk = std::get<0>(kv.first), cb = std::get<1>(kv.first);
       const bool lean = std::get<2>(kv.first);
+      if (dl) {
+        direct->ok = true;
+        direct->hk = hk;
+        direct->cb = cb;
+        direct->grid = lb[0];
+        direct->la = la[0];
+        continue;
+      }
       const uint32_t tag = 100 + (uint32_t)hk * 32 + (lean ? 16u : 0u) + (uint32_t)(cb ? __builtin_ctz((unsigned)cb) : 0);
       ops.add<LeafArgs>(tag, img.add(la, lb), [hk, cb, lean](Launch<LeafArgs> L, uint32_t g, hipStream_t st) {
         if (lean) {
@@ -2899,15 +2984,23 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
   const bool debug = std::any_of(ms.begin(), ms.end(), [](const Member& M) { return M.h->debug != 0; });
   const bool split = std::any_of(ms.begin(), ms.end(), [](const Member& M) { return M.split; });
   const bool graph = use_graph && !debug;
-  ctx.ctl_all.ensure(B);
-  for (size_t i = 0; i < B; i++) ms[i].ctl_out = ctx.ctl_all.p + i;
+  // Each member's last kernel writes its control block straight into pinned host
+  // memory: no copy (and no idle time before one) ends the step.
+  if (ctx.ctl_h_cap < B) {
+    if (ctx.ctl_h) (void)hipHostFree(ctx.ctl_h);
+    ctx.ctl_h = nullptr;
+    MTC_HIP(hipHostMalloc((void**)&ctx.ctl_h, B * sizeof(Ctl)));
+    ctx.ctl_h_cap = (uint32_t)B;
+  }
+  for (size_t i = 0; i < B; i++) ms[i].ctl_out = ctx.ctl_h + i;
   // Uploads an arena image and runs its launches, replayed from a graph or eagerly.
   // Phase boundaries are wall-clock stamps the kernels write into each member's
   // control block: an event recorded between two launches stalled the queue for
   // about 6 us each.
   auto run_ops = [&](Ops& ops, ArenaImage& img, bool as_graph, DevBuf<uint8_t>& arena_d, uint8_t*& arena_h,
-                     uint64_t& arena_h_cap) {
-    const uint64_t bytes = img.bytes.size();
+                     uint64_t& arena_h_cap, DirectLeaf* direct = nullptr) {
+    const uint64_t bytes = (img.bytes.size() + 15) & ~15ull;  // whole 16-byte words
+    img.bytes.resize(bytes);
     const uint8_t* old_arena = arena_d.p;
     arena_d.ensure(bytes);
     if (arena_h_cap < bytes) {
@@ -2917,7 +3010,16 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
       arena_h_cap = bytes;
     }
     std::memcpy(arena_h, img.bytes.data(), bytes);
-    MTC_HIP(hipMemcpyAsync(arena_d.p, arena_h, bytes, hipMemcpyHostToDevice, st));
+    if (direct && direct->ok) {  // the leaf kernel copies the arena
+      direct->pro.src = reinterpret_cast<const uint4*>(arena_h);
+      direct->pro.dst = reinterpret_cast<uint4*>(arena_d.p);
+      direct->pro.n16 = bytes / 16;
+      if (direct->hk == HASH_MD5) launch_leaf1_direct<HASH_MD5>(direct->cb, direct->la, direct->pro, direct->grid, st);
+      else launch_leaf1_direct<HASH_MURMUR3>(direct->cb, direct->la, direct->pro, direct->grid, st);
+      MTC_HIP(hipGetLastError());
+    } else {
+      MTC_HIP(hipMemcpyAsync(arena_d.p, arena_h, bytes, hipMemcpyHostToDevice, st));
+    }
     uint8_t* base = arena_d.p;
     if (as_graph) {
       ops.sig.push_back(reinterpret_cast<uint64_t>(base));
@@ -2968,9 +3070,10 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
   } else {
     Ops ops;
     ArenaImage img;
-    build_ops(ms, ops, img, 0, false);
+    DirectLeaf direct;
+    build_ops(ms, ops, img, 0, false, &direct);
     ht.mark("build");
-    run_ops(ops, img, false, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap);
+    run_ops(ops, img, false, ctx.arena_d, ctx.arena_h, ctx.arena_h_cap, &direct);
   }
   if (debug)
     for (auto& M : ms)
@@ -2979,19 +3082,10 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
         MTC_HIP(hipMemcpyAsync(M.h->dbg_labels.p + (M.n - 1), M.h->labels.p, M.n, hipMemcpyDeviceToDevice, st));
       }
   ht.mark("launch");
-  if (ctx.ctl_h_cap < B) {
-    if (ctx.ctl_h) (void)hipHostFree(ctx.ctl_h);
-    ctx.ctl_h = nullptr;
-    MTC_HIP(hipHostMalloc((void**)&ctx.ctl_h, B * sizeof(Ctl)));
-    ctx.ctl_h_cap = (uint32_t)B;
-  }
-  MTC_HIP(hipMemcpyAsync(ctx.ctl_h, ctx.ctl_all.p, B * sizeof(Ctl), hipMemcpyDeviceToHost, st));
-  MTC_HIP(hipEventRecord(ctx.ev[3], st));
   MTC_HIP(hipStreamSynchronize(st));
   ht.mark("sync");
-  float ms_setup = 0, ms_total = 0;
+  float ms_setup = 0;
   MTC_HIP(hipEventElapsedTime(&ms_setup, ctx.ev[0], ctx.ev[1]));
-  MTC_HIP(hipEventElapsedTime(&ms_total, ctx.ev[1], ctx.ev[3]));
   if (ctx.clock_khz == 0) {
     int dev = 0, khz = 0;
     MTC_HIP(hipGetDevice(&dev));
@@ -3017,6 +3111,8 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
     h->has_state = true;
     // k_assemble reset the listed chunks' labels unless the debug view needed them
     h->labels_clean = h->approach != MTC_APPROACH_BASIC && !h->debug;
+    h->blk_zero = std::max<uint64_t>(ms[i].blk_zero, ms[i].nblk_leaf);
+    h->blk_zero_p = h->blk_cnt.p;
     h->copy_pending[ms[i].arena] = false;  // the step waited for that copy and has completed
     *h->ctl_h = c;
     h->table_size += c.claims;
@@ -3026,9 +3122,11 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
     // Timers (src/tree_approach.cpp:1640,1657,1672): set-up (buffers, table reset);
     // labels and forests (leaf kernel, table passes, both sweeps); the diff (root
     // compaction to assembly).  The basic approach has no forests: its whole step
-    // counts as the second.  The leaf time runs from the leaf kernel's first block
-    // to the leaf scan's first block.
+    // counts as the second.  The step runs from the leaf kernel's first block to
+    // its last kernel writing the member's record; the leaf time, to the leaf
+    // scan's first block.
     const double tick_ms = 1.0 / ctx.clock_khz;
+    const float ms_total = c.t_end > c.t_begin ? (float)((c.t_end - c.t_begin) * tick_ms) : 0.0f;
     const float ms_leaf = c.t_leaf > c.t_begin ? (float)((c.t_leaf - c.t_begin) * tick_ms) : 0.0f;
     const float ms_forest = c.t_forest > c.t_begin ? std::min(ms_total, (float)((c.t_forest - c.t_begin) * tick_ms)) : ms_total;
     h->last_ms[0] = ms_setup;

@@ -90,7 +90,8 @@ struct Ctl {
   // first block starts, when the leaf scan's first block starts and when sweep B
   // ends: the phase timers, without event records inside the step.
   unsigned long long t_begin, t_leaf, t_forest;
-  uint32_t pad[46];
+  unsigned long long t_end;  // ... and when the last kernel writes this record out
+  uint32_t pad[44];
 };
 static_assert(sizeof(Ctl) == 256, "the control block is one 256-byte record");
 
@@ -115,12 +116,11 @@ struct StepCtx {
   uint64_t arena_h_cap = 0;
   uint8_t* lead_h = nullptr;  // pinned staging of the arena of its first part (fills, leaf kernels)
   uint64_t lead_h_cap = 0;
-  DevBuf<Ctl> ctl_all;
-  Ctl* ctl_h = nullptr;  // pinned
+  Ctl* ctl_h = nullptr;  // pinned: each member's last kernel writes its record here
   uint32_t ctl_h_cap = 0;
   uint32_t* cnt_h = nullptr;  // pinned: chunks to insert per member (steps that pause after the leaf kernel)
   uint32_t cnt_h_cap = 0;
-  hipEvent_t ev[5] = {};
+  hipEvent_t ev[2] = {};  // step start, set-up done
   int clock_khz = 0;  // device wall clock rate (the control block's stamps)
   hipGraphExec_t exec = nullptr;
   std::vector<uint64_t> sig;
@@ -183,6 +183,8 @@ struct mtc_dedup {
   mtc::DevBuf<uint32_t> pend;         // listed chunks left for k_fixmin / k_resolve
   mtc::DevBuf<uint8_t> pcnt;          // their number per group of 64 dense entries
   mtc::DevBuf<uint32_t> blk_cnt, blk_off;
+  uint64_t blk_zero = 0;             // leading entries of blk_cnt known to be zero (k_compact clears them)
+  const void* blk_zero_p = nullptr;  // ... in this allocation
   mtc::DevBuf<uint8_t> tile_flags;    // per stage-0 tile: has-first, then busy
   mtc::DevBuf<mtc::Slot> table;
   mtc::DevBuf<mtc::Slot> table_tmp;   // rehash target

@@ -58,6 +58,10 @@ struct State {
   std::vector<std::string> launches;  // kernel names in launch order
   std::vector<int> leaf_all_labels;   // per leaf member launched: its all_labels flag
   std::vector<int> label_resets;      // per k_assemble member launched: reset_labels is set
+  // Words of a buffer known to hold zero, from its start (fills; k_compact clears
+  // the leaf block counts it reads; a leaf launch makes its counts unknown).
+  std::map<const void*, uint64_t> zero_upto;
+  std::map<uintptr_t, uint64_t> host_allocs;  // pinned host blocks: start -> size
   hipStream_t capturing = nullptr;
   std::vector<Node> captured;
   uint64_t bytes_alive = 0;
@@ -189,13 +193,45 @@ void check_sweep(const std::string& w, const SweepArgs& a) {
   if (a.d_in <= a.d_out || a.d_in - a.d_out > 8) violation(w + ": stage depths out of range");
 }
 
+// A pinned host block (hipHostMalloc) holding [p, p + bytes).
+void need_host(const std::string& where, const char* field, const void* p, uint64_t bytes) {
+  auto& h = S().host_allocs;
+  auto it = h.upper_bound(reinterpret_cast<uintptr_t>(p));
+  if (!p || it == h.begin() || (--it, reinterpret_cast<uintptr_t>(p) + bytes > it->first + it->second))
+    violation(where + ": " + field + " is not inside a pinned host block");
+}
+
+// A fill: its words hold the value (zero counts for the block-count tracking).
+void note_fill(const FillSeg& f) {
+  if (f.v != 0) return;
+  uint64_t& z = S().zero_upto[f.p];
+  z = std::max<uint64_t>(z, f.count);
+}
+
+void check_leaf(const std::string& w, const LeafArgs& a, uint32_t nblk, bool adds) {
+  need(w, "data", a.data, a.len);
+  need(w, "leaf_dg", a.leaf_dg, 16ull * a.n);
+  need(w, "labels", a.labels, a.n);
+  need(w, "list", a.list, 4ull * a.n);
+  need(w, "blk_cnt", a.blk_cnt, 4ull * nblk);
+  maybe(w, "list_dg", a.list_dg, 16ull * a.n);
+  maybe(w, "stamp", a.stamp, 8);
+  if ((uint64_t)nblk * a.tiles_per_block * 256 < a.n) violation(w + ": leaf blocks do not cover the chunks");
+  uint64_t& z = S().zero_upto[a.blk_cnt];
+  if (adds && z < nblk)
+    violation(w + ": the leaf kernel adds to block counts not known to be zero (" + std::to_string(z) + " of " +
+              std::to_string(nblk) + ")");
+  z = 0;
+  S().leaf_all_labels.push_back(a.all_labels);
+}
+
 void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
   LaunchRaw L;
   std::memcpy(&L, arg0, sizeof(L));
   auto has = [&](const char* t) { return name.find(t) != std::string::npos; };
   if (has("7FillSeg")) {
     check_members<FillSeg>(name, L, grid, [](const std::string& w, const FillSeg& a, uint32_t) {
-      need(w, "p", a.p, 4 * a.count);
+      if (need(w, "p", a.p, 4 * a.count)) note_fill(a);
     });
   } else if (has("7ScanSeg")) {
     check_members<ScanSeg>(name, L, grid, [](const std::string& w, const ScanSeg& a, uint32_t) {
@@ -207,19 +243,14 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       maybe(w, "stamp", a.stamp, 8);
     });
   } else if (has("8LeafArgs")) {
-    check_members<LeafArgs>(name, L, grid, [](const std::string& w, const LeafArgs& a, uint32_t nblk) {
-      need(w, "data", a.data, a.len);
-      need(w, "leaf_dg", a.leaf_dg, 16ull * a.n);
-      need(w, "labels", a.labels, a.n);
-      need(w, "list", a.list, 4ull * a.n);
-      need(w, "blk_cnt", a.blk_cnt, 4ull * nblk);
-      maybe(w, "list_dg", a.list_dg, 16ull * a.n);
-      maybe(w, "stamp", a.stamp, 8);
-      if ((uint64_t)nblk * a.tiles_per_block * 256 < a.n) violation(w + ": leaf blocks do not cover the chunks");
-      S().leaf_all_labels.push_back(a.all_labels);
+    const bool adds = has("7k_leaf1I");  // k_leaf1 adds to its block's count; k_leaf stores it
+    check_members<LeafArgs>(name, L, grid, [adds](const std::string& w, const LeafArgs& a, uint32_t nblk) {
+      check_leaf(w, a, nblk, adds);
     });
   } else if (has("11CompactArgs")) {
     check_members<CompactArgs>(name, L, grid, [](const std::string& w, const CompactArgs& a, uint32_t) {
+      uint64_t& z = S().zero_upto[a.blk_cnt];
+      z = std::max<uint64_t>(z, a.nblk);  // cleared after reading
       need(w, "list", a.list, 4);
       need(w, "blk_cnt", a.blk_cnt, 4ull * a.nblk);
       need(w, "blk_off", a.blk_off, 4ull * (a.nblk + 1));
@@ -314,7 +345,7 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       need(w, "data", a.data, a.len);
       need(w, "ctl", a.ctl, kCtl);
       need(w, "ctr", a.ctr, 8);
-      need(w, "out", a.out, kCtl);
+      need_host(w, "out", a.out, kCtl);
     });
   } else if (has("8DiffArgs")) {
     check_members<DiffArgs>(name, L, grid, [](const std::string& w, const DiffArgs& a, uint32_t) {
@@ -328,7 +359,7 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       need(w, "ctl", a.ctl, kCtl);
       need(w, "ctr", a.ctr, 8);
       need(w, "changed", a.changed, 4);
-      need(w, "out", a.out, kCtl);
+      need_host(w, "out", a.out, kCtl);
       need(w, "dense", a.dense, 4ull * a.n);
       maybe(w, "reset_labels", a.reset_labels, a.n);
       S().label_resets.push_back(a.reset_labels ? 1 : 0);
@@ -347,6 +378,28 @@ void check(const void* fn, dim3 grid, dim3 block, void** args) {
   if (block.x * block.y * block.z > 1024) violation(name + ": block too large");
   if (name.find("6Launch") != std::string::npos) {
     check_launch(name, static_cast<const uint8_t*>(args[0]), grid.x);
+  } else if (name.find("k_leaf1_direct") != std::string::npos) {
+    // The leaf kernel of a one-member step with its arguments by value: it copies
+    // the step's argument arena from pinned host memory and does the step's fills.
+    LeafArgs a;
+    Prologue p;
+    std::memcpy(&a, args[0], sizeof(a));
+    std::memcpy(&p, args[1], sizeof(p));
+    const uint64_t bytes = 16 * p.n16;
+    if (bytes) {
+      auto h = st.host_allocs.upper_bound(reinterpret_cast<uintptr_t>(p.src));
+      if (h == st.host_allocs.begin() || (--h, reinterpret_cast<uintptr_t>(p.src) + bytes > h->first + h->second))
+        violation(name + ": arena source is not inside a pinned host block");
+      else if (need(name, "arena", p.dst, bytes))
+        std::memcpy(p.dst, p.src, bytes);
+    }
+    if (p.nfill > kPrologueFills) violation(name + ": too many fills");
+    // Its fills run beside its tiles, so they cannot clear the counts it adds to.
+    check_leaf(name, a, grid.x, true);
+    for (uint32_t f = 0; f < p.nfill && f < kPrologueFills; f++) {
+      if (p.fill[f].p == a.blk_cnt) violation(name + ": fills the block counts it adds to");
+      else if (need(name, "fill", p.fill[f].p, 4 * p.fill[f].count)) note_fill(p.fill[f]);
+    }
   } else if (name.find("k_fill_u64") != std::string::npos) {
     uint64_t* p;
     uint64_t count;
@@ -489,9 +542,16 @@ hipError_t hipFree(void* p) {
 }
 hipError_t hipHostMalloc(void** ptr, size_t size, unsigned int) {
   *ptr = std::calloc(1, size ? size : 1);
-  return *ptr ? hipSuccess : hipErrorOutOfMemory;
+  if (!*ptr) return hipErrorOutOfMemory;
+  std::lock_guard<std::mutex> g(S().mu);
+  S().host_allocs[reinterpret_cast<uintptr_t>(*ptr)] = size;
+  return hipSuccess;
 }
 hipError_t hipHostFree(void* p) {
+  {
+    std::lock_guard<std::mutex> g(S().mu);
+    S().host_allocs.erase(reinterpret_cast<uintptr_t>(p));
+  }
   std::free(p);
   return hipSuccess;
 }

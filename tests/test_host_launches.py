@@ -296,6 +296,71 @@ def test_failed_step_drops_every_members_state(fake, monkeypatch):
     L.mtc_batch_destroy(b)
 
 
+class LeafArgs(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("len", ctypes.c_uint64), ("n", ctypes.c_uint32), ("C", ctypes.c_uint32),
+                ("leaf_dg", ctypes.c_void_p), ("labels", ctypes.c_void_p), ("baseline", ctypes.c_int),
+                ("list", ctypes.c_void_p), ("blk_cnt", ctypes.c_void_p), ("tiles_per_block", ctypes.c_uint64),
+                ("all_labels", ctypes.c_int), ("list_dg", ctypes.c_void_p), ("stamp", ctypes.c_void_p)]
+
+
+class FillSeg(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_void_p), ("count", ctypes.c_uint64), ("v", ctypes.c_uint32)]
+
+
+class Prologue(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("n16", ctypes.c_uint64),
+                ("fill", FillSeg * 12), ("nfill", ctypes.c_uint32)]
+
+
+def test_checker_flags_leaf_counts_not_known_zero(fake):
+    """Negative control: a direct leaf launch whose block counts were neither filled
+    nor cleared by a k_compact is reported, and so is one that fills them itself."""
+    L = fake
+    L.fakehip_reset()
+    n = 1000
+    a = LeafArgs(dev_alloc(L, 128 * n), 128 * n, n, 128, dev_alloc(L, 16 * n), dev_alloc(L, n), 0,
+                 dev_alloc(L, 4 * n), dev_alloc(L, 4 * 4), 1, 0, dev_alloc(L, 16 * n), None)
+    stub = getattr(L, "_ZN3mtc29__device_stub__k_leaf1_directILi0ELi128EEEvNS_8LeafArgsENS_8PrologueE")
+    L.__hipPushCallConfiguration(Dim3(4, 1, 1), Dim3(256, 1, 1), ctypes.c_size_t(0), None)
+    stub(a, Prologue())
+    assert any("not known to be zero" in v for v in violations(L)), violations(L)
+    L.fakehip_reset()
+    pro = Prologue()
+    pro.fill[0] = FillSeg(a.blk_cnt, 4, 0)  # a fill inside the same launch does not count
+    pro.nfill = 1
+    L.__hipPushCallConfiguration(Dim3(4, 1, 1), Dim3(256, 1, 1), ctypes.c_size_t(0), None)
+    stub(a, pro)
+    v = violations(L)
+    assert any("not known to be zero" in x for x in v) and any("fills the block counts" in x for x in v), v
+    L.fakehip_reset()
+
+
+def test_direct_leaf_launch_follows_cleared_counts(fake, monkeypatch):
+    """A one-member step launches its leaf kernel by value (no upload copy, no fill
+    kernel) only when the last completed step's k_compact cleared the block counts;
+    after a failed step it fills them again first."""
+    L = fake
+    L.fakehip_reset()
+    h = create(L, 128)
+    size = 128 * 5000
+    data = dev_alloc(L, size)
+    kinds = []
+    for k in range(6):
+        L.fakehip_reset()
+        if k == 3:
+            monkeypatch.setenv("MTC_TEST_FAIL_MEMBER", "0")
+        rc, _ = checkpoint(L, h, data, size, baseline=(k == 0))
+        monkeypatch.delenv("MTC_TEST_FAIL_MEMBER", raising=False)
+        assert rc == (STATE if k == 3 else OK), L.mtc_last_error(h)
+        names = launches(L)
+        kinds.append("direct" if any("k_leaf1_direct" in x for x in names) else
+                     "filled" if any("k_fill" in x for x in names) else "?")
+        assert violations(L) == [], violations(L)
+    # k == 4 runs after the failed step: a new baseline, counts filled again
+    assert kinds == ["filled", "direct", "direct", "direct", "filled", "direct"], kinds
+    L.mtc_destroy(h)
+
+
 def test_server_admission_and_submit_limits(fake):
     L = fake
     s = ctypes.c_void_p()
