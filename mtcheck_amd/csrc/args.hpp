@@ -48,9 +48,13 @@ struct ScanSeg {
   uint32_t* total;        // optional: sum of all elements
   uint32_t* sums;         // ceil(count_max / kScanTile) words
   unsigned long long* stamp;  // optional: the wall clock when the segment's first block starts
+  // Optional, for a scan of region sizes: tile_lo[b] = the element whose range of
+  // the running total holds position 256 b (the first region of diff tile b).
+  uint32_t* tile_lo;
   __device__ ScanSeg global() const {
     ScanSeg c = *this;
     MTC_GLOBAL(in); MTC_GLOBAL(out); MTC_GLOBAL(count_dev); MTC_GLOBAL(total); MTC_GLOBAL(sums); MTC_GLOBAL(stamp);
+    MTC_GLOBAL(tile_lo);
     return c;
   }
 };
@@ -261,6 +265,7 @@ struct DiffArgs {
   const uint32_t* ptable;
   const uint64_t* shift_sorted;  // (prev << 32 | node)
   const uint32_t* region_off;
+  const uint32_t* tile_lo;  // per 256 diff chunks: the region holding the first
   const uint32_t* total_chunks;
   const uint8_t* data;
   uint64_t len;
@@ -277,7 +282,7 @@ struct DiffArgs {
   uint8_t* reset_labels;          // when set: their labels go back to L_FIXED
   __device__ DiffArgs global() const {
     DiffArgs c = *this;
-    MTC_GLOBAL(diff); MTC_GLOBAL(first_roots); MTC_GLOBAL(ptable); MTC_GLOBAL(shift_sorted); MTC_GLOBAL(region_off); MTC_GLOBAL(total_chunks); MTC_GLOBAL(data); MTC_GLOBAL(ctl);
+    MTC_GLOBAL(diff); MTC_GLOBAL(first_roots); MTC_GLOBAL(ptable); MTC_GLOBAL(shift_sorted); MTC_GLOBAL(region_off); MTC_GLOBAL(tile_lo); MTC_GLOBAL(total_chunks); MTC_GLOBAL(data); MTC_GLOBAL(ctl);
     MTC_GLOBAL(ctr); MTC_GLOBAL(changed); MTC_GLOBAL(out); MTC_GLOBAL(dense); MTC_GLOBAL(reset_labels);
     return c;
   }

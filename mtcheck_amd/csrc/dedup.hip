@@ -171,6 +171,12 @@ __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t
 // count_max.
 // ---------------------------------------------------------------------------
 
+// Element e covers [o, o + size) of the running total: it is the first element of
+// every 256-position tile that starts inside that range.
+__device__ __forceinline__ void mark_tiles(uint32_t* tile_lo, uint32_t o, uint32_t size, uint32_t e) {
+  for (uint32_t b = (o + 255) / 256; (uint64_t)b * 256 < (uint64_t)o + size; b++) tile_lo[b] = e;
+}
+
 __device__ __forceinline__ uint32_t scan_count(const ScanSeg& s) {
   return s.count_dev ? min(*s.count_dev, s.count_max) : s.count_max;
 }
@@ -231,7 +237,10 @@ __device__ __forceinline__ void ph_scan_down2(const Launch<ScanSeg>& L, const Wh
     const uint32_t off = pre + block_scan_excl(run, &tot, ws);
 #pragma unroll
     for (uint32_t j = 0; j < 8; j++)
-      if (base + j < cnt) s.out[base + j] = off + v[j];
+      if (base + j < cnt) {
+        if (s.tile_lo) mark_tiles(s.tile_lo, off + v[j], (j + 1 < 8 ? v[j + 1] : run) - v[j], (uint32_t)(base + j));
+        s.out[base + j] = off + v[j];
+      }
     if (tile + 1 == ntiles && threadIdx.x == 0 && s.total) *s.total = pre + tot;
     pre += tot;  // this tile's sum
     done = tile + 1;
@@ -263,7 +272,10 @@ __device__ __forceinline__ void ph_scan_one(const Launch<ScanSeg>& L, const Wher
   const uint32_t off = block_scan_excl(run, &tot, ws);
 #pragma unroll
   for (uint32_t j = 0; j < 8; j++)
-    if (base + j < cnt) s.out[base + j] = off + v[j];
+    if (base + j < cnt) {
+      if (s.tile_lo) mark_tiles(s.tile_lo, off + v[j], (j + 1 < 8 ? v[j + 1] : run) - v[j], base + j);
+      s.out[base + j] = off + v[j];
+    }
   if (threadIdx.x == 0 && s.total) *s.total = tot;
 }
 
@@ -680,19 +692,26 @@ __device__ __forceinline__ void ph_compact(const Launch<CompactArgs>& L, const W
   uint32_t* s_pre = reinterpret_cast<uint32_t*>(lds);
   const CompactArgs a = L.args[w.m].global();
   const uint32_t b0 = w.blk * kCompactPer, nb = min(kCompactPer, a.nblk - b0);
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (uint32_t i = 0; i < nb; i++) {
-      s_pre[i] = run;
-      run += a.blk_cnt[b0 + i];
+  // One wavefront loads the counts at once and scans them (a walk by one thread
+  // waits for each load in turn).
+  if (threadIdx.x < 64) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t v = lane < nb ? a.blk_cnt[b0 + lane] : 0u;
+    if (lane == 0) s_pre[kCompactPer + 1] = a.blk_off[b0];
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= (uint32_t)o) x += y;
     }
-    s_pre[nb] = run;
+    if (lane < nb) s_pre[lane] = x - v;
+    if (lane + 1 == nb) s_pre[nb] = x;
   }
   __syncthreads();
   // The last reader of the counts clears them (the leaf scan read them before).
   if (threadIdx.x < nb) a.blk_cnt[b0 + threadIdx.x] = 0;
   const uint32_t total = s_pre[nb];
-  const uint32_t off = a.blk_off[b0];
+  const uint32_t off = s_pre[kCompactPer + 1];
   uint32_t* dst = a.dense + off;
   for (uint32_t e = threadIdx.x; e < total; e += kThreads) {
     uint32_t lo = 0, hi = nb;  // last region whose start <= e
@@ -708,7 +727,7 @@ __device__ __forceinline__ void ph_compact(const Launch<CompactArgs>& L, const W
 }
 
 __global__ __launch_bounds__(kThreads) void k_compact(Launch<CompactArgs> L) {
-  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * (kCompactPer + 1)];
+  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * (kCompactPer + 2)];
   ph_compact(L, where(L), lds);
 }
 
@@ -1780,6 +1799,8 @@ __device__ void diff_meta(const DiffArgs& a, const Where& w) {
   }
 }
 
+constexpr uint32_t kAsmWindow = 512;  // region offsets staged per tile of 256 diff chunks
+
 // The diff in one launch: metadata (every block, grid-stride), then block 0 folds
 // the counters and copies the control block out (its diff size is final once
 // block 0's own metadata thread has written it), then the chunks of every
@@ -1788,6 +1809,7 @@ __device__ void diff_meta(const DiffArgs& a, const Where& w) {
 __device__ __forceinline__ void ph_assemble(const Launch<DiffArgs>& L, const Where& wh, uint8_t* lds) {
   uint32_t* s_src = reinterpret_cast<uint32_t*>(lds);
   unsigned long long* s_fold = reinterpret_cast<unsigned long long*>(lds + sizeof(uint32_t) * kThreads);
+  uint32_t* s_off = reinterpret_cast<uint32_t*>(lds + sizeof(uint32_t) * kThreads + sizeof(unsigned long long) * CTR_KINDS);
   static_assert(kThreads == kStripes, "one block folds the striped counters");
   const DiffArgs a = L.args[wh.m].global();
   diff_meta(a, wh);
@@ -1812,16 +1834,36 @@ __device__ __forceinline__ void ph_assemble(const Launch<DiffArgs>& L, const Whe
   for (uint64_t tile = wh.blk; tile * kThreads < total; tile += wh.nblk) {
     const uint64_t j0 = tile * kThreads;
     const uint64_t j = j0 + threadIdx.x;
+    // The tile's regions start at tile_lo[tile]: their offsets are staged in LDS
+    // (a search over every region's offset in global memory waited for about 18
+    // dependent loads per chunk).
+    const uint32_t lo0 = a.tile_lo[tile];
+    const uint32_t win = min(F - lo0, kAsmWindow);
+    for (uint32_t t = threadIdx.x; t < win; t += kThreads) s_off[t] = a.region_off[lo0 + t];
+    __syncthreads();
     if (j < total) {
-      uint32_t lo = 0, hi = F;  // last region whose offset <= j
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a.region_off[mid] <= j) lo = mid;
-        else hi = mid;
+      uint32_t lo, hi;  // last region whose offset <= j
+      if (win == kAsmWindow && j >= s_off[kAsmWindow - 1]) {  // many empty regions: the rest in global memory
+        lo = lo0 + kAsmWindow - 1;
+        hi = F;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (a.region_off[mid] <= j) lo = mid;
+          else hi = mid;
+        }
+      } else {
+        uint32_t l = 0, h = win;
+        while (h - l > 1) {
+          const uint32_t mid = (l + h) >> 1;
+          if (s_off[mid] <= j) l = mid;
+          else h = mid;
+        }
+        lo = lo0 + l;
       }
       uint32_t st[2], ct[2];
       const int r = region_runs(a.g, a.first_roots[lo], a.compat, st, ct);
-      s_src[threadIdx.x] = run_at(r, st, ct, (uint32_t)(j - a.region_off[lo]));
+      const uint32_t off = lo - lo0 < win ? s_off[lo - lo0] : a.region_off[lo];
+      s_src[threadIdx.x] = run_at(r, st, ct, (uint32_t)(j - off));
     }
     __syncthreads();
     const uint64_t cnt = (total - j0 < kThreads) ? total - j0 : kThreads;
@@ -1848,7 +1890,8 @@ __device__ __forceinline__ void ph_assemble(const Launch<DiffArgs>& L, const Whe
 }
 
 __global__ __launch_bounds__(kThreads) void k_assemble(Launch<DiffArgs> L) {
-  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * kThreads + sizeof(unsigned long long) * CTR_KINDS];
+  __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * kThreads + sizeof(unsigned long long) * CTR_KINDS +
+                                       sizeof(uint32_t) * kAsmWindow];
   ph_assemble(L, where(L), lds);
 }
 
@@ -2185,6 +2228,7 @@ static void ensure_buffers(mtc_dedup* h, hipStream_t st) {
   }
   h->sort_hist.ensure(256ull * sort_blocks(h->n));
   h->region_off.ensure(n + 1);
+  h->tile_lo.ensure(n / 256 + 2);
   // scan partial sums: leaf blocks, two root-segment arrays, the sort, the regions
   h->scan_ws.ensure(scan_sums(max_blk) + 2 * scan_sums(nseg) + scan_sums(256ull * sort_blocks(h->n)) + scan_sums(n));
   // P <= checkpoints since the baseline; reserve in steps of 4096 so the arena is
@@ -2740,7 +2784,8 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       ra.push_back(RegionArgs{h->first_roots.p, &h->ctl_d->F, M.g, h->region_off.p, h->compat,
                               PtableArgs{h->src_hist.p, M.nbins, h->baseline_id, h->ptable.p, h->ctl_d}});
       rb.push_back(grid_for(M.n, 16 * kThreads, 256));
-      ss.push_back(ScanSeg{h->region_off.p, h->region_off.p, &h->ctl_d->F, M.n, h->region_off.p + M.n, M.ws_reg});
+      ss.push_back(ScanSeg{h->region_off.p, h->region_off.p, &h->ctl_d->F, M.n, h->region_off.p + M.n, M.ws_reg,
+                           nullptr, h->tile_lo.p});
     }
     ops.add<RegionArgs>(70, img.add(ra, rb), [](Launch<RegionArgs> L, uint32_t g, hipStream_t st) { k_region_sizes<<<g, kThreads, 0, st>>>(L); });
     add_scan(ops, img, ss);
@@ -2755,6 +2800,7 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
       d.ptable = h->ptable.p;
       d.shift_sorted = sorted[i];
       d.region_off = h->region_off.p;
+      d.tile_lo = h->tile_lo.p;
       d.total_chunks = h->region_off.p + M.n;
       d.data = M.data;
       d.len = M.len;

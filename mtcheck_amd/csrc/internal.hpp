@@ -201,6 +201,7 @@ struct mtc_dedup {
   mtc::DevBuf<uint32_t> src_hist;     // per source checkpoint
   mtc::DevBuf<uint32_t> ptable;       // P (chkpt, count) pairs
   mtc::DevBuf<uint32_t> region_off;   // F region offsets (chunks), then total
+  mtc::DevBuf<uint32_t> tile_lo;      // per 256 diff chunks: the region holding the first
   mtc::DevBuf<uint8_t> diff;          // device diff arena 0
   mtc::DevBuf<uint8_t> diff_alt;      // arena 1, allocated once asynchronous copies are used
   int diff_cur = 0;                   // arena holding the last checkpoint's diff

@@ -241,6 +241,7 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       maybe(w, "total", a.total, 4);
       need(w, "sums", a.sums, 4 * tiles(a.count_max));
       maybe(w, "stamp", a.stamp, 8);
+      maybe(w, "tile_lo", a.tile_lo, 4ull * (a.count_max / 256 + 1));  // region sizes: at most count_max chunks
     });
   } else if (has("8LeafArgs")) {
     const bool adds = has("7k_leaf1I");  // k_leaf1 adds to its block's count; k_leaf stores it
@@ -354,6 +355,7 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       need(w, "ptable", a.ptable, 8);
       need(w, "shift_sorted", a.shift_sorted, 8);
       need(w, "region_off", a.region_off, 4ull * (a.n + 1));
+      need(w, "tile_lo", a.tile_lo, 4ull * (a.n / 256 + 1));
       need(w, "total_chunks", a.total_chunks, 4);
       need(w, "data", a.data, a.len);
       need(w, "ctl", a.ctl, kCtl);
