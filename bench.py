@@ -150,14 +150,14 @@ def leaf_traffic(nbytes: int, C: int, hash_name: str, change: float):
 
 
 def leaf_profile(nbytes: int, C: int, hash_name: str, change: float):
-    """(summary file, average k_leaf1 duration in ms) from the committed
+    """(summary file, average leaf-kernel duration in ms) from the committed
     rocprofv3 --kernel-trace --stats summary of this workload, or None."""
     name = f"checkpoint_kernels_{nbytes >> 30}gib_c{C}_{hash_name}" + ("" if change == 0.02 else f"_{change * 100:g}pct")
     path = os.path.join(ROOT, "profiles", name + ".md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("| `mtc::k_leaf1<"):
+        if line.startswith("| `mtc::k_leaf1<") or line.startswith("| `mtc::k_leaf1_direct<"):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             return f"profiles/{name}.md", round(float(cells[2]) / 1e3, 4)
     return None

@@ -191,7 +191,8 @@ class TreeDeduplicator {
     if (!logname.empty()) write_restart_log(chkpt_id, logname);
   }
 
-  // write_chkpt_log, :1853-1896: <logname>.chunk_size.<C>.csv, .size.csv, .timing.csv.
+  // write_chkpt_log, :1853-1896 (basic: src/basic_approach.cpp:632-673): <logname>.chunk_size.<C>.csv,
+  // .size.csv, .timing.csv.
   void write_chkpt_log(const mtc_header& header, const std::vector<uint8_t>& diff_h, const std::string& logname) {
     const std::string base = logname + ".chunk_size." + std::to_string(chunk_size);
     std::ofstream result(base + ".csv", std::ios::app), size(base + ".size.csv", std::ios::app),
@@ -200,18 +201,22 @@ class TreeDeduplicator {
       result << "Approach,Chkpt ID,Chunk Size,Uncompressed Size,Compressed Size,Data Size,Metadata Size,"
                 "Setup Time,Comparison Time,Gather Time,Write Time\n";
     const uint32_t cid = header.chkpt_id;
-    result << "Tree," << cid << "," << chunk_size << "," << header.datalen << "," << datasizes.first + datasizes.second
+    const char* name = approach_name();
+    result << name << "," << cid << "," << chunk_size << "," << header.datalen << "," << datasizes.first + datasizes.second
            << "," << datasizes.first << "," << datasizes.second << "," << timers[0] << "," << timers[1] << ","
            << timers[2] << "," << timers[3] << "\n";
-    timing << "Tree," << cid << "," << chunk_size << "," << timers[1] << "," << timers[2] << "," << timers[3] << "\n";
-    size << "Tree," << cid << "," << chunk_size << "," << datasizes.first << "," << datasizes.second << ",";
-    write_metadata_breakdown(size, header, diff_h, cid);
+    timing << name << "," << cid << "," << chunk_size << "," << timers[1] << "," << timers[2] << "," << timers[3] << "\n";
+    size << name << "," << cid << "," << chunk_size << "," << datasizes.first << "," << datasizes.second << ",";
+    // the basic log counts the checkpoints since the baseline (:672), the tree log the id (:1895)
+    write_metadata_breakdown(size, header, diff_h, approach_ == MTC_APPROACH_BASIC ? cid - header.ref_id + 1 : cid);
   }
-  // write_restart_log, :1904-1915.
+  // write_restart_log, :1904-1915 (basic: src/basic_approach.cpp:682-691).
   void write_restart_log(uint32_t select_chkpt, const std::string& logname) {
     std::ofstream f(logname, std::ios::app);
-    f << "Tree," << select_chkpt << "," << chunk_size << "," << restart_timers[0] << "," << restart_timers[1] << "\n";
+    f << approach_name() << "," << select_chkpt << "," << chunk_size << "," << restart_timers[0] << ","
+      << restart_timers[1] << "\n";
   }
+  const char* approach_name() const { return approach_ == MTC_APPROACH_BASIC ? "Basic" : "Tree"; }
 
   const uint32_t chunk_size;
   std::pair<uint64_t, uint64_t> datasizes{0, 0};  // (data incl. header, metadata), :707-708
@@ -224,14 +229,17 @@ class TreeDeduplicator {
     uint64_t size = 0;
     check(mtc_checkpoint(h_, data_ptr, len, make_baseline, advance, stream, &header, &size), h_);
     diff.resize(size);
+    // the copy of the diff to the host is the fourth timer (:1678-1688)
+    const auto t0 = std::chrono::steady_clock::now();
     check(mtc_copy_diff(h_, diff.data(), size, stream), h_);
+    const double copy_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     const mtc_stats s = stats();
     const uint64_t meta = 4ull * header.num_first_ocur + 8ull * header.num_prior_chkpts + 8ull * header.num_shift_dupl;
     datasizes = {size - meta, meta};
     timers[0] = s.ms_setup / 1e3;
     timers[1] = s.ms_dedup / 1e3;
     timers[2] = s.ms_collect / 1e3;
-    timers[3] = s.ms_copy / 1e3;
+    timers[3] = copy_s;
   }
   void run_restart(const std::vector<const void*>& ptrs, const std::vector<uint64_t>& sizes, int on_device,
                    uint8_t* out, size_t len, uint32_t chkpt_id, void* stream) {

@@ -2,10 +2,11 @@
 //
 // Usage (src/dedup_files.cpp:11-24,42-63):
 //   dedup_chkpt_files chunk_size num_chkpts <mode> file0 file1 ...
-// Writes <file>.hashtree.incr_chkpt next to each file and appends the three CSV
-// logs named after the file's base name in the working directory (:98-102).
-// Every tree mode runs the tree deduplicator; Full, Basic and List are not part of
-// this library and are rejected.
+// Writes <file>.hashtree.incr_chkpt (--run-basic-chkpt: <file>.basic.incr_chkpt)
+// next to each file and appends the three CSV logs named after the file's base
+// name in the working directory (:87-100).  Every tree mode runs the tree
+// deduplicator and Basic the basic one; Full and List are not part of this library
+// and are rejected.
 #include <hip/hip_runtime_api.h>
 
 #include <cstdio>
@@ -35,16 +36,17 @@ int main(int argc, char** argv) {
   const uint32_t num_chkpts = (uint32_t)std::atoi(argv[2]);
   const mtc_approach mode = mtcheck::get_mode(argc, argv);
   if (mode == MTC_APPROACH_UNKNOWN) std::fprintf(stderr, "ERROR: Incorrect mode; running the tree approach\n");
-  if (mode == MTC_APPROACH_FULL || mode == MTC_APPROACH_BASIC || mode == MTC_APPROACH_LIST) {
-    std::fprintf(stderr, "ERROR: only the tree approaches are provided\n");
+  if (mode == MTC_APPROACH_FULL || mode == MTC_APPROACH_LIST) {
+    std::fprintf(stderr, "ERROR: only the tree and basic approaches are provided\n");
     return 2;
   }
+  const bool basic = mode == MTC_APPROACH_BASIC;
   if ((uint32_t)argc < 4 + num_chkpts) {
     std::fprintf(stderr, "ERROR: %u checkpoint files expected\n", num_chkpts);
     return 1;
   }
   try {
-    mtcheck::TreeDeduplicator dedup(chunk_size);
+    mtcheck::TreeDeduplicator dedup(chunk_size, MTC_HASH_MURMUR3, 0, basic ? MTC_APPROACH_BASIC : MTC_APPROACH_TREE);
     uint8_t* dev = nullptr;
     size_t dev_len = 0;
     for (uint32_t idx = 0; idx < num_chkpts; idx++) {
@@ -62,7 +64,7 @@ int main(int argc, char** argv) {
       }
       HIP_OK(hipMemcpy(dev, host.data(), len, hipMemcpyHostToDevice));
       std::string logname = path.substr(path.rfind('/') + 1);
-      dedup.checkpoint(dev, len, path + ".hashtree.incr_chkpt", logname, idx == 0);
+      dedup.checkpoint(dev, len, path + (basic ? ".basic.incr_chkpt" : ".hashtree.incr_chkpt"), logname, idx == 0);
       std::printf("checkpoint %u: %s first=%zu shift=%zu\n", idx, path.c_str(), dedup.num_first_ocur(),
                   dedup.num_shift_dupl());
     }

@@ -55,8 +55,9 @@ def test_file_commands_round_trip(tmp_path):
         log = (tmp_path / f"chk{i}.dat.chunk_size.{C}.csv").read_text().splitlines()
         assert log[0].startswith("Approach,Chkpt ID") and log[1].startswith(f"Tree,{i},{C},{nbytes},")
         row = log[1].split(",")
-        # Comparison Time (labels and forests) and Gather Time (the diff) both measured
-        assert float(row[8]) > 0 and float(row[9]) > 0, row
+        # Comparison Time (labels and forests), Gather Time (the diff) and Write Time
+        # (the copy to the host) all measured
+        assert float(row[8]) > 0 and float(row[9]) > 0 and float(row[10]) > 0, row
         size = (tmp_path / f"chk{i}.dat.chunk_size.{C}.size.csv").read_text().strip().split(",")
         assert size[:3] == ["Tree", str(i), str(C)] and size[5] == "32"
     for rid in (0, k - 1):
@@ -96,3 +97,41 @@ def test_restart_command_basic_mode(tmp_path):
         assert lines[0].split(":")[1].strip() == lines[1].split(":")[1].strip()
         log = (tmp_path / f"chk{rid}.dat.chunk_size.{C}.restart_timing.csv").read_text().splitlines()
         assert log[-1].startswith(f"Basic,{rid},{C},")
+
+
+@pytest.mark.gpu
+def test_file_commands_basic_mode(tmp_path):
+    """dedup_chkpt_files --run-basic-chkpt writes <file>.basic.incr_chkpt identical to the
+    oracle's basic diffs and logs 'Basic' rows (the size log counting the checkpoints
+    since the baseline, src/basic_approach.cpp:672); restart_chkpt_files rebuilds each."""
+    from mtcheck_amd.workload import CheckpointSeries
+    C, nbytes, k = 64, (1 << 19) + 5, 4
+    series = CheckpointSeries(nbytes, C, seed=9)
+    buf = np.zeros(nbytes, np.uint8)
+    orc = O.OracleBasic(C)
+    files, expect = [], []
+    for i in range(k):
+        series.apply_host(buf, i)
+        path = tmp_path / f"chk{i}.dat"
+        path.write_bytes(buf.tobytes())
+        files.append(str(path))
+        expect.append(orc.checkpoint(buf, make_baseline=(i == 0)))
+    print("dedup_chkpt_files basic", flush=True)
+    r = subprocess.run([os.path.join(BIN, "dedup_chkpt_files"), str(C), str(k), "--run-basic-chkpt", *files],
+                       capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for i in range(k):
+        got = open(files[i] + ".basic.incr_chkpt", "rb").read()
+        same = got == expect[i]
+        assert same, (i, first_mismatch(got, expect[i]))
+        log = (tmp_path / f"chk{i}.dat.chunk_size.{C}.csv").read_text().splitlines()
+        assert log[1].startswith(f"Basic,{i},{C},{nbytes},")
+        size = (tmp_path / f"chk{i}.dat.chunk_size.{C}.size.csv").read_text().strip().split(",")
+        assert size[:3] == ["Basic", str(i), str(C)] and len(size) == 9 + i, size
+    for rid in (0, k - 1):
+        print(f"restart_chkpt_files basic {rid}", flush=True)
+        r = subprocess.run([os.path.join(BIN, "restart_chkpt_files"), str(rid), str(k), "1", str(C),
+                            "--run-basic-chkpt", *files], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+        assert r.returncode == 0, r.stderr
+        lines = [l for l in r.stdout.splitlines() if "digest" in l]
+        assert lines[0].split(":")[1].strip() == lines[1].split(":")[1].strip()
