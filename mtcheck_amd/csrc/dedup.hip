@@ -35,6 +35,8 @@ constexpr uint64_t kFirstTag = 0xFFFFFFFFull;  // staging tag of a first-occurre
 // slots are at most 2^31, so neither tag is a slot.
 constexpr uint64_t kLeafTag = 0xFFFFFFFEull;
 constexpr uint32_t kScanTile = 8 * kThreads;   // elements per block-tile of the batched scan
+constexpr uint32_t kScanOneItems = 16;          // elements per thread of a one-launch scan
+constexpr uint32_t kScanOneMax = kScanOneItems * kThreads;
 constexpr uint32_t kSortBlocks = 1024;         // most blocks per member of the stable sort
 #ifndef MTC_TPW_A
 #define MTC_TPW_A 8
@@ -252,17 +254,17 @@ __global__ __launch_bounds__(kThreads) void k_scan_down2(Launch<ScanSeg> L) {
   ph_scan_down2(L, where(L), lds);
 }
 
-// A scan whose every segment fits one tile (count_max <= kScanTile): one block per
+// A scan whose every segment has at most kScanOneMax elements: one block per
 // segment scans it and writes its total, in one launch instead of two.
 __device__ __forceinline__ void ph_scan_one(const Launch<ScanSeg>& L, const Where& w, uint8_t* lds) {
   uint32_t* ws = reinterpret_cast<uint32_t*>(lds);
   const ScanSeg s = L.args[w.m].global();
   scan_stamp(s, w);
   const uint32_t cnt = scan_count(s);
-  const uint32_t base = 8u * threadIdx.x;
-  uint32_t v[8], run = 0;
+  const uint32_t base = kScanOneItems * threadIdx.x;
+  uint32_t v[kScanOneItems], run = 0;
 #pragma unroll
-  for (uint32_t j = 0; j < 8; j++) {
+  for (uint32_t j = 0; j < kScanOneItems; j++) {
     v[j] = (base + j < cnt) ? s.in[base + j] : 0u;
     const uint32_t x = v[j];
     v[j] = run;
@@ -271,9 +273,9 @@ __device__ __forceinline__ void ph_scan_one(const Launch<ScanSeg>& L, const Wher
   uint32_t tot;
   const uint32_t off = block_scan_excl(run, &tot, ws);
 #pragma unroll
-  for (uint32_t j = 0; j < 8; j++)
+  for (uint32_t j = 0; j < kScanOneItems; j++)
     if (base + j < cnt) {
-      if (s.tile_lo) mark_tiles(s.tile_lo, off + v[j], (j + 1 < 8 ? v[j + 1] : run) - v[j], base + j);
+      if (s.tile_lo) mark_tiles(s.tile_lo, off + v[j], (j + 1 < kScanOneItems ? v[j + 1] : run) - v[j], base + j);
       s.out[base + j] = off + v[j];
     }
   if (threadIdx.x == 0 && s.total) *s.total = tot;
@@ -2538,7 +2540,7 @@ static void prepare_member(Member& M, hipStream_t st) {
 static void add_scan(Ops& ops, ArenaImage& img, const std::vector<ScanSeg>& segs) {
   if (segs.empty()) return;
   std::vector<uint32_t> tiles, ones(segs.size(), 1);
-  if (std::all_of(segs.begin(), segs.end(), [](const ScanSeg& s) { return s.count_max <= kScanTile; })) {
+  if (std::all_of(segs.begin(), segs.end(), [](const ScanSeg& s) { return s.count_max <= kScanOneMax; })) {
     ops.add<ScanSeg>(13, img.add(segs, ones), [](Launch<ScanSeg> L, uint32_t g, hipStream_t st) { k_scan_one<<<g, kThreads, 0, st>>>(L); });
     return;
   }
