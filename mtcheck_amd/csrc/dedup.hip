@@ -557,10 +557,10 @@ __global__ __launch_bounds__(kThreads) void k_leaf(Launch<LeafArgs> L) {
 // pass), so both the row-wise stores and the per-lane reads are free of bank
 // conflicts.  (A plain column ^ row swizzle leaves 2-way conflicts on the reads
 // with 256-byte banks: SQ_LDS_BANK_CONFLICT was a quarter of SQ_LDS_IDX_ACTIVE.)
-// Each
-// wavefront reserves its run of the block's region with one atomic add (the
-// block's count is zeroed by the step's first fill), so no block-wide count is
-// needed; within a region the runs are in arrival order.
+// Each wavefront reserves its run of the block's region with one atomic add on
+// the block's count (zero when the kernel starts: filled by the step, or cleared
+// by the previous step's k_compact), so no block-wide count is needed; within a
+// region the runs are in arrival order.
 template <int SEG>
 __device__ __forceinline__ uint32_t lean_col(uint32_t row, uint32_t col) {
   return col ^ (((row / (128 / SEG)) ^ (row / (256 / SEG))) & (SEG / 16 - 1));
