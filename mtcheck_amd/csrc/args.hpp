@@ -210,9 +210,15 @@ struct ShiftArgs {
   uint32_t* src_hist;
   uint32_t nbins;
   Ctl* ctl;
+  // When set: the first sort pass's per-block digit counts (nb0 digit values,
+  // digit-major over the grid), written here instead of by k_sort_count; the
+  // entries are then visited in the sort's block ranges.
+  uint32_t* hist0;
+  uint32_t nb0;
   __device__ ShiftArgs global() const {
     ShiftArgs c = *this;
     MTC_GLOBAL(shift_tmp); MTC_GLOBAL(S); MTC_GLOBAL(leaf_slot); MTC_GLOBAL(tab); MTC_GLOBAL(key); MTC_GLOBAL(val); MTC_GLOBAL(src_hist); MTC_GLOBAL(ctl);
+    MTC_GLOBAL(hist0);
     return c;
   }
 };

@@ -1605,6 +1605,13 @@ __global__ __launch_bounds__(kThreads) void k_scatter_roots(Launch<RootArgs> L) 
 // ---------------------------------------------------------------------------
 constexpr uint32_t kLdsBins = 4096;  // source-checkpoint bins counted in LDS
 
+// Block g of G's contiguous range of the S entries (whole 256-entry rows).
+__device__ __forceinline__ void sort_range(uint32_t S, uint32_t g, uint32_t G, uint32_t& lo, uint32_t& hi) {
+  const uint32_t per = ((S + G - 1) / G + kThreads - 1) / kThreads * kThreads;
+  lo = min(S, g * per);
+  hi = min(S, lo + per);
+}
+
 __device__ __forceinline__ void ph_shift_src(const Launch<ShiftArgs>& L, const Where& w, uint8_t* lds) {
   uint32_t* h = reinterpret_cast<uint32_t*>(lds);
   const ShiftArgs a = L.args[w.m].global();
@@ -1613,7 +1620,13 @@ __device__ __forceinline__ void ph_shift_src(const Launch<ShiftArgs>& L, const W
     for (uint32_t b = threadIdx.x; b < a.nbins; b += kThreads) h[b] = 0;
   __syncthreads();
   const uint32_t S = *a.S;
-  for (uint32_t i = w.blk * kThreads + threadIdx.x; i < S; i += w.nblk * kThreads) {
+  uint32_t lo = 0, hi = S, i0 = w.blk * kThreads, stride = w.nblk * kThreads;
+  if (a.hist0) {  // the first sort pass's block ranges
+    sort_range(S, w.blk, w.nblk, lo, hi);
+    i0 = lo;
+    stride = kThreads;
+  }
+  for (uint32_t i = i0 + threadIdx.x; i < hi; i += stride) {
     const uint64_t e = a.shift_tmp[i];
     const uint32_t node = (uint32_t)e;
     uint32_t slot = (uint32_t)(e >> 32);
@@ -1640,6 +1653,12 @@ __device__ __forceinline__ void ph_shift_src(const Launch<ShiftArgs>& L, const W
   if (in_lds)
     for (uint32_t b = threadIdx.x; b < a.nbins; b += kThreads)
       if (h[b]) atomicAdd(&a.src_hist[b], h[b]);
+  if (a.hist0)  // this block's count of each first-pass digit (key mod nb0)
+    for (uint32_t d = threadIdx.x; d < a.nb0; d += kThreads) {
+      uint32_t c = 0;
+      for (uint32_t b = d; b < a.nbins; b += a.nb0) c += h[b];
+      a.hist0[d * w.nblk + w.blk] = c;
+    }
 }
 
 __global__ __launch_bounds__(kThreads) void k_shift_src(Launch<ShiftArgs> L) {
@@ -1652,11 +1671,6 @@ __global__ __launch_bounds__(kThreads) void k_shift_src(Launch<ShiftArgs> L) {
 // and after a scan of the digit-major (digit, block) counts moves its entries in
 // order.
 
-__device__ __forceinline__ void sort_range(uint32_t S, uint32_t g, uint32_t G, uint32_t& lo, uint32_t& hi) {
-  const uint32_t per = ((S + G - 1) / G + kThreads - 1) / kThreads * kThreads;
-  lo = min(S, g * per);
-  hi = min(S, lo + per);
-}
 
 __device__ __forceinline__ void ph_sort_count(const Launch<SortArgs>& L, const Where& w, uint8_t* lds) {
   uint32_t* h = reinterpret_cast<uint32_t*>(lds);
@@ -2179,6 +2193,13 @@ static void plan_tree(mtc_dedup* h, uint32_t fuse_req, uint32_t group) {
 static uint32_t sweep_b_group(const mtc_dedup* h) {
   if (h->debug) return 1;
   return h->tune.sweep_tiles_b ? h->tune.sweep_tiles_b : kTilesPerWaveB;
+}
+
+// Digit values of the sort's first pass for nbins source checkpoints.
+static uint32_t digits0(uint32_t nbins) {
+  uint32_t bits = 0;
+  while ((1ull << bits) < nbins) bits++;
+  return 1u << std::min<uint32_t>(8, bits);
 }
 
 static uint32_t sort_blocks(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(kSortBlocks, (n + 16 * kThreads - 1) / (16 * kThreads))); }
@@ -2748,8 +2769,13 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
     for (size_t i = 0; i < B; i++) {
       Member& M = ms[i];
       mtc_dedup* h = M.h;
+      // k_shift_src counts the first pass's digits per sort block itself when its
+      // histogram of sources is in LDS (fewer than kLdsBins checkpoints since the
+      // baseline): one launch fewer.
+      const bool fold = M.passes > 0 && M.nbins <= kLdsBins;
       sa.push_back(ShiftArgs{h->shift_tmp.p, &h->ctl_d->S, h->leaf_slot.p, M.n, h->table.p, h->table_cap - 1, h->baseline_id, M.cur,
-                             h->sort_key[0].p, h->sort_val[0].p, h->src_hist.p, M.nbins, h->ctl_d});
+                             h->sort_key[0].p, h->sort_val[0].p, h->src_hist.p, M.nbins, h->ctl_d,
+                             fold ? h->sort_hist.p : nullptr, fold ? digits0(M.nbins) : 0u});
       sb.push_back(M.Gs);
       max_passes = std::max(max_passes, M.passes);
       sorted[i] = h->sort_val[M.passes % 2].p;
@@ -2757,7 +2783,7 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
     ops.add<ShiftArgs>(60, img.add(sa, sb), [](Launch<ShiftArgs> L, uint32_t g, hipStream_t st) { k_shift_src<<<g, kThreads, 0, st>>>(L); });
     for (uint32_t p = 0; p < max_passes; p++) {
       std::vector<SortArgs> so;
-      std::vector<uint32_t> sblk;
+      std::vector<uint32_t> sblk, cblk;
       std::vector<ScanSeg> hs;
       for (auto& M : ms) {
         mtc_dedup* h = M.h;
@@ -2768,10 +2794,11 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
         so.push_back(SortArgs{h->sort_key[p % 2].p, h->sort_val[p % 2].p, h->sort_key[1 - p % 2].p,
                               h->sort_val[1 - p % 2].p, &h->ctl_d->S, shift, nb, h->sort_hist.p});
         sblk.push_back(p < M.passes ? M.Gs : 0u);
+        cblk.push_back(p < M.passes && !(p == 0 && M.nbins <= kLdsBins) ? M.Gs : 0u);  // else counted by k_shift_src
         if (p < M.passes) hs.push_back(ScanSeg{h->sort_hist.p, h->sort_hist.p, nullptr, nb * M.Gs, nullptr, M.ws_sort});
       }
-      const LaunchRec r = img.add(so, sblk);
-      ops.add<SortArgs>(61, r, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_count<<<g, kThreads, 0, st>>>(L); });
+      const LaunchRec r = img.add(so, sblk), rc = img.add(so, cblk);
+      ops.add<SortArgs>(61, rc, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_count<<<g, kThreads, 0, st>>>(L); });
       add_scan(ops, img, hs);
       ops.add<SortArgs>(62, r, [](Launch<SortArgs> L, uint32_t g, hipStream_t st) { k_sort_move<<<g, kThreads, 0, st>>>(L); });
     }

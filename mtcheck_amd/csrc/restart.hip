@@ -116,7 +116,7 @@ __global__ void k_r_cover_first(const uint8_t* first, uint32_t F, const uint32_t
   }
 }
 
-struct ShiftArgs {
+struct RShiftArgs {
   const uint8_t* shift;   // S (node, prev) pairs of this diff
   uint32_t S;
   const uint32_t* cum;    // inclusive running count by checkpoint id, ref..chkpt
@@ -137,7 +137,7 @@ struct ShiftArgs {
 // Shifted-duplicate entries of one diff: each leaf of `node` takes the data slot
 // of the matching leaf of `prev` inside the source diff's region
 // (src/tree_approach.cpp:887-925, 1058-1118).
-__global__ void k_r_cover_shift(ShiftArgs a) {
+__global__ void k_r_cover_shift(RShiftArgs a) {
   const uint32_t i = blockIdx.x * kRThreads + threadIdx.x;
   if (i >= a.S) return;
   const uint32_t node = ld32(a.shift + 8ull * i), prev = ld32(a.shift + 8ull * i + 4);
@@ -496,7 +496,7 @@ void restart_impl(const void* const* diffs, const uint64_t* sizes, uint32_t coun
       MTC_HIP(hipGetLastError());
     }
     if (S) {
-      ShiftArgs sa{d.base + d.off_shift, S, cum_p + cum_off[p], p + 1, ref, p, g, rs.set_keys.p, rs.set_vals.p,
+      RShiftArgs sa{d.base + d.off_shift, S, cum_p + cum_off[p], p + 1, ref, p, g, rs.set_keys.p, rs.set_vals.p,
                    set_cap - 1, meta_p, meta_p + L, region_p, cover_p, err_p};
       k_r_cover_shift<<<rgrid(S), kRThreads, 0, st>>>(sa);
       MTC_HIP(hipGetLastError());
@@ -728,7 +728,7 @@ void restart_noalloc_impl(const void* const* diffs, const uint64_t* sizes, uint3
       MTC_HIP(hipGetLastError());
     }
     if (S) {
-      ShiftArgs sa{b + off_shift[p], S, cum_p + meta_h[6 * L + p], p + 1, ref, p, g, keys_p, vals_p, set_cap - 1,
+      RShiftArgs sa{b + off_shift[p], S, cum_p + meta_h[6 * L + p], p + 1, ref, p, g, keys_p, vals_p, set_cap - 1,
                    meta_p, meta_p + L, region_p, cover_p, err_p};
       k_r_cover_shift<<<rgrid(S), kRThreads, 0, st>>>(sa);
       MTC_HIP(hipGetLastError());

@@ -310,7 +310,9 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       need(w, "shift_tmp", a.shift_tmp, 8);
     });
   } else if (has("9ShiftArgs")) {
-    check_members<ShiftArgs>(name, L, grid, [](const std::string& w, const ShiftArgs& a, uint32_t) {
+    check_members<ShiftArgs>(name, L, grid, [](const std::string& w, const ShiftArgs& a, uint32_t nblk) {
+      maybe(w, "hist0", a.hist0, 4ull * a.nb0 * nblk);  // the first sort pass's block counts
+      if (a.hist0 && (a.nb0 == 0 || a.nb0 > 256 || (a.nb0 & (a.nb0 - 1)))) violation(w + ": first-pass digits out of range");
       need(w, "shift_tmp", a.shift_tmp, 8);
       need(w, "S", a.S, 4);
       need(w, "leaf_slot", a.leaf_slot, 4ull * a.n);
