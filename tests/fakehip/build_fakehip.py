@@ -37,12 +37,13 @@ def build() -> str:
     if not os.path.exists(fake_o) or os.path.getmtime(fake_o) < max(os.path.getmtime(p) for p in deps):
         cmd = [B.HIPCC, "-x", "hip", f"--offload-arch={B.ARCH}", "-O1", "-std=c++17", "-fPIC", "-w",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-               "-I", os.path.join(ROOT, "include"), "-c", src, "-o", fake_o]
+               "-I", os.path.join(ROOT, "include"), "-c", src, "-o", fake_o + f".{os.getpid()}"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"fake_hip.cpp failed to compile:\n{r.stdout}\n{r.stderr}")
+        os.replace(fake_o + f".{os.getpid()}", fake_o)  # test processes may build at once
     if not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(p) for p in objs + [fake_o]):
-        tmp = LIB + ".tmp"
+        tmp = LIB + f".{os.getpid()}"
         cmd = ["g++", "-shared", "-Wl,-Bsymbolic", "-o", tmp, fake_o, *objs, "-lpthread"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
