@@ -78,6 +78,9 @@ struct LeafArgs {
   // k_insert moves it to leaf_dg.  Null: leaf_dg is written here.
   Digest* list_dg;
   unsigned long long* stamp;  // the wall clock when the member's first block starts
+  // 1 (tree members, k_leaf1): no digest is stored here; k_compact hashes the
+  // listed chunks again (they are few when few chunks change)
+  int hash_later;
   __device__ LeafArgs global() const {
     LeafArgs c = *this;
     MTC_GLOBAL(data); MTC_GLOBAL(leaf_dg); MTC_GLOBAL(labels); MTC_GLOBAL(list); MTC_GLOBAL(blk_cnt); MTC_GLOBAL(list_dg);
@@ -106,9 +109,16 @@ struct CompactArgs {
   uint32_t nblk;            // leaf blocks
   const Digest* list_dg;    // digests beside the list entries (tree members), or null
   Digest* dense_dg;         // their dense copy
+  // hash_chunks: dense_dg from the chunk bytes (Murmur3; 16-byte aligned data, C a
+  // multiple of 16) instead of list_dg
+  const uint8_t* data;
+  uint64_t len;
+  uint32_t C;
+  int hash_chunks;
   __device__ CompactArgs global() const {
     CompactArgs c = *this;
     MTC_GLOBAL(list); MTC_GLOBAL(blk_cnt); MTC_GLOBAL(blk_off); MTC_GLOBAL(dense); MTC_GLOBAL(list_dg); MTC_GLOBAL(dense_dg);
+    MTC_GLOBAL(data);
     return c;
   }
 };

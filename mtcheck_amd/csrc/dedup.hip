@@ -433,7 +433,7 @@ __device__ __forceinline__ void leaf_epilogue(const LeafArgs& a, uint64_t t, con
   if (ins) {
     const uint32_t k = base + __popcll(m & ((1ull << lane) - 1));
     region[k] = (uint32_t)c;
-    if (a.list_dg) a.list_dg[(region - a.list) + k] = d;
+    if (a.list_dg && !a.hash_later) a.list_dg[(region - a.list) + k] = d;
   }
   run += tot;
   __syncthreads();  // s_wcnt is reused by the next tile
@@ -648,7 +648,7 @@ __device__ __forceinline__ void leaf1_tile(const LeafArgs& a, uint32_t blk, uint
   if (ins) {
     const uint64_t k = t * kThreads + base + __popcll(m & ((1ull << lane) - 1));
     a.list[k] = (uint32_t)c;
-    if (a.list_dg) a.list_dg[k] = d;
+    if (a.list_dg && !a.hash_later) a.list_dg[k] = d;
   }
 }
 
@@ -684,6 +684,31 @@ __global__ __launch_bounds__(kThreads, 5) void k_leaf1_direct(LeafArgs a, Prolog
 }
 
 // Concatenate the per-block regions into the dense list of chunks to insert.
+
+__device__ __forceinline__ Digest chunk_m3_bytes(const uint8_t* p, uint64_t len) { return m3_bytes(p, len); }
+
+// Murmur3 digest of chunk c read straight from memory with 16-byte loads, eight in
+// flight (data 16-byte aligned, C a multiple of 16); a short last chunk byte by byte.
+__device__ __forceinline__ Digest chunk_m3(const uint8_t* data, uint64_t len, uint32_t C, uint32_t c) {
+  const uint64_t off = (uint64_t)c * C;
+  if (off + C > len) return chunk_m3_bytes(data + off, len - off);
+  const uint4* p = reinterpret_cast<const uint4*>(data + off);
+  uint64_t h1 = 0, h2 = 0;
+  const uint32_t nb = C / 16;
+  uint32_t q = 0;
+  for (; q + 8 <= nb; q += 8) {
+    uint4 w[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) w[k] = p[q + k];
+#pragma unroll
+    for (int k = 0; k < 8; k++) m3_block(h1, h2, (uint64_t)w[k].x | ((uint64_t)w[k].y << 32), (uint64_t)w[k].z | ((uint64_t)w[k].w << 32));
+  }
+  for (; q < nb; q++) {
+    const uint4 w = p[q];
+    m3_block(h1, h2, (uint64_t)w.x | ((uint64_t)w.y << 32), (uint64_t)w.z | ((uint64_t)w.w << 32));
+  }
+  return m3_final(h1, h2, C);
+}
 
 constexpr uint32_t kCompactPer = 32;  // leaf blocks per compaction block
 
@@ -723,12 +748,13 @@ __device__ __forceinline__ void ph_compact(const Launch<CompactArgs>& L, const W
       else hi = mid;
     }
     const uint64_t src = (uint64_t)(b0 + lo) * a.tiles_per_block * kThreads + (e - s_pre[lo]);
-    dst[e] = a.list[src];
-    if (a.dense_dg) a.dense_dg[off + e] = a.list_dg[src];
+    const uint32_t c = a.list[src];
+    dst[e] = c;
+    if (a.dense_dg) a.dense_dg[off + e] = a.hash_chunks ? chunk_m3(a.data, a.len, a.C, c) : a.list_dg[src];
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_compact(Launch<CompactArgs> L) {
+__global__ __launch_bounds__(kThreads, 8) void k_compact(Launch<CompactArgs> L) {
   __shared__ __align__(16) uint8_t lds[sizeof(uint32_t) * (kCompactPer + 2)];
   ph_compact(L, where(L), lds);
 }
@@ -2453,6 +2479,7 @@ struct Member {
   int cb = 0;
   bool split = false;  // the table bound is checked after the leaf kernel
   bool all_labels = true;  // the leaf kernel labels every chunk, not only the listed ones
+  bool hash_later = false;  // k_compact hashes the listed chunks; the leaf kernel stores no digest
   uint64_t blk_zero = 0;   // leading block counts known to be zero when the step starts
   Geo g{};
   uint32_t* ws_blk = nullptr;  // scan partial sums of this member
@@ -2461,6 +2488,19 @@ struct Member {
   uint32_t* ws_sort = nullptr;
   uint32_t* ws_reg = nullptr;
 };
+
+// A step hashes its listed chunks again in k_compact when the previous checkpoint
+// of the handle changed fewer than n / kHashLaterBelow chunks, on buffers of at
+// least kHashLaterMinChunks chunks (8 GiB of 128-byte chunks at 2% change: the
+// leaf kernel 45 us faster, k_compact 25 us slower; 64 MiB: 4 us slower).
+constexpr uint64_t kHashLaterBelow = 16;
+constexpr uint32_t kHashLaterMinChunks = 1u << 21;
+// MTC_HASH_LATER_MIN_CHUNKS=k replaces the size bound, so that tests can run the
+// mode on small buffers (results never depend on it).
+static uint64_t hash_later_min_chunks() {
+  const char* e = std::getenv("MTC_HASH_LATER_MIN_CHUNKS");
+  return e ? std::strtoull(e, nullptr, 10) : kHashLaterMinChunks;
+}
 
 // Recorded launches of a step: run eagerly or captured into a graph.  A tag names
 // each launch's kernel (sweep B: 44 and 45, sweep A's tails: 48 and 49, one chunk: 30, leaf
@@ -2530,6 +2570,13 @@ static void prepare_member(Member& M, hipStream_t st) {
   const uint64_t ntiles = (M.n + kThreads - 1) / kThreads;
   M.tpb = std::max<uint64_t>(1, h->tune.leaf_tiles_per_block);
   M.nblk_leaf = (uint32_t)((ntiles + M.tpb - 1) / M.tpb);
+  // The leaf kernel's digest stores cost it far more than their bytes (stores
+  // beside its read stream); when the previous checkpoint changed few chunks of a
+  // large buffer, k_compact hashes the listed chunks again instead (Murmur3,
+  // single-tile leaf).
+  M.hash_later = !M.baseline && h->approach != MTC_APPROACH_BASIC && h->hash_kind == HASH_MURMUR3 && M.cb >= 16 &&
+                 M.tpb == 1 && M.n >= hash_later_min_chunks() && h->ctl_h &&
+                 h->ctl_h->changed * kHashLaterBelow <= M.n;
   // Block counts that k_compact cleared at the end of the last completed step (the
   // same buffer); unknown again until this step completes.
   M.blk_zero = h->blk_zero_p == h->blk_cnt.p ? h->blk_zero : 0;
@@ -2594,7 +2641,7 @@ static void build_tree_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img) {
     for (auto& M : ms) {
       mtc_dedup* h = M.h;
       ca.push_back(CompactArgs{h->list.p, h->blk_cnt.p, h->blk_off.p, M.tpb, h->dense.p, M.nblk_leaf, h->list_dg.p,
-                               h->dense_dg.p});
+                               h->dense_dg.p, M.data, M.len, h->chunk_size, M.hash_later ? 1 : 0});
       cb.push_back((M.nblk_leaf + kCompactPer - 1) / kCompactPer);
       da.push_back(DenseArgs{h->dense.p, h->blk_off.p + M.nblk_leaf, M.n, h->leaf_dg.p, h->leaf_slot.p, h->labels.p,
                              h->table.p, h->table_cap - 1, M.cur, h->ctl_d, h->ctr.p, M.g, M.R0, h->tile_flags.p,
@@ -2964,7 +3011,8 @@ static void build_ops(std::vector<Member>& ms, Ops& ops, ArenaImage& img, int ph
         const int reset = !M.baseline ? 0 : (h->approach == MTC_APPROACH_BASIC && M.cur == 0) ? 2 : 1;
         la.push_back(LeafArgs{M.data, M.len, M.n, h->chunk_size, h->leaf_dg.p, h->labels.p, reset,
                               h->list.p, h->blk_cnt.p, M.tpb, M.all_labels ? 1 : 0,
-                              h->approach == MTC_APPROACH_BASIC ? nullptr : h->list_dg.p, &h->ctl_d->t_begin});
+                              h->approach == MTC_APPROACH_BASIC ? nullptr : h->list_dg.p, &h->ctl_d->t_begin,
+                              M.hash_later ? 1 : 0});
         lb.push_back(M.nblk_leaf);
       }
       const int hk = std::get<0>(kv.first), cb = std::get<1>(kv.first);

@@ -58,6 +58,8 @@ struct State {
   std::vector<std::string> launches;  // kernel names in launch order
   std::vector<int> leaf_all_labels;   // per leaf member launched: its all_labels flag
   std::vector<int> label_resets;      // per k_assemble member launched: reset_labels is set
+  std::vector<int> leaf_hash_later;   // per leaf member launched: it stores no digest
+  std::vector<int> compact_hash;      // per k_compact member launched: it hashes the listed chunks
   // Words of a buffer known to hold zero, from its start (fills; k_compact clears
   // the leaf block counts it reads; a leaf launch makes its counts unknown).
   std::map<const void*, uint64_t> zero_upto;
@@ -223,6 +225,7 @@ void check_leaf(const std::string& w, const LeafArgs& a, uint32_t nblk, bool add
               std::to_string(nblk) + ")");
   z = 0;
   S().leaf_all_labels.push_back(a.all_labels);
+  S().leaf_hash_later.push_back(a.hash_later);
 }
 
 void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
@@ -259,6 +262,12 @@ void check_launch(const std::string& name, const uint8_t* arg0, uint32_t grid) {
       if (!a.list_dg != !a.dense_dg) violation(w + ": list_dg and dense_dg come together");
       maybe(w, "list_dg", a.list_dg, 16);
       maybe(w, "dense_dg", a.dense_dg, 16);
+      if (a.hash_chunks) {  // digests from the chunk bytes, 16 bytes at a time
+        need(w, "data", a.data, a.len);
+        if (!a.dense_dg) violation(w + ": chunks hashed with nowhere to put the digests");
+        if (a.C == 0 || a.C % 16 || reinterpret_cast<uintptr_t>(a.data) % 16) violation(w + ": chunks hashed with unaligned loads");
+      }
+      S().compact_hash.push_back(a.hash_chunks);
     });
   } else if (has("9DenseArgs")) {
     check_members<DenseArgs>(name, L, grid, [](const std::string& w, const DenseArgs& a, uint32_t) {
@@ -438,6 +447,14 @@ int fakehip_label_resets(int i) {
   std::lock_guard<std::mutex> g(S().mu);
   return (i >= 0 && i < (int)S().label_resets.size()) ? S().label_resets[i] : -1;
 }
+int fakehip_leaf_hash_later(int i) {
+  std::lock_guard<std::mutex> g(S().mu);
+  return (i >= 0 && i < (int)S().leaf_hash_later.size()) ? S().leaf_hash_later[i] : -1;
+}
+int fakehip_compact_hash(int i) {
+  std::lock_guard<std::mutex> g(S().mu);
+  return (i >= 0 && i < (int)S().compact_hash.size()) ? S().compact_hash[i] : -1;
+}
 const char* fakehip_launch_name(int i) {
   std::lock_guard<std::mutex> g(S().mu);
   return (i >= 0 && i < (int)S().launches.size()) ? S().launches[i].c_str() : "";
@@ -448,6 +465,8 @@ void fakehip_reset() {
   S().launches.clear();
   S().leaf_all_labels.clear();
   S().label_resets.clear();
+  S().leaf_hash_later.clear();
+  S().compact_hash.clear();
 }
 uint64_t fakehip_bytes_alive() {
   std::lock_guard<std::mutex> g(S().mu);

@@ -185,6 +185,49 @@ def test_leaf_labels_written_for_listed_chunks_once_reset(fake, monkeypatch):
     L.mtc_destroy(b)
 
 
+def test_listed_digests_stored_or_hashed_again(fake):
+    """An incremental step of a buffer of at least 2^21 chunks after one that changed
+    few chunks stores no digest in the leaf kernel and hashes the listed chunks in
+    k_compact instead (Murmur3, one tile per leaf block, aligned data); a baseline,
+    a smaller buffer, MD5, several tiles per leaf block and unstaged chunk sizes
+    store them.  (No kernel runs here, so the previous step's changed count reads
+    zero.)"""
+    L = fake
+    big = 1 << 21
+
+    def flags(h, size, baseline):
+        L.fakehip_reset()
+        data = dev_alloc(L, size)
+        rc, _ = checkpoint(L, h, data, size, baseline=baseline)
+        assert rc == OK, L.mtc_last_error(h)
+        assert violations(L) == []
+        leaf, comp = [], []
+        while L.fakehip_leaf_hash_later(len(leaf)) >= 0:
+            leaf.append(L.fakehip_leaf_hash_later(len(leaf)))
+        while L.fakehip_compact_hash(len(comp)) >= 0:
+            comp.append(L.fakehip_compact_hash(len(comp)))
+        return leaf, comp
+
+    h = create(L, 128)
+    assert flags(h, 128 * big, True) == ([0], [0])
+    assert flags(h, 128 * big, False) == ([1], [1])
+    assert flags(h, 128 * 3000, True) == ([0], [0])
+    assert flags(h, 128 * 3000, False) == ([0], [0])
+    assert L.mtc_set_tuning(h, 4, ctypes.c_float(0.125), 8) == OK
+    assert flags(h, 128 * big, True) == ([0], [0])
+    assert flags(h, 128 * big, False) == ([0], [0])
+    L.mtc_destroy(h)
+    for C, hk in [(128, 1), (100, 0)]:
+        h = create(L, C, hk=hk)
+        assert flags(h, C * big + 7, True) == ([0], [0])
+        assert flags(h, C * big + 7, False) == ([0], [0])
+        L.mtc_destroy(h)
+    h = create(L, 128)
+    assert flags(h, 128 * big + 5, True) == ([0], [0])
+    assert flags(h, 128 * big + 5, False) == ([1], [1])  # a short last chunk: hashed byte by byte
+    L.mtc_destroy(h)
+
+
 def test_batch_of_tree_and_basic_members_with_graph_replay(fake):
     L = fake
     L.fakehip_reset()
@@ -300,7 +343,8 @@ class LeafArgs(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("len", ctypes.c_uint64), ("n", ctypes.c_uint32), ("C", ctypes.c_uint32),
                 ("leaf_dg", ctypes.c_void_p), ("labels", ctypes.c_void_p), ("baseline", ctypes.c_int),
                 ("list", ctypes.c_void_p), ("blk_cnt", ctypes.c_void_p), ("tiles_per_block", ctypes.c_uint64),
-                ("all_labels", ctypes.c_int), ("list_dg", ctypes.c_void_p), ("stamp", ctypes.c_void_p)]
+                ("all_labels", ctypes.c_int), ("list_dg", ctypes.c_void_p), ("stamp", ctypes.c_void_p),
+                ("hash_later", ctypes.c_int)]
 
 
 class FillSeg(ctypes.Structure):
@@ -319,7 +363,7 @@ def test_checker_flags_leaf_counts_not_known_zero(fake):
     L.fakehip_reset()
     n = 1000
     a = LeafArgs(dev_alloc(L, 128 * n), 128 * n, n, 128, dev_alloc(L, 16 * n), dev_alloc(L, n), 0,
-                 dev_alloc(L, 4 * n), dev_alloc(L, 4 * 4), 1, 0, dev_alloc(L, 16 * n), None)
+                 dev_alloc(L, 4 * n), dev_alloc(L, 4 * 4), 1, 0, dev_alloc(L, 16 * n), None, 0)
     stub = getattr(L, "_ZN3mtc29__device_stub__k_leaf1_directILi0ELi128EEEvNS_8LeafArgsENS_8PrologueE")
     L.__hipPushCallConfiguration(Dim3(4, 1, 1), Dim3(256, 1, 1), ctypes.c_size_t(0), None)
     stub(a, Prologue())
