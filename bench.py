@@ -561,7 +561,7 @@ def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, wa
     for k in range(warmup):
         d.checkpoint_device(vs.get(k), make_baseline=(k == 0))
     torch.cuda.synchronize()
-    lat, leaf_ms, changed, balg, dsize = [], [], [], [], []
+    lat, leaf_ms, changed, balg, dsize, leaf_bytes = [], [], [], [], [], []
     for k in range(warmup, total):
         buf = vs.get(k)
         torch.cuda.synchronize()
@@ -571,13 +571,15 @@ def plain_run(nbytes: int, C: int, hash_name: str, change: float, steps: int, wa
         st = d.stats()
         leaf_ms.append(st.ms_leaf)
         changed.append(st.changed_chunks)
+        # Leaf kernel: C B read and the 16 B previous digest per chunk; per changed
+        # chunk a 4 B list entry and, unless the compaction hashes it again, its
+        # 16 B digest; a label byte per chunk only when it labels every chunk.
+        leaf_bytes.append(nbytes + 16 * n + (n if st.leaf_labels_written else 0)
+                          + (20 if st.leaf_digests_stored else 4) * st.changed_chunks)
         dsize.append(size)
         meta = 32 + 4 * hdr.num_first_ocur + 8 * hdr.num_prior_chkpts + 8 * hdr.num_shift_dupl
         balg.append(nbytes + 32 * n + 2 * (size - meta) + meta)
     elapsed = sum(lat)
-    # Leaf kernel: C B read + 16 B previous digest read + 1 B label per chunk,
-    # 16 B digest + 4 B slot written per changed chunk.
-    leaf_bytes = [nbytes + 17 * n + 20 * c for c in changed]
     leaf_avg_ms = float(np.mean(leaf_ms))
     achieved = float(np.mean(leaf_bytes)) / (leaf_avg_ms / 1e3)
     d.close()

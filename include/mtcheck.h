@@ -81,6 +81,12 @@ typedef struct mtc_stats {
   float ms_setup, ms_dedup, ms_collect, ms_copy; /* timers[0..3] */
   uint64_t changed_chunks; /* leaves that are not fixed duplicates */
   float ms_leaf;           /* the leaf (hash + label + insert) kernel alone */
+  /* What the last leaf kernel wrote besides its list of changed chunks (4 bytes
+   * each): a label byte for every chunk (a baseline, or after a debug or failed
+   * step), and the changed chunks' digests (16 bytes each; otherwise the
+   * compaction hashes those chunks again). */
+  uint32_t leaf_labels_written;
+  uint32_t leaf_digests_stored;
 } mtc_stats;
 
 /* TreeDeduplicator(uint32_t bytes_per_chunk), src/tree_approach.cpp:5-9, plus the

@@ -52,7 +52,8 @@ class Stats(ctypes.Structure):
                 ("data_len", ctypes.c_uint64), ("diff_size", ctypes.c_uint64), ("data_chunks", ctypes.c_uint64),
                 ("table_entries", ctypes.c_uint64), ("table_capacity", ctypes.c_uint64),
                 ("ms_setup", ctypes.c_float), ("ms_dedup", ctypes.c_float), ("ms_collect", ctypes.c_float),
-                ("ms_copy", ctypes.c_float), ("changed_chunks", ctypes.c_uint64), ("ms_leaf", ctypes.c_float)]
+                ("ms_copy", ctypes.c_float), ("changed_chunks", ctypes.c_uint64), ("ms_leaf", ctypes.c_float),
+                ("leaf_labels_written", ctypes.c_uint32), ("leaf_digests_stored", ctypes.c_uint32)]
 
 
 _lib = None

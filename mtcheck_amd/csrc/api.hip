@@ -213,6 +213,8 @@ int mtc_get_stats(const mtc_dedup* h, mtc_stats* out) {
   out->ms_copy = h->last_ms[3];
   out->changed_chunks = h->ctl_h ? h->ctl_h->changed : 0;
   out->ms_leaf = h->last_ms[4];
+  out->leaf_labels_written = h->last_leaf_labels;
+  out->leaf_digests_stored = h->last_leaf_digests;
   return MTC_OK;
 }
 

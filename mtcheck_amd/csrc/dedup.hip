@@ -3257,6 +3257,8 @@ static void run_step_body(StepCtx& ctx, std::vector<Member>& ms, hipStream_t st,
     h->last_ms[2] = ms_total - ms_forest;
     h->last_ms[3] = 0;
     h->last_ms[4] = ms_leaf;
+    h->last_leaf_labels = ms[i].all_labels ? 1 : 0;
+    h->last_leaf_digests = ms[i].hash_later ? 0 : 1;
   }
   ht.print(B);
 }

@@ -227,5 +227,7 @@ struct mtc_dedup {
   mtc_header last_header{};
   uint64_t last_diff_size = 0;
   float last_ms[5] = {0, 0, 0, 0, 0};  // setup, dedup, collect, copy, leaf kernel
+  uint32_t last_leaf_labels = 0;       // the last leaf kernel labelled every chunk
+  uint32_t last_leaf_digests = 0;      // the last leaf kernel stored the listed chunks' digests
   std::string last_error;
 };
