@@ -2490,16 +2490,20 @@ struct Member {
 };
 
 // A step hashes its listed chunks again in k_compact when the previous checkpoint
-// of the handle changed fewer than n / kHashLaterBelow chunks, on buffers of at
-// least kHashLaterMinChunks chunks (8 GiB of 128-byte chunks at 2% change: the
-// leaf kernel 45 us faster, k_compact 25 us slower; 64 MiB: 4 us slower).
+// of the handle changed fewer than n / kHashLaterBelow chunks, for 128-byte chunks
+// and buffers of at least kHashLaterMinChunks chunks.  Measured per checkpoint at
+// 2% change: 8 GiB of 128-byte chunks 20 us faster (the leaf kernel 45 us faster,
+// k_compact 25 us slower); 64 MiB 4 us slower; 64-byte chunks no faster; 4096-byte
+// chunks 70 us slower (one lane reads a whole chunk).
 constexpr uint64_t kHashLaterBelow = 16;
 constexpr uint32_t kHashLaterMinChunks = 1u << 21;
-// MTC_HASH_LATER_MIN_CHUNKS=k replaces the size bound, so that tests can run the
-// mode on small buffers (results never depend on it).
-static uint64_t hash_later_min_chunks() {
+constexpr uint32_t kHashLaterChunk = 128;
+// MTC_HASH_LATER_MIN_CHUNKS=k replaces both bounds (k chunks, any staged chunk
+// size), so that tests can run the mode on any buffer (results never depend on it).
+static bool hash_later_fits(uint64_t n, uint32_t C) {
   const char* e = std::getenv("MTC_HASH_LATER_MIN_CHUNKS");
-  return e ? std::strtoull(e, nullptr, 10) : kHashLaterMinChunks;
+  if (e) return n >= std::strtoull(e, nullptr, 10);
+  return C == kHashLaterChunk && n >= kHashLaterMinChunks;
 }
 
 // Recorded launches of a step: run eagerly or captured into a graph.  A tag names
@@ -2575,7 +2579,7 @@ static void prepare_member(Member& M, hipStream_t st) {
   // large buffer, k_compact hashes the listed chunks again instead (Murmur3,
   // single-tile leaf).
   M.hash_later = !M.baseline && h->approach != MTC_APPROACH_BASIC && h->hash_kind == HASH_MURMUR3 && M.cb >= 16 &&
-                 M.tpb == 1 && M.n >= hash_later_min_chunks() && h->ctl_h &&
+                 M.tpb == 1 && hash_later_fits(M.n, h->chunk_size) && h->ctl_h &&
                  h->ctl_h->changed * kHashLaterBelow <= M.n;
   // Block counts that k_compact cleared at the end of the last completed step (the
   // same buffer); unknown again until this step completes.

@@ -1,9 +1,9 @@
 """Steps whose leaf kernel stores no digest, with k_compact hashing the listed
-chunks again from their bytes (the mode a large buffer takes after a checkpoint
-that changed few chunks; forced here on every size through
-MTC_HASH_LATER_MIN_CHUNKS=0), write the oracle's diffs byte for byte: chunk sizes
-from 16 to 4096 bytes, chunk counts that are not powers of two and short last
-chunks."""
+chunks again from their bytes (the mode a large buffer of 128-byte chunks takes
+after a checkpoint that changed few chunks; forced here for every buffer and chunk
+size through MTC_HASH_LATER_MIN_CHUNKS=0), write the oracle's diffs byte for byte:
+chunk sizes from 16 to 4096 bytes, chunk counts that are not powers of two and
+short last chunks."""
 import numpy as np
 import pytest
 

@@ -186,12 +186,12 @@ def test_leaf_labels_written_for_listed_chunks_once_reset(fake, monkeypatch):
 
 
 def test_listed_digests_stored_or_hashed_again(fake):
-    """An incremental step of a buffer of at least 2^21 chunks after one that changed
-    few chunks stores no digest in the leaf kernel and hashes the listed chunks in
-    k_compact instead (Murmur3, one tile per leaf block, aligned data); a baseline,
-    a smaller buffer, MD5, several tiles per leaf block and unstaged chunk sizes
-    store them.  (No kernel runs here, so the previous step's changed count reads
-    zero.)"""
+    """An incremental step of a buffer of at least 2^21 chunks of 128 bytes after one
+    that changed few chunks stores no digest in the leaf kernel and hashes the
+    listed chunks in k_compact instead (Murmur3, one tile per leaf block, aligned
+    data); a baseline, a smaller buffer, other chunk sizes, MD5 and several tiles
+    per leaf block store them.  (No kernel runs here, so the previous step's changed
+    count reads zero.)"""
     L = fake
     big = 1 << 21
 
@@ -217,7 +217,7 @@ def test_listed_digests_stored_or_hashed_again(fake):
     assert flags(h, 128 * big, True) == ([0], [0])
     assert flags(h, 128 * big, False) == ([0], [0])
     L.mtc_destroy(h)
-    for C, hk in [(128, 1), (100, 0)]:
+    for C, hk in [(128, 1), (100, 0), (64, 0), (4096, 0)]:
         h = create(L, C, hk=hk)
         assert flags(h, C * big + 7, True) == ([0], [0])
         assert flags(h, C * big + 7, False) == ([0], [0])
